@@ -1,0 +1,629 @@
+"""FP64 CPU oracle of the primal-dual iLQR KKT hot path.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product imports this module; only
+``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` use it, as the checker.
+
+What it restates.  The reference calls one SQP iteration per
+``optimizers.mpc(cost, dynamics, hessian_approx, limited_memory, reference,
+parameter, W, x0, X_in, U_in, V_in) -> (X, U, V)``
+(mpx/utils/mpc_wrapper.py:74, :150-158, :244; mpx/examples/acrobat.py:101-113)
+and evaluates ``optimizers.model_evaluator_helper(cost, dynamics, x0, X, U) ->
+(g, c)`` (mpx/utils/mpc_wrapper.py:422, :452-460).  The solver itself lives in
+the un-vendored submodule ``primal_dual_ilqr`` (.gitmodules:1-3, commit pin
+unknown), so the arithmetic below follows the published algorithm it is built
+on -- Sousa-Pinto & Orban, "Primal-Dual iLQR" (2024) for the SQP step and
+merit, and Saerkkae & Garcia-Fernandez, "Temporal parallelization of dynamic
+programming and linear quadratic control" (IEEE TAC 2023) for the associative
+scan -- with these fixed decisions:
+
+D1  V_t is the multiplier of the constraint c_t in L = f + sum V^T c, with
+    c_init = x0 - X_0 paired with V_0 and c_t = f(x_t,u_t) - x_{t+1} paired with
+    V_{t+1}; V is then the gradient of the cost-to-go (V_t = q_t + A_t^T V_{t+1}).
+D2  No Hessian regularisation; Gauss-Newton Hessians of the quadratic tracking
+    cost are exact.
+D3  Merit penalty rho = clip(max(rho0, 2 (d_f + d_c + kappa/2) / |c|^2), rho_max)
+    with d_f = grad^T dz, d_c = (dV - V)^T c, kappa = dz^T H dz; this gives
+    phi'(0) <= -rho |c|^2 / 2 - kappa / 2 < 0.
+D4  Armijo backtracking over alpha = alpha_max beta^j, j < max_trials (alpha_max = 1,
+    or the primal fraction-to-boundary step when boxes are present), accepting the first
+    phi(alpha) <= phi(0) + c1 alpha phi'(0) + slack, slack = armijo_slack (1+|phi(0)|);
+    if none passes the smallest trial is taken and status = 2.  limited_memory
+    only changes how many trials are evaluated at once, never the chosen alpha.
+D5  Trial points are the linear multiple-shooting step (X + a dX, U + a dU,
+    V + a dV); defects are re-evaluated with the nonlinear dynamics.
+D6  Equality rows D x + E u + e = 0 enter through an augmented Lagrangian with
+    fixed penalty rho_c: Hessian += rho_c G^T G, gradient += G^T (mu + rho_c h),
+    and after the step mu+ = mu + rho_c h(z+).  Boxes u_lo <= u <= u_hi are
+    handled primal-dual interior-point style with a fixed barrier mu_b: slacks
+    s_hi = u_hi - u, s_lo = u - u_lo, multipliers nu > 0 (nu <= 0 on entry is
+    replaced by mu_b / s); the LQR gets R += diag(nu_hi/s_hi + nu_lo/s_lo) and
+    r += mu_b/s_hi - mu_b/s_lo; the dual step is
+    dnu_hi = mu_b/s_hi - nu_hi + nu_hi du / s_hi,
+    dnu_lo = mu_b/s_lo - nu_lo - nu_lo du / s_lo;
+    primal and dual steps are capped by fraction-to-boundary tau, and the merit
+    carries the barrier -mu_b sum log s.  A U outside the box on entry is first
+    pulled inside by clipping to [lo + 1e-3 w, hi - 1e-3 w], w = hi - lo.
+D7  The scan combine uses (I + C J)^{-1}; a leaf element combined with anything
+    uses the rank-m Woodbury form through H = R + B^T J B.
+D8  KKT residual = 2-norm of (grad_x L, grad_u L, c, h, nu * s - mu_b) (the
+    barrier-perturbed complementarity for boxes); the relative residual is that
+    norm / max(1, norm at the first iterate).
+D9  Workload ambiguities: see ``mpx_amd.synthetic``.
+D10 Warm-start shift copies mpx/utils/mpc_wrapper.py:81-83 (drop ``shift`` rows,
+    repeat the last row).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+DEFAULT_PARAMS = dict(
+    rho0=1.0,          # merit penalty floor
+    rho_max=1e8,       # merit penalty cap
+    rho_c=10.0,        # augmented-Lagrangian penalty of equality rows
+    mu_b=1e-2,         # box barrier parameter
+    tau=0.995,         # fraction-to-boundary factor
+    beta=0.5,          # backtracking factor
+    c1=1e-4,           # Armijo constant
+    max_trials=10,     # line-search trials
+    armijo_slack=1e-6, # relative noise floor of the Armijo test
+)
+
+
+# ---------------------------------------------------------------------------
+# model: dynamics, Jacobians, cost
+# ---------------------------------------------------------------------------
+
+def dynamics(inst, x, u, t):
+    """x_{t+1} = f(x_t, u_t, t); signature of mpx/utils/models.py:6 (x, u, t)."""
+    if inst["model"] == "ltv":
+        return inst["A"][t] @ x + inst["B"][t] @ u
+    return x + inst["h"] * (inst["A"] @ x + inst["B"] @ u
+                            + inst["kappa"] * np.tanh(inst["W"] @ x))
+
+
+def dynamics_jac(inst, x, u, t):
+    """(f_x, f_u).  tanh model: f_x = I + h (A + kappa diag(1 - tanh^2(Wx)) W),
+    f_u = h B."""
+    if inst["model"] == "ltv":
+        return inst["A"][t], inst["B"][t]
+    n = inst["n"]
+    s = np.tanh(inst["W"] @ x)
+    fx = np.eye(n) + inst["h"] * (inst["A"] + inst["kappa"] * (1.0 - s * s)[:, None] * inst["W"])
+    fu = inst["h"] * inst["B"]
+    return fx, fu
+
+
+def Q_at(inst, t):
+    Q = inst["Q"]
+    return Q[0] if Q.shape[0] == 1 else Q[t]
+
+
+def R_at(inst, t):
+    R = inst["R"]
+    return R[0] if R.shape[0] == 1 else R[t]
+
+
+def stage_cost(inst, x, u, t):
+    """Quadratic tracking cost; terminal form at t == N (mpx/utils/objectives.py:37)."""
+    N = inst["N"]
+    dx = x - inst["xref"][t]
+    if t == N:
+        return 0.5 * dx @ inst["QN"] @ dx
+    du = u - inst["uref"][t]
+    return 0.5 * dx @ Q_at(inst, t) @ dx + 0.5 * du @ R_at(inst, t) @ du
+
+
+def has_box(inst):
+    return inst.get("ulo") is not None
+
+
+def eq_rows(inst, x, u, t):
+    return inst["D"][t] @ x + inst["E"][t] @ u + inst["e"][t]
+
+
+def box_g(inst, u):
+    """Box constraint values g <= 0: [u - u_hi, u_lo - u]."""
+    return np.concatenate([u - inst["uhi"], inst["ulo"] - u])
+
+
+def al_cost(inst, x, u, t, yeq, rho_c, mu_b):
+    """Stage cost plus the augmented-Lagrangian terms of the equality rows and the
+    box barrier -mu_b sum log s (s = -g > 0)."""
+    val = stage_cost(inst, x, u, t)
+    if t == inst["N"]:
+        return val
+    if inst["p"] > 0:
+        h = eq_rows(inst, x, u, t)
+        val += yeq @ h + 0.5 * rho_c * h @ h
+    if has_box(inst):
+        s = -box_g(inst, u)
+        val -= mu_b * np.sum(np.log(s))
+    return val
+
+
+def box_interior(inst, U):
+    """U pulled strictly inside the box when it is not (D6)."""
+    if not has_box(inst):
+        return U
+    lo, hi = inst["ulo"], inst["uhi"]
+    if np.all(U < hi) and np.all(U > lo):
+        return U
+    w = 1e-3 * (hi - lo)
+    return np.clip(U, lo + w, hi - w)
+
+
+def box_duals(inst, U, Ybox, mu_b):
+    """Slacks (N, 2m) and positive box multipliers (N, 2m) (nu <= 0 -> mu_b / s)."""
+    S = -np.array([box_g(inst, u) for u in U])
+    Y = np.where(Ybox > 0.0, Ybox, mu_b / S)
+    return S, Y
+
+
+def constraints(inst, X, U):
+    """Constraint rows (N+1, n): [x0 - X_0, f(x_t, u_t) - x_{t+1}]."""
+    N = inst["N"]
+    c = np.zeros((N + 1, inst["n"]))
+    c[0] = inst["x0"] - X[0]
+    for t in range(N):
+        c[t + 1] = dynamics(inst, X[t], U[t], t) - X[t + 1]
+    return c
+
+
+def model_evaluator_helper(inst, X, U):
+    """(g, c): per-stage costs (N+1,) and constraint rows (N+1, n).
+
+    Mirrors ``optimizers.model_evaluator_helper(cost, dynamics, x0, X, U)``
+    used at mpx/utils/mpc_wrapper.py:422 and printed as sum g^2, sum c^2 at :456-460.
+    """
+    N = inst["N"]
+    Upad = np.vstack([U, np.zeros((1, inst["m"]))])
+    g = np.array([stage_cost(inst, X[t], Upad[t], t) for t in range(N + 1)])
+    return g, constraints(inst, X, U)
+
+
+# ---------------------------------------------------------------------------
+# linearisation + augmented-Lagrangian quadratisation  -> LQR data
+# ---------------------------------------------------------------------------
+
+def linearize(inst, X, U, Yeq, Ybox, rho_c, mu_b=1e-2):
+    """LQR subproblem data at (X, U) with the AL terms folded in (D6).
+
+    Returns dict: A (N,n,n), B (N,n,m), Q (N,n,n), R (N,m,m), M (N,n,m),
+    q (N,n), r (N,m), c (N,n) dynamics defects, QN, qN, c0 = x0 - X_0.
+    """
+    N, n, m, p = inst["N"], inst["n"], inst["m"], inst["p"]
+    L = dict(A=np.zeros((N, n, n)), B=np.zeros((N, n, m)), Q=np.zeros((N, n, n)),
+             R=np.zeros((N, m, m)), M=np.zeros((N, n, m)), q=np.zeros((N, n)),
+             r=np.zeros((N, m)), c=np.zeros((N, n)))
+    for t in range(N):
+        x, u = X[t], U[t]
+        fx, fu = dynamics_jac(inst, x, u, t)
+        L["A"][t], L["B"][t] = fx, fu
+        L["c"][t] = dynamics(inst, x, u, t) - X[t + 1]
+        Qt, Rt = Q_at(inst, t), R_at(inst, t)
+        L["Q"][t] = Qt.copy()
+        L["R"][t] = Rt.copy()
+        L["q"][t] = Qt @ (x - inst["xref"][t])
+        L["r"][t] = Rt @ (u - inst["uref"][t])
+        if p > 0:
+            D, E = inst["D"][t], inst["E"][t]
+            h = eq_rows(inst, x, u, t)
+            w = Yeq[t] + rho_c * h
+            L["Q"][t] += rho_c * D.T @ D
+            L["R"][t] += rho_c * E.T @ E
+            L["M"][t] += rho_c * D.T @ E
+            L["q"][t] += D.T @ w
+            L["r"][t] += E.T @ w
+        if has_box(inst):
+            # Ybox holds positive multipliers here (see box_duals)
+            s = -box_g(inst, u)
+            L["r"][t] += mu_b / s[:m] - mu_b / s[m:]
+            L["R"][t] += np.diag(Ybox[t][:m] / s[:m] + Ybox[t][m:] / s[m:])
+    L["QN"] = inst["QN"].copy()
+    L["qN"] = inst["QN"] @ (X[N] - inst["xref"][N])
+    L["c0"] = inst["x0"] - X[0]
+    return L
+
+
+# ---------------------------------------------------------------------------
+# LQR solvers: dense KKT, sequential Riccati, associative scan
+# ---------------------------------------------------------------------------
+
+def lqr_dense(L):
+    """Brute-force solve of the full LQR KKT system (for tiny problems).
+
+    Variables z = [dx_0..dx_N, du_0..du_{N-1}], constraints E z = d with rows
+    dx_0 = c0 and dx_{t+1} - A dx_t - B du_t = c_t; stationarity H z + g - E^T lam = 0.
+    """
+    A, B = L["A"], L["B"]
+    N, n, m = B.shape
+    nx, nu = (N + 1) * n, N * m
+    nz = nx + nu
+    H = np.zeros((nz, nz))
+    g = np.zeros(nz)
+    for t in range(N):
+        xi, ui = slice(t * n, (t + 1) * n), slice(nx + t * m, nx + (t + 1) * m)
+        H[xi, xi] = L["Q"][t]
+        H[ui, ui] = L["R"][t]
+        H[xi, ui] = L["M"][t]
+        H[ui, xi] = L["M"][t].T
+        g[xi] = L["q"][t]
+        g[ui] = L["r"][t]
+    xN = slice(N * n, (N + 1) * n)
+    H[xN, xN] = L["QN"]
+    g[xN] = L["qN"]
+    E = np.zeros((nx, nz))
+    d = np.zeros(nx)
+    E[0:n, 0:n] = np.eye(n)
+    d[0:n] = L["c0"]
+    for t in range(N):
+        rows = slice((t + 1) * n, (t + 2) * n)
+        E[rows, (t + 1) * n:(t + 2) * n] = np.eye(n)
+        E[rows, t * n:(t + 1) * n] = -A[t]
+        E[rows, nx + t * m:nx + (t + 1) * m] = -B[t]
+        d[rows] = L["c"][t]
+    K = np.block([[H, -E.T], [E, np.zeros((nx, nx))]])
+    sol = np.linalg.solve(K, np.concatenate([-g, d]))
+    z, lam = sol[:nz], sol[nz:]
+    return dict(dx=z[:nx].reshape(N + 1, n), du=z[nx:].reshape(N, m),
+                lam=lam.reshape(N + 1, n))
+
+
+def riccati_step(A, B, Q, R, M, q, r, c, S, s):
+    """One backward Riccati step with cross term M (value V(x) = x'Sx/2 + s'x).
+
+    Returns (S_t, s_t, K_t, k_t) with du = -K dx - k.
+    """
+    Sc = S @ c + s
+    H = R + B.T @ S @ B
+    G = M.T + B.T @ S @ A
+    gu = r + B.T @ Sc
+    Hc = np.linalg.cholesky(H)
+    K = np.linalg.solve(Hc.T, np.linalg.solve(Hc, G))
+    k = np.linalg.solve(Hc.T, np.linalg.solve(Hc, gu))
+    S_t = Q + A.T @ S @ A - G.T @ K
+    S_t = 0.5 * (S_t + S_t.T)
+    s_t = q + A.T @ Sc - G.T @ k
+    return S_t, s_t, K, k
+
+
+def _forward(L, K, k, S, s):
+    A, B = L["A"], L["B"]
+    N, n, m = B.shape
+    dx = np.zeros((N + 1, n))
+    du = np.zeros((N, m))
+    dx[0] = L["c0"]
+    for t in range(N):
+        du[t] = -K[t] @ dx[t] - k[t]
+        dx[t + 1] = A[t] @ dx[t] + B[t] @ du[t] + L["c"][t]
+    lam = np.einsum("tij,tj->ti", S, dx) + s
+    return dx, du, lam
+
+
+def lqr_riccati(L):
+    """Sequential Riccati recursion, then forward rollout; costates lam = S dx + s."""
+    A, B = L["A"], L["B"]
+    N, n, m = B.shape
+    S = np.zeros((N + 1, n, n))
+    s = np.zeros((N + 1, n))
+    K = np.zeros((N, m, n))
+    k = np.zeros((N, m))
+    S[N], s[N] = L["QN"], L["qN"]
+    for t in range(N - 1, -1, -1):
+        S[t], s[t], K[t], k[t] = riccati_step(A[t], B[t], L["Q"][t], L["R"][t], L["M"][t],
+                                              L["q"][t], L["r"][t], L["c"][t], S[t + 1], s[t + 1])
+    dx, du, lam = _forward(L, K, k, S, s)
+    return dict(dx=dx, du=du, lam=lam, S=S, s=s, K=K, k=k)
+
+
+# --- associative scan elements (Saerkkae & Garcia-Fernandez, D7) ----------
+# An element e = (A, b, C, eta, J) is the conditional cost
+#   V_e(x, y) = x'Jx/2 + eta'x + max_l l'(y - A x - b) - l'C l/2,
+# and e_ij (x) e_jk is min_y V_ij(x, y) + V_jk(y, z).  A value (suffix ending at
+# the terminal stage) has A = b = C = 0 and V(x) = x'Jx/2 + eta'x, i.e. (S, s).
+
+def leaf_element(L, t):
+    """Element of stage t after eliminating u (u = -R^{-1}(M'x + r) + v)."""
+    A, B, Q, R, M = L["A"][t], L["B"][t], L["Q"][t], L["R"][t], L["M"][t]
+    Ri = np.linalg.inv(R)
+    At = A - B @ Ri @ M.T
+    b = L["c"][t] - B @ Ri @ L["r"][t]
+    C = B @ Ri @ B.T
+    J = Q - M @ Ri @ M.T
+    eta = L["q"][t] - M @ Ri @ L["r"][t]
+    return (At, b, 0.5 * (C + C.T), eta, 0.5 * (J + J.T))
+
+
+def terminal_element(L):
+    n = L["QN"].shape[0]
+    return (np.zeros((n, n)), np.zeros(n), np.zeros((n, n)), L["qN"].copy(), L["QN"].copy())
+
+
+def identity_element(n):
+    return (np.eye(n), np.zeros(n), np.zeros((n, n)), np.zeros(n), np.zeros((n, n)))
+
+
+def combine(e1, e2):
+    """General combine e_ij (x) e_jk with X = (I + C_ij J_jk)^{-1}."""
+    A1, b1, C1, eta1, J1 = e1
+    A2, b2, C2, eta2, J2 = e2
+    n = A1.shape[0]
+    X = np.linalg.inv(np.eye(n) + C1 @ J2)
+    F = A2 @ X
+    A = F @ A1
+    b = F @ (b1 - C1 @ eta2) + b2
+    C = F @ C1 @ A2.T + C2
+    eta = A1.T @ X.T @ (eta2 + J2 @ b1) + eta1
+    J = A1.T @ X.T @ J2 @ A1 + J1
+    return (A, b, 0.5 * (C + C.T), eta, 0.5 * (J + J.T))
+
+
+def combine_leaf(L, t, e2):
+    """e_t (x) e2 for a leaf e_t, using C_t = B R^{-1} B^T (rank m) and Woodbury:
+    (I + C_t J2)^{-1} = I - B H^{-1} B^T J2, H = R + B^T J2 B; then
+    X C_t = B H^{-1} B^T.  Same result as ``combine(leaf_element(L, t), e2)``."""
+    A2, b2, C2, eta2, J2 = e2
+    At, b1, _, eta1, J1 = leaf_element(L, t)
+    B, R = L["B"][t], L["R"][t]
+    H = R + B.T @ J2 @ B
+    Hi = np.linalg.inv(H)
+    JB = J2 @ B
+    n = At.shape[0]
+    XT = np.eye(n) - JB @ Hi @ B.T          # X^T = I - J2 B H^{-1} B^T
+    XAt = At - B @ Hi @ (JB.T @ At)         # X A_t
+    A = A2 @ XAt
+    # b = A2 X (b1 - C_t eta2) + b2 with X b1 = b1 - B H^{-1} B^T J2 b1, X C_t = B H^{-1} B^T
+    b = A2 @ (b1 - B @ (Hi @ (B.T @ (J2 @ b1 + eta2)))) + b2
+    P = A2 @ B
+    C = P @ Hi @ P.T + C2
+    Jt = J2 - JB @ Hi @ JB.T
+    J = J1 + At.T @ Jt @ At
+    eta = eta1 + At.T @ (XT @ (eta2 + J2 @ b1))
+    return (A, b, 0.5 * (C + C.T), eta, 0.5 * (J + J.T))
+
+
+def value_combine(e, S, s):
+    """e (x) value(S, s) -> value (J, eta): a general element followed by a suffix."""
+    A1, b1, C1, eta1, J1 = e
+    n = A1.shape[0]
+    XT = np.linalg.inv(np.eye(n) + S @ C1)   # (I + S C)^{-1} = X^T
+    Jn = J1 + A1.T @ XT @ S @ A1
+    etan = eta1 + A1.T @ XT @ (s + S @ b1)
+    return 0.5 * (Jn + Jn.T), etan
+
+
+def segments(N1, P):
+    """Split element indices 0..N1-1 into P contiguous segments (as equal as
+    possible, earlier segments get the remainder)."""
+    P = max(1, min(P, N1))
+    base, rem = divmod(N1, P)
+    bounds = [0]
+    for i in range(P):
+        bounds.append(bounds[-1] + base + (1 if i < rem else 0))
+    return [(bounds[i], bounds[i + 1]) for i in range(P)]
+
+
+def _policy(L, S, s):
+    A, B = L["A"], L["B"]
+    N, n, m = B.shape
+    K = np.zeros((N, m, n))
+    k = np.zeros((N, m))
+    for t in range(N):
+        H = L["R"][t] + B[t].T @ S[t + 1] @ B[t]
+        G = L["M"][t].T + B[t].T @ S[t + 1] @ A[t]
+        gu = L["r"][t] + B[t].T @ (S[t + 1] @ L["c"][t] + s[t + 1])
+        K[t] = np.linalg.solve(H, G)
+        k[t] = np.linalg.solve(H, gu)
+    return K, k
+
+
+def lqr_scan(L, P=4, tree=False):
+    """Backward pass by associative scan over the N+1 elements, policy, then the
+    forward affine-map scan split into P segments; costates lam = S dx + s.
+
+    ``tree=False``: chunked scan as the device runs it -- each of the P segments
+    reduces its leaves to one aggregate with leaf combines (the last segment,
+    which holds the terminal element, is a plain Riccati pass), the segment
+    suffixes are formed by value combines, then every segment runs Riccati from
+    its suffix.  ``tree=True``: a reverse inclusive scan with general combines
+    only (Hillis-Steele doubling), to pin the general combine.
+    """
+    A, B = L["A"], L["B"]
+    N, n, m = B.shape
+    N1 = N + 1
+    S = np.zeros((N1, n, n))
+    s = np.zeros((N1, n))
+    if tree:
+        elems = [leaf_element(L, t) for t in range(N)] + [terminal_element(L)]
+        cur = list(elems)
+        d = 1
+        while d < N1:
+            nxt = list(cur)
+            for i in range(N1):
+                if i + d < N1:
+                    nxt[i] = combine(cur[i], cur[i + d])
+            cur = nxt
+            d *= 2
+        for t in range(N1):
+            S[t], s[t] = cur[t][4], cur[t][3]
+    else:
+        segs = segments(N1, P)
+        # last segment: Riccati from the terminal element
+        a, b = segs[-1]
+        S[N], s[N] = L["QN"], L["qN"]
+        for t in range(N - 1, a - 1, -1):
+            S[t], s[t], _, _ = riccati_step(A[t], B[t], L["Q"][t], L["R"][t], L["M"][t],
+                                            L["q"][t], L["r"][t], L["c"][t], S[t + 1], s[t + 1])
+        # up-sweep of the other segments: leaf combines from the right
+        aggs = []
+        for (a, b) in segs[:-1]:
+            agg = leaf_element(L, b - 1)
+            for t in range(b - 2, a - 1, -1):
+                agg = combine_leaf(L, t, agg)
+            aggs.append(agg)
+        # suffix values at segment starts: start_i = agg_i (x) start_{i+1}
+        starts = [None] * len(segs)
+        starts[-1] = (S[segs[-1][0]].copy(), s[segs[-1][0]].copy())
+        for i in range(len(segs) - 2, -1, -1):
+            starts[i] = value_combine(aggs[i], *starts[i + 1])
+        # down-sweep: Riccati inside each segment from the next segment's start
+        for i, (a, b) in enumerate(segs[:-1]):
+            S[b], s[b] = starts[i + 1]
+            for t in range(b - 1, a - 1, -1):
+                S[t], s[t], _, _ = riccati_step(A[t], B[t], L["Q"][t], L["R"][t], L["M"][t],
+                                                L["q"][t], L["r"][t], L["c"][t], S[t + 1], s[t + 1])
+    K, k = _policy(L, S, s)
+    # forward pass as an affine-map scan over P segments
+    F = np.stack([A[t] - B[t] @ K[t] for t in range(N)])
+    f = np.stack([L["c"][t] - B[t] @ k[t] for t in range(N)])
+    segs = segments(N, P)
+    maps = []
+    for (a, b) in segs:
+        Phi, phi = np.eye(n), np.zeros(n)
+        for t in range(a, b):
+            Phi, phi = F[t] @ Phi, F[t] @ phi + f[t]
+        maps.append((Phi, phi))
+    dx = np.zeros((N1, n))
+    x = L["c0"].copy()
+    for (a, b), (Phi, phi) in zip(segs, maps):
+        dx[a] = x
+        for t in range(a, b):
+            dx[t + 1] = F[t] @ dx[t] + f[t]
+        x = Phi @ x + phi
+        assert np.allclose(x, dx[b], rtol=1e-8, atol=1e-8 * (1 + np.abs(x).max()))
+    du = np.stack([-K[t] @ dx[t] - k[t] for t in range(N)])
+    lam = np.einsum("tij,tj->ti", S, dx) + s
+    return dict(dx=dx, du=du, lam=lam, S=S, s=s, K=K, k=k)
+
+
+# ---------------------------------------------------------------------------
+# merit, line search, SQP step
+# ---------------------------------------------------------------------------
+
+def merit(inst, X, U, V, Yeq, rho, rho_c, mu_b):
+    """phi = sum AL/barrier cost + V^T c + rho/2 |c|^2 (D3, D6)."""
+    N = inst["N"]
+    c = constraints(inst, X, U)
+    val = 0.0
+    for t in range(N + 1):
+        u = U[t] if t < N else np.zeros(inst["m"])
+        val += al_cost(inst, X[t], u, t, Yeq[t] if t < N and inst["p"] > 0 else None,
+                       rho_c, mu_b)
+    val += np.sum(V * c) + 0.5 * rho * np.sum(c * c)
+    return val
+
+
+def _ftb(v, dv, tau):
+    """Largest a in (0, 1] with v + a dv >= (1 - tau) v for v > 0."""
+    neg = dv < 0.0
+    if not np.any(neg):
+        return 1.0
+    return float(min(1.0, np.min(-tau * v[neg] / dv[neg])))
+
+
+def sqp_step(inst, X, U, V, Yeq, Ybox, params=None, lqr="riccati", P=4):
+    """One primal-dual SQP iteration (the reference's ``mpc`` call).
+
+    Returns dict with X, U, V, Yeq, Ybox (new iterate) and diagnostics
+    alpha, rho, phi0, dphi, trials, status, dX, dU, dV.
+    """
+    prm = dict(DEFAULT_PARAMS)
+    if params:
+        prm.update(params)
+    N, n, m, p = inst["N"], inst["n"], inst["m"], inst["p"]
+    rho_c, mu_b, tau = prm["rho_c"], prm["mu_b"], prm["tau"]
+    box = has_box(inst)
+    U = box_interior(inst, U)
+    if box:
+        Sl, Ybox = box_duals(inst, U, Ybox, mu_b)
+    L = linearize(inst, X, U, Yeq, Ybox, rho_c, mu_b)
+    if lqr == "riccati":
+        sol = lqr_riccati(L)
+    elif lqr == "dense":
+        sol = lqr_dense(L)
+    elif lqr == "scan":
+        sol = lqr_scan(L, P=P)
+    else:
+        raise ValueError(lqr)
+    dX, dU, lam = sol["dx"], sol["du"], sol["lam"]
+    dV = lam - V
+    c = np.vstack([L["c0"][None], L["c"]])
+    d_f = np.sum(L["q"] * dX[:N]) + np.sum(L["r"] * dU) + L["qN"] @ dX[N]
+    d_c = np.sum((dV - V) * c)
+    cc = np.sum(c * c)
+    kap = dX[N] @ L["QN"] @ dX[N]
+    for t in range(N):
+        kap += dX[t] @ L["Q"][t] @ dX[t] + dU[t] @ L["R"][t] @ dU[t] + 2.0 * dX[t] @ L["M"][t] @ dU[t]
+    num = d_f + d_c + 0.5 * kap
+    rho = prm["rho0"]
+    if cc > 0.0:
+        rho = max(rho, 2.0 * num / cc)
+    rho = min(rho, prm["rho_max"])
+    dphi = d_f + d_c - rho * cc
+    amax, adual = 1.0, 1.0
+    if box:
+        dS = np.hstack([-dU, dU])                       # slack steps (hi, lo)
+        dY = mu_b / Sl - Ybox - Ybox * dS / Sl          # Newton dual step
+        amax = _ftb(Sl.ravel(), dS.ravel(), tau)
+        adual = _ftb(Ybox.ravel(), dY.ravel(), tau)
+    phi0 = merit(inst, X, U, V, Yeq, rho, rho_c, mu_b)
+    slack = prm["armijo_slack"] * (1.0 + abs(phi0))
+    alpha, status, trials = None, 0, 0
+    for j in range(prm["max_trials"]):
+        a = amax * prm["beta"] ** j
+        trials += 1
+        phi = merit(inst, X + a * dX, U + a * dU, V + a * dV, Yeq, rho, rho_c, mu_b)
+        if phi <= phi0 + prm["c1"] * a * dphi + slack:
+            alpha = a
+            break
+    if alpha is None:
+        alpha, status = amax * prm["beta"] ** (prm["max_trials"] - 1), 2
+    Xn, Un, Vn = X + alpha * dX, U + alpha * dU, V + alpha * dV
+    Yeq_n, Ybox_n = Yeq.copy(), Ybox.copy()
+    if p > 0:
+        for t in range(N):
+            Yeq_n[t] = Yeq[t] + rho_c * eq_rows(inst, Xn[t], Un[t], t)
+    if box:
+        Ybox_n = Ybox + adual * dY
+    return dict(X=Xn, U=Un, V=Vn, Yeq=Yeq_n, Ybox=Ybox_n, alpha=alpha, rho=rho,
+                phi0=phi0, dphi=dphi, trials=trials, status=status, amax=amax,
+                adual=adual, dX=dX, dU=dU, dV=dV, lqr=L, Uin=U, Ybox_in=Ybox)
+
+
+def kkt_residual(inst, X, U, V, Yeq, Ybox, mu_b=DEFAULT_PARAMS["mu_b"]):
+    """2-norm of the KKT residual of the constrained problem (D8) at (X,U,V,mu,nu)."""
+    N, n, m, p = inst["N"], inst["n"], inst["m"], inst["p"]
+    parts = []
+    for t in range(N):
+        x, u = X[t], U[t]
+        fx, fu = dynamics_jac(inst, x, u, t)
+        gx = Q_at(inst, t) @ (x - inst["xref"][t]) + fx.T @ V[t + 1] - V[t]
+        gu = R_at(inst, t) @ (u - inst["uref"][t]) + fu.T @ V[t + 1]
+        if p > 0:
+            gx = gx + inst["D"][t].T @ Yeq[t]
+            gu = gu + inst["E"][t].T @ Yeq[t]
+            parts.append(eq_rows(inst, x, u, t))
+        if has_box(inst):
+            gu = gu + Ybox[t][:m] - Ybox[t][m:]
+            parts.append(Ybox[t] * (-box_g(inst, u)) - mu_b)
+        parts += [gx, gu]
+    parts.append(inst["QN"] @ (X[N] - inst["xref"][N]) - V[N])
+    parts.append(constraints(inst, X, U).ravel())
+    r = np.concatenate([np.ravel(v) for v in parts])
+    return float(np.sqrt(r @ r))
+
+
+def shift_warmstart(Z, shift):
+    """mpx/utils/mpc_wrapper.py:81-83: drop the first ``shift`` rows and repeat the last."""
+    return np.concatenate([Z[shift:], np.repeat(Z[-1:], shift, axis=0)])
+
+
+def solve(inst, X, U, V, Yeq, Ybox, iters, params=None, lqr="riccati"):
+    """``iters`` SQP iterations; returns the list of per-iteration step dicts."""
+    out = []
+    for _ in range(iters):
+        st = sqp_step(inst, X, U, V, Yeq, Ybox, params, lqr=lqr)
+        X, U, V, Yeq, Ybox = st["X"], st["U"], st["V"], st["Yeq"], st["Ybox"]
+        out.append(st)
+    return out

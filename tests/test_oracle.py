@@ -1,0 +1,225 @@
+"""Pins the FP64 oracle against what the mathematics fixes.
+
+Each check is chosen so that a plausible slip (sign of V, a dropped cross term
+M, a transposed Jacobian, a wrong combine operand order) fails at least one of
+them: dense KKT brute force, textbook Riccati, associativity / identity of the
+scan combine, exactness of one Newton step on an LQ problem, finite differences
+of the Jacobians and of the merit slope, and the multiplier closed forms.
+"""
+import numpy as np
+import pytest
+
+from mpx_amd import synthetic as syn
+from oracle import pdilqr as orc
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(1.0, np.max(np.abs(b)))
+
+
+def _random_lqr(rng, N, n, m, cross=True):
+    L = dict(A=np.eye(n)[None] + 0.3 * rng.standard_normal((N, n, n)),
+             B=rng.standard_normal((N, n, m)),
+             q=rng.standard_normal((N, n)), r=rng.standard_normal((N, m)),
+             c=rng.standard_normal((N, n)))
+    Q, R, M = [], [], []
+    for _ in range(N):
+        G = rng.standard_normal((n + m, n + m))
+        H = G @ G.T + 0.5 * np.eye(n + m)
+        Q.append(H[:n, :n]); R.append(H[n:, n:]); M.append(H[:n, n:] if cross else np.zeros((n, m)))
+    L["Q"], L["R"], L["M"] = np.array(Q), np.array(R), np.array(M)
+    G = rng.standard_normal((n, n))
+    L["QN"] = G @ G.T + np.eye(n)
+    L["qN"] = rng.standard_normal(n)
+    L["c0"] = rng.standard_normal(n)
+    return L
+
+
+@pytest.mark.parametrize("N,n,m", [(1, 2, 1), (4, 3, 2), (7, 5, 3), (6, 4, 4)])
+def test_riccati_matches_dense_kkt(N, n, m):
+    rng = np.random.default_rng(N * 100 + n)
+    L = _random_lqr(rng, N, n, m)
+    d = orc.lqr_dense(L)
+    r = orc.lqr_riccati(L)
+    for k in ("dx", "du", "lam"):
+        assert _rel(r[k], d[k]) < 1e-10, k
+
+
+@pytest.mark.parametrize("P", [1, 2, 3, 4, 8, 64])
+def test_chunked_scan_matches_riccati(P):
+    rng = np.random.default_rng(7)
+    L = _random_lqr(rng, 11, 5, 3)
+    r = orc.lqr_riccati(L)
+    s = orc.lqr_scan(L, P=P)
+    for k in ("dx", "du", "lam", "S", "s", "K", "k"):
+        assert _rel(s[k], r[k]) < 1e-9, (k, P)
+
+
+def test_tree_scan_matches_riccati():
+    rng = np.random.default_rng(8)
+    L = _random_lqr(rng, 9, 4, 2)
+    r = orc.lqr_riccati(L)
+    s = orc.lqr_scan(L, tree=True)
+    for k in ("S", "s", "dx", "du", "lam"):
+        assert _rel(s[k], r[k]) < 1e-9, k
+
+
+def _rand_elem(rng, n, full_C=True):
+    A = rng.standard_normal((n, n))
+    b = rng.standard_normal(n)
+    G = rng.standard_normal((n, n if full_C else 2))
+    C = G @ G.T
+    H = rng.standard_normal((n, n))
+    J = H @ H.T
+    return (A, b, C, rng.standard_normal(n), J)
+
+
+def test_combine_associative_and_identity():
+    rng = np.random.default_rng(3)
+    n = 5
+    e1, e2, e3 = (_rand_elem(rng, n) for _ in range(3))
+    lhs = orc.combine(orc.combine(e1, e2), e3)
+    rhs = orc.combine(e1, orc.combine(e2, e3))
+    for a, b in zip(lhs, rhs):
+        assert _rel(a, b) < 1e-10
+    idn = orc.identity_element(n)
+    for a, b in zip(orc.combine(e1, idn), e1):
+        assert _rel(a, b) < 1e-12
+    for a, b in zip(orc.combine(idn, e1), e1):
+        assert _rel(a, b) < 1e-12
+
+
+def test_leaf_combine_equals_general_and_riccati():
+    rng = np.random.default_rng(4)
+    L = _random_lqr(rng, 3, 5, 2)
+    e2 = _rand_elem(rng, 5)
+    a = orc.combine_leaf(L, 1, e2)
+    b = orc.combine(orc.leaf_element(L, 1), e2)
+    for x, y in zip(a, b):
+        assert _rel(x, y) < 1e-10
+    # leaf (x) value == one Riccati step
+    S = L["QN"]; s = L["qN"]
+    J, eta = orc.value_combine(orc.leaf_element(L, 2), S, s)
+    St, st, _, _ = orc.riccati_step(L["A"][2], L["B"][2], L["Q"][2], L["R"][2], L["M"][2],
+                                    L["q"][2], L["r"][2], L["c"][2], S, s)
+    assert _rel(J, St) < 1e-10 and _rel(eta, st) < 1e-10
+    # (leaf (x) terminal) through the general combine gives the same value
+    v = orc.combine(orc.leaf_element(L, 2), orc.terminal_element(L))
+    assert _rel(v[4], St) < 1e-10 and _rel(v[3], st) < 1e-10
+    assert np.max(np.abs(v[0])) == 0 and np.max(np.abs(v[2])) == 0
+
+
+def test_tanh_jacobian_finite_difference():
+    inst = syn.make_instance(2, 0, n=6, m=3, N=4)
+    rng = np.random.default_rng(0)
+    x, u = rng.standard_normal(6), rng.standard_normal(3)
+    fx, fu = orc.dynamics_jac(inst, x, u, 0)
+    eps = 1e-6
+    for j in range(6):
+        e = np.zeros(6); e[j] = eps
+        fd = (orc.dynamics(inst, x + e, u, 0) - orc.dynamics(inst, x - e, u, 0)) / (2 * eps)
+        assert np.max(np.abs(fd - fx[:, j])) < 1e-8
+    for j in range(3):
+        e = np.zeros(3); e[j] = eps
+        fd = (orc.dynamics(inst, x, u + e, 0) - orc.dynamics(inst, x, u - e, 0)) / (2 * eps)
+        assert np.max(np.abs(fd - fu[:, j])) < 1e-8
+
+
+def test_lq_one_step_is_exact():
+    """Workload 1: linear dynamics, quadratic cost -> one Newton step solves it,
+    alpha = 1 is accepted and V+ = lam makes the Lagrangian stationary."""
+    inst = syn.make_instance(1, 0)
+    X, U, V, Yeq, Ybox = syn.initial_iterate(inst, "tile")
+    k0 = orc.kkt_residual(inst, X, U, V, Yeq, Ybox)
+    st = orc.sqp_step(inst, X, U, V, Yeq, Ybox)
+    assert st["alpha"] == 1.0 and st["status"] == 0
+    k1 = orc.kkt_residual(inst, st["X"], st["U"], st["V"], st["Yeq"], st["Ybox"])
+    assert k1 / max(1.0, k0) < 1e-10
+    # the same step through the dense KKT and the chunked scan
+    st_d = orc.sqp_step(inst, X, U, V, Yeq, Ybox, lqr="scan", P=5)
+    assert _rel(st_d["X"], st["X"]) < 1e-10 and _rel(st_d["V"], st["V"]) < 1e-10
+
+
+def test_lq_small_dense_path():
+    inst = syn.make_instance(1, 0, n=4, m=2, N=6)
+    X, U, V, Yeq, Ybox = syn.initial_iterate(inst, "tile")
+    a = orc.sqp_step(inst, X, U, V, Yeq, Ybox, lqr="dense")
+    b = orc.sqp_step(inst, X, U, V, Yeq, Ybox, lqr="riccati")
+    for k in ("X", "U", "V"):
+        assert _rel(a[k], b[k]) < 1e-10
+
+
+@pytest.mark.parametrize("wl", [2, 3, 4])
+def test_merit_slope_matches_finite_difference(wl):
+    inst = syn.make_instance(wl, 1, n=6, m=3, N=5, p=2 if wl != 2 else 0)
+    X, U, V, Yeq, Ybox = syn.initial_iterate(inst, "zero")
+    rng = np.random.default_rng(wl)
+    X = X + 0.3 * rng.standard_normal(X.shape)
+    U = U + 0.8 * rng.standard_normal(U.shape)
+    V = 0.2 * rng.standard_normal(V.shape)
+    Yeq = 0.1 * rng.standard_normal(Yeq.shape)
+    Ybox = np.abs(0.1 * rng.standard_normal(Ybox.shape))
+    U = np.clip(U, -0.9, 0.9)
+    st = orc.sqp_step(inst, X, U, V, Yeq, Ybox)
+    rho, rc, mb = st["rho"], orc.DEFAULT_PARAMS["rho_c"], orc.DEFAULT_PARAMS["mu_b"]
+    dX, dU, dV = st["dX"], st["dU"], st["dV"]
+    h = 1e-7
+    fp = orc.merit(inst, X + h * dX, U + h * dU, V + h * dV, Yeq, rho, rc, mb)
+    fm = orc.merit(inst, X - h * dX, U - h * dU, V - h * dV, Yeq, rho, rc, mb)
+    fd = (fp - fm) / (2 * h)
+    assert abs(fd - st["dphi"]) < 1e-5 * (1 + abs(fd))
+    assert st["dphi"] < 0.0
+    # Armijo holds at the accepted alpha
+    a = st["alpha"]
+    phi = orc.merit(inst, st["X"], st["U"], st["V"], Yeq, rho, rc, mb)
+    slack = orc.DEFAULT_PARAMS["armijo_slack"] * (1 + abs(st["phi0"]))
+    assert phi <= st["phi0"] + orc.DEFAULT_PARAMS["c1"] * a * st["dphi"] + slack
+
+
+def test_box_and_equality_converge():
+    """Small workload-4-like instance: equality rows and boxes are met and the
+    KKT residual falls by orders of magnitude over SQP iterations."""
+    inst = syn.make_instance(4, 1, n=8, m=6, N=6, p=2)
+    X, U, V, Yeq, Ybox = syn.initial_iterate(inst, "zero")
+    k0 = orc.kkt_residual(inst, X, U, V, Yeq, Ybox)
+    steps = orc.solve(inst, X, U, V, Yeq, Ybox, 40)
+    last = steps[-1]
+    k = orc.kkt_residual(inst, last["X"], last["U"], last["V"], last["Yeq"], last["Ybox"])
+    assert k / k0 < 1e-8
+    assert np.all(last["Ybox"] > 0)
+    assert np.all(last["U"] < 1) and np.all(last["U"] > -1)
+    h = np.array([orc.eq_rows(inst, last["X"][t], last["U"][t], t) for t in range(inst["N"])])
+    assert np.max(np.abs(h)) < 1e-6
+
+
+def test_box_active_somewhere():
+    """The box workloads actually hit their bounds (else the box path is untested)."""
+    inst = syn.make_instance(2, 0, n=24, m=12, N=20)
+    X, U, V, Yeq, Ybox = syn.initial_iterate(inst, "rollout")
+    steps = orc.solve(inst, X, U, V, Yeq, Ybox, 5)
+    # some box multiplier far above the central-path value mu_b / s at s ~ 1
+    assert np.max(steps[-1]["Ybox"]) > 1.0
+
+
+def test_box_dual_step_closed_form():
+    """Complementarity is linearised exactly: (nu + dnu)(s + ds) - mu_b = dnu ds."""
+    inst = syn.make_instance(2, 0, n=6, m=3, N=4)
+    X, U, V, Yeq, Ybox = syn.initial_iterate(inst, "rollout")
+    st = orc.sqp_step(inst, X, U, V, Yeq, Ybox)
+    mb, m = orc.DEFAULT_PARAMS["mu_b"], 3
+    S, Y = orc.box_duals(inst, st["Uin"], st["Ybox_in"], mb)
+    dS = np.hstack([-st["dU"], st["dU"]])
+    dY = (st["Ybox"] - Y) / st["adual"]
+    lhs = (Y + dY) * (S + dS) - mb
+    assert np.max(np.abs(lhs - dY * dS)) < 1e-10
+
+
+def test_model_evaluator_and_shift():
+    inst = syn.make_instance(2, 0, n=5, m=2, N=4)
+    X, U, V, Yeq, Ybox = syn.initial_iterate(inst, "rollout")
+    g, c = orc.model_evaluator_helper(inst, X, U)
+    assert g.shape == (5,) and c.shape == (5, 5)
+    assert np.max(np.abs(c)) < 1e-14          # rollout is feasible
+    Z = np.arange(12.0).reshape(4, 3)
+    S = orc.shift_warmstart(Z, 1)
+    assert np.array_equal(S[:3], Z[1:]) and np.array_equal(S[3], Z[3])
