@@ -1,0 +1,155 @@
+/*
+ * mpx_kkt.h -- C ABI of the MI355X (gfx950) primal-dual iLQR KKT hot path.
+ *
+ * The reference reaches its solver through two Python/JAX entry points of the
+ * un-vendored submodule primal_dual_ilqr (.gitmodules:1-3):
+ *   optimizers.mpc(cost, dynamics, hessian_approx, limited_memory, reference,
+ *                  parameter, W, x0, X_in, U_in, V_in) -> (X, U, V)
+ *       called at mpx/utils/mpc_wrapper.py:74, :150-158, :244, :346-354,
+ *       mpx/utils/mpc_wrapper_srbd.py:68, :124-132, mpx/examples/acrobat.py:101-113
+ *   optimizers.model_evaluator_helper(cost, dynamics, x0, X, U) -> (g, c)
+ *       called at mpx/utils/mpc_wrapper.py:422, :452
+ * and keeps its warm start with update_and_extract (mpx/utils/mpc_wrapper.py:79-96,
+ * :261-282; mpx/utils/mpc_wrapper_srbd.py:135-137).  The functions below replace
+ * those calls for a whole batch of instances (the jax.vmap of
+ * mpx/utils/mpc_wrapper_srbd.py:77 and mpx/examples/multi_env.py:49) in one
+ * stream-ordered call.
+ *
+ * Conventions
+ *  - All pointers are device pointers (HBM) owned by the caller; nothing is
+ *    allocated by the library except inside mpxkkt_* calls documented so.
+ *  - Calls are enqueued on `stream` (a hipStream_t passed as void*; NULL is the
+ *    default stream) and never synchronise the host.
+ *  - Every entry point returns an mpxkkt_status; on error the reason is kept
+ *    per thread in mpxkkt_last_error().  NaNs in the data propagate and are
+ *    reported per instance in mpxkkt_stats.status, never as an error code.
+ *  - Iterates are dense, instance-major, row-major:
+ *      X [batch][N+1][n], U [batch][N][m], V [batch][N+1][n],
+ *      Y_eq [batch][N][p], Y_box [batch][N][2m] (upper bounds, then lower).
+ *  - Problem data is given as strided views: element (b, t, ...) of a view is
+ *    at ptr + b*batch_stride + t*stage_stride (in elements), the trailing
+ *    matrix row-major and dense.  A zero stride shares data across
+ *    instances or stages (time-invariant Q, R; per-instance A, B, W).
+ */
+#ifndef MPX_KKT_H
+#define MPX_KKT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  MPXKKT_OK = 0,
+  MPXKKT_ERR_ARG = 1,          /* invalid argument (dims, null pointer, dtype) */
+  MPXKKT_ERR_UNSUPPORTED = 2,  /* shape exceeds the device limits of a kernel */
+  MPXKKT_ERR_WORKSPACE = 3,    /* workspace too small */
+  MPXKKT_ERR_DEVICE = 4        /* HIP runtime error (launch, attribute) */
+} mpxkkt_status;
+
+enum { MPXKKT_F32 = 0, MPXKKT_F64 = 1 };
+
+/* dynamics families evaluated on the device */
+enum {
+  MPXKKT_MODEL_LTV = 0,   /* x' = A_t x + B_t u                                   */
+  MPXKKT_MODEL_TANH = 1   /* x' = x + h (A x + B u + kappa tanh(W x))            */
+};
+
+typedef struct {
+  const void* ptr;
+  int64_t batch_stride;
+  int64_t stage_stride;
+} mpxkkt_view;
+
+/* One batch of same-shaped instances.  Cost: sum_t 1/2 |x_t - xref_t|_Q_t^2 +
+ * 1/2 |u_t - uref_t|_R_t^2 + 1/2 |x_N - xref_N|_QN^2 (the Gauss-Newton
+ * (Q, R, M = 0) triple of hessian_approx, mpx/utils/objectives.py:75), with
+ * optional rows D_t x + E_t u + e_t = 0 and boxes ulo <= u <= uhi. */
+typedef struct {
+  int32_t dtype;        /* MPXKKT_F32 / MPXKKT_F64 for every array */
+  int32_t model;        /* MPXKKT_MODEL_* */
+  int32_t batch, n, m, horizon, n_eq;
+  int32_t has_box;
+  double h, kappa;      /* MPXKKT_MODEL_TANH constants */
+  mpxkkt_view A, B, W;  /* LTV: A_t (n x n), B_t (n x m); TANH: A, B, W per instance */
+  mpxkkt_view Q, R, QN, xref, uref;
+  mpxkkt_view D, E, e;  /* p x n, p x m, p (NULL when n_eq == 0) */
+  mpxkkt_view ulo, uhi; /* m (NULL when has_box == 0) */
+  mpxkkt_view x0;       /* n */
+} mpxkkt_problem;
+
+typedef struct {
+  void* X;
+  void* U;
+  void* V;
+  void* Y_eq;   /* may be NULL when n_eq == 0 */
+  void* Y_box;  /* may be NULL when has_box == 0 */
+} mpxkkt_iterate;
+
+typedef struct {
+  double rho0, rho_max;   /* merit penalty floor and cap                    */
+  double rho_c;           /* augmented-Lagrangian penalty of equality rows  */
+  double mu_b, tau;       /* box barrier and fraction-to-boundary factor   */
+  double beta, c1;        /* backtracking factor and Armijo constant       */
+  double armijo_slack;    /* relative noise floor of the Armijo test       */
+  int32_t max_trials;     /* line-search trials                            */
+  int32_t limited_memory; /* 1: evaluate one trial at a time               */
+  int32_t segments;       /* time-parallel scan segments per instance (1 = sequential Riccati) */
+  int32_t block_threads;  /* threads per block of the per-instance kernels (0 = auto) */
+  int32_t trial_group;    /* trials evaluated together (0 = all); 1 if limited_memory */
+  int32_t reserved;
+} mpxkkt_params;
+
+/* Per-instance outputs of one SQP iteration (each may be NULL). */
+typedef struct {
+  void* alpha;     /* accepted step            [batch] dtype */
+  void* rho;       /* merit penalty            [batch] dtype */
+  void* merit0;    /* phi(0)                   [batch] dtype */
+  void* dmerit;    /* phi'(0)                  [batch] dtype */
+  int32_t* status; /* 0 ok, 1 non-finite, 2 line search exhausted [batch] */
+  int32_t* trials; /* trials evaluated         [batch] */
+} mpxkkt_stats;
+
+/* Defaults of the oracle's decisions D3-D6. */
+mpxkkt_status mpxkkt_default_params(mpxkkt_params* out);
+
+/* Bytes of device workspace mpxkkt_sqp_iterate / mpxkkt_kkt_residual need. */
+mpxkkt_status mpxkkt_workspace_size(const mpxkkt_problem* prob, const mpxkkt_params* prm,
+                                    size_t* bytes);
+
+/* One primal-dual SQP iteration for every instance, in place on `it`:
+ * linearise + augment (K1), LQR by sequential Riccati or the chunked
+ * associative scan (K2/K3), multiplier step, merit line search and update (K4).
+ * Replaces optimizers.mpc (mpx/utils/mpc_wrapper.py:74, :150-158). */
+mpxkkt_status mpxkkt_sqp_iterate(void* stream, const mpxkkt_problem* prob,
+                                 const mpxkkt_params* prm, mpxkkt_iterate* it,
+                                 mpxkkt_stats* stats, void* workspace, size_t workspace_bytes);
+
+/* KKT residual 2-norm per instance at `it` (out: [batch], dtype).  The
+ * stop test of runOffline, mpx/utils/mpc_wrapper.py:452-466, in KKT form. */
+mpxkkt_status mpxkkt_kkt_residual(void* stream, const mpxkkt_problem* prob,
+                                  const mpxkkt_params* prm, const mpxkkt_iterate* it,
+                                  void* out, void* workspace, size_t workspace_bytes);
+
+/* (g, c) of optimizers.model_evaluator_helper (mpx/utils/mpc_wrapper.py:422):
+ * g [batch][N+1] per-stage cost, c [batch][N+1][n] = [x0 - X_0, f(x_t,u_t) - x_{t+1}]. */
+mpxkkt_status mpxkkt_model_eval(void* stream, const mpxkkt_problem* prob, const void* X,
+                                const void* U, void* g, void* c);
+
+/* Warm-start shift with per-instance NaN fallback, update_and_extract of
+ * mpx/utils/mpc_wrapper.py:79-96: if U[b,0,0] is NaN the instance is reset to
+ * X_t = x0, U_t = u_reset (or 0), V = Y = 0 and reset[b] = 1; else every array
+ * drops its first `shift` rows and repeats its last row. */
+mpxkkt_status mpxkkt_shift_warmstart(void* stream, const mpxkkt_problem* prob, int32_t shift,
+                                     mpxkkt_iterate* it, const void* u_reset,
+                                     int32_t* reset);
+
+const char* mpxkkt_last_error(void);
+int32_t mpxkkt_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPX_KKT_H */

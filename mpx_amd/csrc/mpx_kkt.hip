@@ -1,0 +1,441 @@
+// C ABI of the primal-dual iLQR KKT hot path for gfx950 (declared in
+// include/mpx_kkt.h).  Host side: argument checks, workspace carving and the
+// launch sequence K1 (linearise) -> K2/K3 (LQR) -> K4 (line search + update).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "mpx_kkt.h"
+#include "common.h"
+#include "k_linearize.h"
+#include "k_lqr.h"
+#include "k_scan.h"
+#include "k_linesearch.h"
+#include "k_misc.h"
+
+using namespace mpxk;
+
+namespace {
+
+thread_local std::string g_err;
+
+mpxkkt_status fail(mpxkkt_status s, const std::string& msg) {
+  g_err = msg;
+  return s;
+}
+
+size_t dtype_size(int dt) { return dt == MPXKKT_F64 ? 8 : 4; }
+
+int pow2_threads(int want, int lo, int hi) {
+  int t = lo;
+  while (t < want && t < hi) t <<= 1;
+  return t;
+}
+
+template <typename T>
+View<T> view(const mpxkkt_view& v) {
+  View<T> r;
+  r.p = static_cast<const T*>(v.ptr);
+  r.sb = v.batch_stride;
+  r.st = v.stage_stride;
+  return r;
+}
+
+template <typename T>
+Prob<T> make_prob(const mpxkkt_problem* pr) {
+  Prob<T> P;
+  P.B = pr->batch; P.n = pr->n; P.m = pr->m; P.N = pr->horizon; P.p = pr->n_eq;
+  P.box = pr->has_box ? 1 : 0; P.model = pr->model;
+  P.h = (T)pr->h; P.kappa = (T)pr->kappa;
+  P.A = view<T>(pr->A); P.Bm = view<T>(pr->B); P.W = view<T>(pr->W);
+  P.Q = view<T>(pr->Q); P.R = view<T>(pr->R); P.QN = view<T>(pr->QN);
+  P.xref = view<T>(pr->xref); P.uref = view<T>(pr->uref);
+  P.D = view<T>(pr->D); P.E = view<T>(pr->E); P.e = view<T>(pr->e);
+  P.ulo = view<T>(pr->ulo); P.uhi = view<T>(pr->uhi); P.x0 = view<T>(pr->x0);
+  return P;
+}
+
+int effective_segments(const mpxkkt_problem* pr, const mpxkkt_params* prm) {
+  int P = prm->segments < 1 ? 1 : prm->segments;
+  const int maxP = (pr->horizon + 1) / 2;  // at least two elements per segment
+  if (P > maxP) P = maxP < 1 ? 1 : maxP;
+  return P;
+}
+
+template <typename T>
+size_t carve(const mpxkkt_problem* pr, int P, Ws<T>* ws, char* base) {
+  const size_t B = pr->batch, n = pr->n, m = pr->m, N = pr->horizon, N1 = N + 1;
+  size_t off = 0;
+  auto take = [&](size_t cnt) -> T* {
+    off = (off + 255) & ~(size_t)255;
+    T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+    off += cnt * sizeof(T);
+    return p;
+  };
+  const bool tanh_model = pr->model == MPXKKT_MODEL_TANH;
+  ws->Alin = take(tanh_model ? B * N * n * n : 0);
+  ws->Blin = take(tanh_model ? B * n * m : 0);
+  ws->qh = take(B * N1 * n);
+  ws->rh = take(B * N * m);
+  ws->rd = take(B * N * m);
+  ws->cdef = take(B * N1 * n);
+  ws->S = take(B * N1 * n * n);
+  ws->s = take(B * N1 * n);
+  ws->K = take(B * N * m * n);
+  ws->k = take(B * N * m);
+  ws->dx = take(B * N1 * n);
+  ws->du = take(B * N * m);
+  ws->dv = take(B * N1 * n);
+  ws->mpart = take(B * N1 * NMP);
+  ws->fpart = take(B * P * NFP);
+  ws->agg = take(P > 1 ? B * P * (3 * n * n + 2 * n) : 0);
+  ws->fmap = take(P > 1 ? B * P * (n * n + n) : 0);
+  ws->sval = take(P > 1 ? B * P * (n * n + n) : 0);
+  ws->xs = take(P > 1 ? B * P * n : 0);
+  ws->kpart = take(B * N1);
+  return off + 256;
+}
+
+size_t workspace_bytes(const mpxkkt_problem* pr, const mpxkkt_params* prm) {
+  const int P = effective_segments(pr, prm);
+  if (pr->dtype == MPXKKT_F64) {
+    Ws<double> w;
+    return carve<double>(pr, P, &w, nullptr);
+  }
+  Ws<float> w;
+  return carve<float>(pr, P, &w, nullptr);
+}
+
+mpxkkt_status check_view(const mpxkkt_view& v, const char* name, bool needed) {
+  if (needed && v.ptr == nullptr) return fail(MPXKKT_ERR_ARG, std::string("null view: ") + name);
+  if (v.batch_stride < 0 || v.stage_stride < 0)
+    return fail(MPXKKT_ERR_ARG, std::string("negative stride: ") + name);
+  return MPXKKT_OK;
+}
+
+mpxkkt_status check_problem(const mpxkkt_problem* pr) {
+  if (!pr) return fail(MPXKKT_ERR_ARG, "null problem");
+  if (pr->dtype != MPXKKT_F32 && pr->dtype != MPXKKT_F64) return fail(MPXKKT_ERR_ARG, "bad dtype");
+  if (pr->model != MPXKKT_MODEL_LTV && pr->model != MPXKKT_MODEL_TANH)
+    return fail(MPXKKT_ERR_ARG, "bad model");
+  if (pr->batch < 1 || pr->batch > 65535) return fail(MPXKKT_ERR_ARG, "batch must be in [1, 65535]");
+  if (pr->n < 1 || pr->m < 1 || pr->horizon < 1 || pr->n_eq < 0)
+    return fail(MPXKKT_ERR_ARG, "dimensions must be positive");
+  if (pr->n > 512 || pr->m > 512 || pr->n_eq > 512 || pr->horizon > 100000)
+    return fail(MPXKKT_ERR_UNSUPPORTED, "dimension too large");
+  mpxkkt_status s;
+  const bool eq = pr->n_eq > 0, box = pr->has_box != 0;
+  if ((s = check_view(pr->A, "A", true)) != MPXKKT_OK) return s;
+  if ((s = check_view(pr->B, "B", true)) != MPXKKT_OK) return s;
+  if ((s = check_view(pr->W, "W", pr->model == MPXKKT_MODEL_TANH)) != MPXKKT_OK) return s;
+  if ((s = check_view(pr->Q, "Q", true)) != MPXKKT_OK) return s;
+  if ((s = check_view(pr->R, "R", true)) != MPXKKT_OK) return s;
+  if ((s = check_view(pr->QN, "QN", true)) != MPXKKT_OK) return s;
+  if ((s = check_view(pr->xref, "xref", true)) != MPXKKT_OK) return s;
+  if ((s = check_view(pr->uref, "uref", true)) != MPXKKT_OK) return s;
+  if ((s = check_view(pr->D, "D", eq)) != MPXKKT_OK) return s;
+  if ((s = check_view(pr->E, "E", eq)) != MPXKKT_OK) return s;
+  if ((s = check_view(pr->e, "e", eq)) != MPXKKT_OK) return s;
+  if ((s = check_view(pr->ulo, "ulo", box)) != MPXKKT_OK) return s;
+  if ((s = check_view(pr->uhi, "uhi", box)) != MPXKKT_OK) return s;
+  if ((s = check_view(pr->x0, "x0", true)) != MPXKKT_OK) return s;
+  return MPXKKT_OK;
+}
+
+mpxkkt_status check_iterate(const mpxkkt_problem* pr, const mpxkkt_iterate* it) {
+  if (!it || !it->X || !it->U || !it->V) return fail(MPXKKT_ERR_ARG, "null iterate array");
+  if (pr->n_eq > 0 && !it->Y_eq) return fail(MPXKKT_ERR_ARG, "Y_eq required when n_eq > 0");
+  if (pr->has_box && !it->Y_box) return fail(MPXKKT_ERR_ARG, "Y_box required when has_box");
+  return MPXKKT_OK;
+}
+
+int g_max_lds = -1;
+
+mpxkkt_status lds_limit(int* out) {
+  if (g_max_lds < 0) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(MPXKKT_ERR_DEVICE, "hipGetDevice failed");
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      return fail(MPXKKT_ERR_DEVICE, "cannot query LDS size");
+    g_max_lds = v;
+  }
+  *out = g_max_lds;
+  return MPXKKT_OK;
+}
+
+template <typename K>
+mpxkkt_status prep_lds(K kernel, size_t bytes) {
+  int lim = 0;
+  mpxkkt_status s = lds_limit(&lim);
+  if (s != MPXKKT_OK) return s;
+  if (bytes > (size_t)lim)
+    return fail(MPXKKT_ERR_UNSUPPORTED, "shape needs " + std::to_string(bytes) +
+                                            " bytes of LDS per block, device has " + std::to_string(lim));
+  if (bytes > 65536) {
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
+      return fail(MPXKKT_ERR_DEVICE, "hipFuncSetAttribute failed");
+  }
+  return MPXKKT_OK;
+}
+
+mpxkkt_status launch_check(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(MPXKKT_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+  return MPXKKT_OK;
+}
+
+template <typename T>
+Prm<T> make_prm(const mpxkkt_params* prm, int P) {
+  Prm<T> r;
+  r.rho0 = (T)prm->rho0; r.rho_max = (T)prm->rho_max; r.rho_c = (T)prm->rho_c;
+  r.mu_b = (T)prm->mu_b; r.tau = (T)prm->tau; r.beta = (T)prm->beta; r.c1 = (T)prm->c1;
+  r.slack = (T)prm->armijo_slack;
+  r.max_trials = std::max(1, std::min(prm->max_trials, MAX_TRIALS));
+  int g = prm->trial_group <= 0 ? r.max_trials : prm->trial_group;
+  if (prm->limited_memory) g = 1;
+  r.group = std::max(1, std::min(g, r.max_trials));
+  r.P = P;
+  return r;
+}
+
+template <typename T>
+Iter<T> make_iter(const mpxkkt_iterate* it) {
+  Iter<T> r;
+  r.X = static_cast<T*>(it->X); r.U = static_cast<T*>(it->U); r.V = static_cast<T*>(it->V);
+  r.Yeq = static_cast<T*>(it->Y_eq); r.Ybox = static_cast<T*>(it->Y_box);
+  return r;
+}
+
+template <typename T>
+LqrIn<T> make_lqr_in(const mpxkkt_problem* pr, const Ws<T>& ws) {
+  LqrIn<T> in;
+  if (pr->model == MPXKKT_MODEL_TANH) {
+    const long long n = pr->n, m = pr->m, N = pr->horizon;
+    in.A.p = ws.Alin; in.A.sb = N * n * n; in.A.st = n * n;
+    in.Bm.p = ws.Blin; in.Bm.sb = n * m; in.Bm.st = 0;
+  } else {
+    in.A = view<T>(pr->A);
+    in.Bm = view<T>(pr->B);
+  }
+  return in;
+}
+
+int k1_threads(int n) { return n <= 64 ? 64 : 128; }
+
+template <typename T>
+mpxkkt_status run_scan(hipStream_t st, const mpxkkt_problem* pr, int nt, const Prob<T>& P, const Prm<T>& R,
+                       const Iter<T>& I, const Ws<T>& W, const LqrIn<T>& in) {
+  const int n = pr->n, m = pr->m, p = pr->n_eq, B = pr->batch, Pn = R.P;
+  mpxkkt_status s;
+  const size_t up = std::max(up_smem_elems<T>(n, m, p, nt), ric_smem_elems<T>(n, m, p, nt)) * sizeof(T);
+  if ((s = prep_lds(k_scan_up<T>, up)) != MPXKKT_OK) return s;
+  hipLaunchKernelGGL(k_scan_up<T>, dim3(Pn, B), dim3(nt), up, st, P, R, I, W, in);
+  if ((s = launch_check("k_scan_up")) != MPXKKT_OK) return s;
+  const size_t mid = mid_smem_elems<T>(n, nt) * sizeof(T) + 64;
+  if ((s = prep_lds(k_scan_mid<T>, mid)) != MPXKKT_OK) return s;
+  hipLaunchKernelGGL(k_scan_mid<T>, dim3(B), dim3(nt), mid, st, P, R, W);
+  if ((s = launch_check("k_scan_mid")) != MPXKKT_OK) return s;
+  const size_t down = ric_smem_elems<T>(n, m, p, nt) * sizeof(T);
+  if ((s = prep_lds(k_scan_down<T>, down)) != MPXKKT_OK) return s;
+  hipLaunchKernelGGL(k_scan_down<T>, dim3(Pn - 1, B), dim3(nt), down, st, P, R, I, W, in);
+  if ((s = launch_check("k_scan_down")) != MPXKKT_OK) return s;
+  const size_t fu = fwd_smem_elems<T>(n, m, nt) * sizeof(T);
+  if ((s = prep_lds(k_fwd_up<T>, fu)) != MPXKKT_OK) return s;
+  hipLaunchKernelGGL(k_fwd_up<T>, dim3(Pn - 1, B), dim3(nt), fu, st, P, R, I, W, in);
+  if ((s = launch_check("k_fwd_up")) != MPXKKT_OK) return s;
+  hipLaunchKernelGGL(k_fwd_mid<T>, dim3(B), dim3(64), 2 * n * sizeof(T), st, P, R, W);
+  if ((s = launch_check("k_fwd_mid")) != MPXKKT_OK) return s;
+  const size_t fd = ((size_t)2 * n + m + 4 * nt + 8) * sizeof(T);
+  if ((s = prep_lds(k_fwd_down<T>, fd)) != MPXKKT_OK) return s;
+  hipLaunchKernelGGL(k_fwd_down<T>, dim3(Pn - 1, B), dim3(nt), fd, st, P, R, I, W, in);
+  return launch_check("k_fwd_down");
+}
+
+template <typename T>
+mpxkkt_status run_linearize(hipStream_t st, const mpxkkt_problem* pr, const Prob<T>& P, const Prm<T>& R,
+                            const Iter<T>& I, const Ws<T>& W) {
+  const int nt = k1_threads(pr->n);
+  const size_t lds = ((size_t)2 * pr->n + pr->m + pr->n_eq + NMP * nt) * sizeof(T);
+  mpxkkt_status s = prep_lds(k_linearize<T>, lds);
+  if (s != MPXKKT_OK) return s;
+  dim3 grid(pr->horizon + 1, pr->batch);
+  hipLaunchKernelGGL(k_linearize<T>, grid, dim3(nt), lds, st, P, R, I, W);
+  return launch_check("k_linearize");
+}
+
+template <typename T>
+mpxkkt_status run_lqr(hipStream_t st, const mpxkkt_problem* pr, const mpxkkt_params* prm, const Prob<T>& P,
+                      const Prm<T>& R, const Iter<T>& I, const Ws<T>& W, const LqrIn<T>& in) {
+  const int nt = prm->block_threads > 0 ? pow2_threads(prm->block_threads, 64, 1024) : 256;
+  const size_t lds = ric_smem_elems<T>(pr->n, pr->m, pr->n_eq, nt) * sizeof(T);
+  mpxkkt_status s;
+  if (R.P <= 1) {
+    if ((s = prep_lds(k_lqr_seq<T>, lds)) != MPXKKT_OK) return s;
+    hipLaunchKernelGGL(k_lqr_seq<T>, dim3(pr->batch), dim3(nt), lds, st, P, R, I, W, in);
+    return launch_check("k_lqr_seq");
+  }
+  return run_scan<T>(st, pr, nt, P, R, I, W, in);
+}
+
+template <typename T>
+mpxkkt_status run_linesearch(hipStream_t st, const mpxkkt_problem* pr, const Prob<T>& P, const Prm<T>& R,
+                             const Iter<T>& I, const Ws<T>& W, const LqrIn<T>& in, const mpxkkt_stats* so) {
+  const int nt = 256;
+  const size_t lds = (size_t)MAX_TRIALS * nt * sizeof(T);
+  mpxkkt_status s = prep_lds(k_linesearch<T>, lds);
+  if (s != MPXKKT_OK) return s;
+  T* a = so ? static_cast<T*>(so->alpha) : nullptr;
+  T* r = so ? static_cast<T*>(so->rho) : nullptr;
+  T* f0 = so ? static_cast<T*>(so->merit0) : nullptr;
+  T* df = so ? static_cast<T*>(so->dmerit) : nullptr;
+  int* stt = so ? so->status : nullptr;
+  int* tr = so ? so->trials : nullptr;
+  hipLaunchKernelGGL(k_linesearch<T>, dim3(pr->batch), dim3(nt), lds, st, P, R, I, W, in, a, r, f0, df, stt, tr);
+  return launch_check("k_linesearch");
+}
+
+template <typename T>
+mpxkkt_status sqp_iterate_t(hipStream_t st, const mpxkkt_problem* pr, const mpxkkt_params* prm,
+                            mpxkkt_iterate* it, mpxkkt_stats* so, void* wsp) {
+  const int Pseg = effective_segments(pr, prm);
+  Ws<T> W;
+  carve<T>(pr, Pseg, &W, static_cast<char*>(wsp));
+  const Prob<T> P = make_prob<T>(pr);
+  const Prm<T> R = make_prm<T>(prm, Pseg);
+  const Iter<T> I = make_iter<T>(it);
+  const LqrIn<T> in = make_lqr_in<T>(pr, W);
+  mpxkkt_status s;
+  if ((s = run_linearize<T>(st, pr, P, R, I, W)) != MPXKKT_OK) return s;
+  if ((s = run_lqr<T>(st, pr, prm, P, R, I, W, in)) != MPXKKT_OK) return s;
+  return run_linesearch<T>(st, pr, P, R, I, W, in, so);
+}
+
+template <typename T>
+mpxkkt_status kkt_t(hipStream_t st, const mpxkkt_problem* pr, const mpxkkt_params* prm, const mpxkkt_iterate* it,
+                    void* out, void* wsp) {
+  Ws<T> W;
+  carve<T>(pr, effective_segments(pr, prm), &W, static_cast<char*>(wsp));
+  const Prob<T> P = make_prob<T>(pr);
+  const Prm<T> R = make_prm<T>(prm, 1);
+  const Iter<T> I = make_iter<T>(it);
+  const int nt = k1_threads(pr->n);
+  const size_t lds = ((size_t)3 * pr->n + pr->m + nt) * sizeof(T);
+  mpxkkt_status s = prep_lds(k_kkt_stage<T>, lds);
+  if (s != MPXKKT_OK) return s;
+  hipLaunchKernelGGL(k_kkt_stage<T>, dim3(pr->horizon + 1, pr->batch), dim3(nt), lds, st, P, R, I, W);
+  if ((s = launch_check("k_kkt_stage")) != MPXKKT_OK) return s;
+  hipLaunchKernelGGL(k_kkt_reduce<T>, dim3((pr->batch + 255) / 256), dim3(256), 0, st, pr->batch,
+                     pr->horizon + 1, (const T*)W.kpart, static_cast<T*>(out));
+  return launch_check("k_kkt_reduce");
+}
+
+}  // namespace
+
+extern "C" {
+
+mpxkkt_status mpxkkt_default_params(mpxkkt_params* out) {
+  if (!out) return fail(MPXKKT_ERR_ARG, "null params");
+  std::memset(out, 0, sizeof(*out));
+  out->rho0 = 1.0;
+  out->rho_max = 1e8;
+  out->rho_c = 10.0;
+  out->mu_b = 1e-2;
+  out->tau = 0.995;
+  out->beta = 0.5;
+  out->c1 = 1e-4;
+  out->armijo_slack = 1e-6;
+  out->max_trials = 10;
+  out->limited_memory = 0;
+  out->segments = 1;
+  out->block_threads = 0;
+  out->trial_group = 0;
+  return MPXKKT_OK;
+}
+
+mpxkkt_status mpxkkt_workspace_size(const mpxkkt_problem* prob, const mpxkkt_params* prm, size_t* bytes) {
+  mpxkkt_status s = check_problem(prob);
+  if (s != MPXKKT_OK) return s;
+  if (!prm || !bytes) return fail(MPXKKT_ERR_ARG, "null params or output");
+  *bytes = workspace_bytes(prob, prm);
+  return MPXKKT_OK;
+}
+
+mpxkkt_status mpxkkt_sqp_iterate(void* stream, const mpxkkt_problem* prob, const mpxkkt_params* prm,
+                                 mpxkkt_iterate* it, mpxkkt_stats* stats, void* workspace,
+                                 size_t workspace_bytes_) {
+  mpxkkt_status s = check_problem(prob);
+  if (s != MPXKKT_OK) return s;
+  if (!prm) return fail(MPXKKT_ERR_ARG, "null params");
+  if ((s = check_iterate(prob, it)) != MPXKKT_OK) return s;
+  if (prm->max_trials < 1 || prm->max_trials > MAX_TRIALS)
+    return fail(MPXKKT_ERR_ARG, "max_trials must be in [1, 16]");
+  if (!(prm->beta > 0.0 && prm->beta < 1.0) || !(prm->tau > 0.0 && prm->tau < 1.0) || !(prm->mu_b > 0.0) ||
+      !(prm->rho_c >= 0.0))
+    return fail(MPXKKT_ERR_ARG, "line-search / barrier parameters out of range");
+  const size_t need = workspace_bytes(prob, prm);
+  if (!workspace || workspace_bytes_ < need)
+    return fail(MPXKKT_ERR_WORKSPACE, "workspace needs " + std::to_string(need) + " bytes");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (prob->dtype == MPXKKT_F64) return sqp_iterate_t<double>(st, prob, prm, it, stats, workspace);
+  return sqp_iterate_t<float>(st, prob, prm, it, stats, workspace);
+}
+
+mpxkkt_status mpxkkt_kkt_residual(void* stream, const mpxkkt_problem* prob, const mpxkkt_params* prm,
+                                  const mpxkkt_iterate* it, void* out, void* workspace,
+                                  size_t workspace_bytes_) {
+  mpxkkt_status s = check_problem(prob);
+  if (s != MPXKKT_OK) return s;
+  if (!prm || !out) return fail(MPXKKT_ERR_ARG, "null params or output");
+  if ((s = check_iterate(prob, it)) != MPXKKT_OK) return s;
+  const size_t need = workspace_bytes(prob, prm);
+  if (!workspace || workspace_bytes_ < need)
+    return fail(MPXKKT_ERR_WORKSPACE, "workspace needs " + std::to_string(need) + " bytes");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (prob->dtype == MPXKKT_F64) return kkt_t<double>(st, prob, prm, it, out, workspace);
+  return kkt_t<float>(st, prob, prm, it, out, workspace);
+}
+
+mpxkkt_status mpxkkt_model_eval(void* stream, const mpxkkt_problem* prob, const void* X, const void* U, void* g,
+                                void* c) {
+  mpxkkt_status s = check_problem(prob);
+  if (s != MPXKKT_OK) return s;
+  if (!X || !U || !g || !c) return fail(MPXKKT_ERR_ARG, "null array");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int nt = k1_threads(prob->n);
+  dim3 grid(prob->horizon + 1, prob->batch);
+  if (prob->dtype == MPXKKT_F64) {
+    hipLaunchKernelGGL(k_model_eval<double>, grid, dim3(nt), nt * sizeof(double), st, make_prob<double>(prob),
+                       (const double*)X, (const double*)U, (double*)g, (double*)c);
+  } else {
+    hipLaunchKernelGGL(k_model_eval<float>, grid, dim3(nt), nt * sizeof(float), st, make_prob<float>(prob),
+                       (const float*)X, (const float*)U, (float*)g, (float*)c);
+  }
+  return launch_check("k_model_eval");
+}
+
+mpxkkt_status mpxkkt_shift_warmstart(void* stream, const mpxkkt_problem* prob, int32_t shift, mpxkkt_iterate* it,
+                                     const void* u_reset, int32_t* reset) {
+  mpxkkt_status s = check_problem(prob);
+  if (s != MPXKKT_OK) return s;
+  if ((s = check_iterate(prob, it)) != MPXKKT_OK) return s;
+  if (shift < 0 || shift > prob->horizon) return fail(MPXKKT_ERR_ARG, "shift must be in [0, horizon]");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (prob->dtype == MPXKKT_F64) {
+    hipLaunchKernelGGL(k_shift<double>, dim3(prob->batch), dim3(64), 0, st, make_prob<double>(prob), shift,
+                       make_iter<double>(it), (const double*)u_reset, reset);
+  } else {
+    hipLaunchKernelGGL(k_shift<float>, dim3(prob->batch), dim3(64), 0, st, make_prob<float>(prob), shift,
+                       make_iter<float>(it), (const float*)u_reset, reset);
+  }
+  return launch_check("k_shift");
+}
+
+const char* mpxkkt_last_error(void) { return g_err.c_str(); }
+
+int32_t mpxkkt_version(void) { return 100; }
+
+}  // extern "C"

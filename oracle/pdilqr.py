@@ -42,8 +42,8 @@ D6  Equality rows D x + E u + e = 0 enter through an augmented Lagrangian with
     dnu_hi = mu_b/s_hi - nu_hi + nu_hi du / s_hi,
     dnu_lo = mu_b/s_lo - nu_lo - nu_lo du / s_lo;
     primal and dual steps are capped by fraction-to-boundary tau, and the merit
-    carries the barrier -mu_b sum log s.  A U outside the box on entry is first
-    pulled inside by clipping to [lo + 1e-3 w, hi - 1e-3 w], w = hi - lo.
+    carries the barrier -mu_b sum log s.  A coordinate of U on or outside the
+    box on entry is first moved to lo + 1e-3 w or hi - 1e-3 w, w = hi - lo.
 D7  The scan combine uses (I + C J)^{-1}; a leaf element combined with anything
     uses the rank-m Woodbury form through H = R + B^T J B.
 D8  KKT residual = 2-norm of (grad_x L, grad_u L, c, h, nu * s - mu_b) (the
@@ -143,14 +143,14 @@ def al_cost(inst, x, u, t, yeq, rho_c, mu_b):
 
 
 def box_interior(inst, U):
-    """U pulled strictly inside the box when it is not (D6)."""
+    """U with every coordinate outside the open box moved to lo + w / hi - w,
+    w = 1e-3 (hi - lo) (D6); coordinates already inside are untouched."""
     if not has_box(inst):
         return U
     lo, hi = inst["ulo"], inst["uhi"]
-    if np.all(U < hi) and np.all(U > lo):
-        return U
     w = 1e-3 * (hi - lo)
-    return np.clip(U, lo + w, hi - w)
+    U = np.where(U >= hi, hi - w, U)
+    return np.where(U <= lo, lo + w, U)
 
 
 def box_duals(inst, U, Ybox, mu_b):

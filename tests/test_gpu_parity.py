@@ -1,0 +1,128 @@
+"""GPU parity of the HIP SQP iteration against the FP64 oracle.
+
+FP64 on the device must reproduce the oracle's iterates to 1e-9 (same
+algorithm, only rounding differs); FP32 must stay within the stated
+tolerances of the FP64 oracle, iteration by iteration.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from mpx_amd import synthetic as syn  # noqa: E402
+from oracle import pdilqr as orc  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+# FP32 tolerances (relative to max(1, |oracle|)) per SQP iteration
+TOL32_X = 2e-4
+TOL32_KKT = 1e-5      # |eps_gpu - eps_fp64|, eps = kkt / max(1, kkt at the start)
+
+
+def _solver(insts, dtype, **tune):
+    from mpx_amd.problem import DeviceBatch, SQPSolver
+    batch = DeviceBatch.from_instances(insts, dtype=dtype)
+    return batch, SQPSolver(batch, **tune)
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(1.0, np.max(np.abs(b))))
+
+
+def _run(wl, B, iters, dtype, start=None, shape=None, **tune):
+    shape = shape or {}
+    insts = [syn.make_instance(wl, i, **shape) for i in range(B)]
+    start = start or syn.WORKLOADS.get(wl, syn.WORKLOADS[2])["start"]
+    starts = [syn.initial_iterate(d, start) for d in insts]
+    batch, solver = _solver(insts, dtype, **tune)
+    it = batch.iterate_from(*[np.stack([s[k] for s in starts]) for k in range(5)])
+    refs = [orc.solve(d, *s, iters) for d, s in zip(insts, starts)]
+    out = []
+    for k in range(iters):
+        st = solver.iterate(it)
+        torch.cuda.synchronize()
+        g = {key: it[key].double().cpu().numpy() for key in ("X", "U", "V", "Yeq", "Ybox")}
+        out.append((g, {key: v.cpu().numpy() for key, v in st.items()}))
+    return insts, starts, refs, out, solver, it
+
+
+@pytest.mark.parametrize("segments", [1, 3])
+@pytest.mark.parametrize("wl,shape,iters", [
+    (1, dict(n=6, m=3, N=9), 1),
+    (2, dict(n=8, m=4, N=12), 4),
+    (3, dict(n=8, m=4, N=10, p=2), 3),
+    (4, dict(n=8, m=6, N=8, p=2), 6),
+])
+def test_fp64_matches_oracle(wl, shape, iters, segments):
+    insts, starts, refs, out, _, _ = _run(wl, 3, iters, torch.float64, shape=shape, segments=segments)
+    for k in range(iters):
+        g, st = out[k]
+        for i in range(3):
+            r = refs[i][k]
+            assert abs(st["alpha"][i] - r["alpha"]) < 1e-12, (k, i, st["alpha"][i], r["alpha"])
+            for key in ("X", "U", "V"):
+                assert _rel(g[key][i], r[key]) < 1e-9, (k, i, key, _rel(g[key][i], r[key]))
+            if insts[i]["p"] > 0:
+                assert _rel(g["Yeq"][i], r["Yeq"]) < 1e-9
+            if insts[i].get("ulo") is not None:
+                assert _rel(g["Ybox"][i], r["Ybox"]) < 1e-9
+
+
+@pytest.mark.parametrize("segments", [1, 4])
+def test_fp32_workload3_shape(segments):
+    """n=36, m=12, N=50, p=8 (the benchmark shape), 3 iterations, FP32."""
+    insts, starts, refs, out, solver, it = _run(3, 4, 3, torch.float32, segments=segments)
+    for k in range(3):
+        g, st = out[k]
+        for i in range(4):
+            r = refs[i][k]
+            assert st["status"][i] == 0
+            assert _rel(g["X"][i], r["X"]) < TOL32_X, (k, i, _rel(g["X"][i], r["X"]))
+            assert _rel(g["U"][i], r["U"]) < TOL32_X * 10, (k, i, _rel(g["U"][i], r["U"]))
+
+
+def test_lq_exact_one_step_fp32():
+    """Workload 1 (LTV, n=m=12, N=50): one step reaches the optimum."""
+    insts, starts, refs, out, solver, it = _run(1, 2, 1, torch.float32)
+    g, st = out[0]
+    for i in range(2):
+        assert st["alpha"][i] == 1.0
+        assert _rel(g["X"][i], refs[i][0]["X"]) < TOL32_X
+        k0 = orc.kkt_residual(insts[i], *starts[i])
+        k_ref = orc.kkt_residual(insts[i], *[refs[i][0][key] for key in ("X", "U", "V", "Yeq", "Ybox")])
+        k_gpu = float(solver.kkt(it)[i].cpu())
+        assert abs(k_gpu - k_ref) / max(1.0, k0) < TOL32_KKT
+
+
+def test_kkt_and_model_eval_fp64():
+    insts = [syn.make_instance(4, i, n=8, m=6, N=8, p=2) for i in range(2)]
+    starts = [syn.initial_iterate(d, "zero") for d in insts]
+    batch, solver = _solver(insts, torch.float64)
+    it = batch.iterate_from(*[np.stack([s[k] for s in starts]) for k in range(5)])
+    for _ in range(3):
+        solver.iterate(it)
+    kk = solver.kkt(it).cpu().numpy()
+    g, c = solver.model_eval(it["X"], it["U"])
+    for i in range(2):
+        host = {k: it[k][i].cpu().numpy() for k in ("X", "U", "V", "Yeq", "Ybox")}
+        ref = orc.kkt_residual(insts[i], host["X"], host["U"], host["V"], host["Yeq"], host["Ybox"])
+        assert abs(kk[i] - ref) < 1e-9 * max(1.0, ref)
+        gr, cr = orc.model_evaluator_helper(insts[i], host["X"], host["U"])
+        assert _rel(g[i].cpu().numpy(), gr) < 1e-12 and _rel(c[i].cpu().numpy(), cr) < 1e-12
+
+
+def test_shift_and_nan_fallback():
+    insts = [syn.make_instance(2, i, n=5, m=2, N=6) for i in range(3)]
+    starts = [syn.initial_iterate(d, "rollout") for d in insts]
+    batch, solver = _solver(insts, torch.float64)
+    it = batch.iterate_from(*[np.stack([s[k] for s in starts]) for k in range(5)])
+    solver.iterate(it)
+    before = {k: it[k].cpu().numpy().copy() for k in ("X", "U", "V", "Ybox")}
+    it["U"][1, 0, 0] = float("nan")
+    reset = solver.shift(it, 2).cpu().numpy()
+    assert list(reset) == [0, 1, 0]
+    for i in (0, 2):
+        for k in ("X", "U", "V", "Ybox"):
+            assert np.array_equal(it[k][i].cpu().numpy(), orc.shift_warmstart(before[k][i], 2))
+    assert np.all(it["X"][1].cpu().numpy() == insts[1]["x0"][None])
+    assert np.all(it["U"][1].cpu().numpy() == 0) and np.all(it["V"][1].cpu().numpy() == 0)
