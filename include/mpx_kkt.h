@@ -99,7 +99,7 @@ typedef struct {
   int32_t segments;       /* time-parallel scan segments per instance (1 = sequential Riccati) */
   int32_t block_threads;  /* threads per block of the per-instance kernels (0 = auto) */
   int32_t trial_group;    /* trials evaluated together (0 = all); 1 if limited_memory */
-  int32_t reserved;
+  int32_t flags;          /* bit 0: FP32 products on VALU instead of the matrix cores */
 } mpxkkt_params;
 
 /* Per-instance outputs of one SQP iteration (each may be NULL). */
@@ -145,6 +145,13 @@ mpxkkt_status mpxkkt_model_eval(void* stream, const mpxkkt_problem* prob, const 
 mpxkkt_status mpxkkt_shift_warmstart(void* stream, const mpxkkt_problem* prob, int32_t shift,
                                      mpxkkt_iterate* it, const void* u_reset,
                                      int32_t* reset);
+
+/* Phase timing.  While enabled, every mpxkkt_sqp_iterate records HIP events on
+ * its stream at the phase boundaries (linearise | LQR | line search).
+ * mpxkkt_profile_collect waits for the recorded events, returns the summed
+ * milliseconds per phase and the number of calls, and clears the record. */
+mpxkkt_status mpxkkt_profile_enable(int32_t on);
+mpxkkt_status mpxkkt_profile_collect(double ms_per_phase[3], int32_t* calls);
 
 const char* mpxkkt_last_error(void);
 int32_t mpxkkt_version(void);

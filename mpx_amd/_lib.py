@@ -22,6 +22,8 @@ EXPORTED = (
     "mpxkkt_kkt_residual",
     "mpxkkt_model_eval",
     "mpxkkt_shift_warmstart",
+    "mpxkkt_profile_enable",
+    "mpxkkt_profile_collect",
     "mpxkkt_last_error",
     "mpxkkt_version",
 )
@@ -56,7 +58,7 @@ class Params(C.Structure):
         ("mu_b", C.c_double), ("tau", C.c_double), ("beta", C.c_double), ("c1", C.c_double),
         ("armijo_slack", C.c_double),
         ("max_trials", C.c_int32), ("limited_memory", C.c_int32), ("segments", C.c_int32),
-        ("block_threads", C.c_int32), ("trial_group", C.c_int32), ("reserved", C.c_int32),
+        ("block_threads", C.c_int32), ("trial_group", C.c_int32), ("flags", C.c_int32),
     ]
 
 
@@ -92,6 +94,8 @@ def load():
                                       C.c_void_p]
     lib.mpxkkt_shift_warmstart.argtypes = [C.c_void_p, P(Problem), C.c_int32, P(Iterate),
                                            C.c_void_p, C.c_void_p]
+    lib.mpxkkt_profile_enable.argtypes = [C.c_int32]
+    lib.mpxkkt_profile_collect.argtypes = [P(C.c_double * 3), P(C.c_int32)]
     lib.mpxkkt_last_error.restype = C.c_char_p
     lib.mpxkkt_version.restype = C.c_int32
     for name in EXPORTED[:-2]:
@@ -112,3 +116,15 @@ def default_params(**over):
     for k, v in over.items():
         setattr(p, k, v)
     return p
+
+
+def profile_enable(on=True):
+    check(load().mpxkkt_profile_enable(1 if on else 0))
+
+
+def profile_collect():
+    """(ms per phase [linearise, LQR, line search] summed over calls, calls)."""
+    ms = (C.c_double * 3)()
+    calls = C.c_int32(0)
+    check(load().mpxkkt_profile_collect(C.byref(ms), C.byref(calls)))
+    return [ms[0], ms[1], ms[2]], calls.value

@@ -26,6 +26,7 @@ template <typename T>
 struct Prm {
   T rho0, rho_max, rho_c, mu_b, tau, beta, c1, slack;
   int max_trials, group, P;
+  int flags;  // bit 0: form FP32 products on VALU instead of the matrix cores
 };
 
 template <typename T>
@@ -35,8 +36,8 @@ struct Iter {
 
 // per-stage merit partials written by the linearisation kernel
 enum { MP_COST = 0, MP_VC = 1, MP_CC = 2, NMP = 3 };
-// per-segment partials of the forward pass
-enum { FP_DF = 0, FP_DC = 1, FP_KAP = 2, NFP = 4 };
+// per-segment partials of the forward pass: q'dx + r'du, (dV - V)'c, lam'c
+enum { FP_DF = 0, FP_DC = 1, FP_LC = 2, NFP = 4 };
 
 constexpr int MAX_TRIALS = 16;
 
@@ -52,11 +53,13 @@ struct Ws {
   T* s;      // [B][N1][n]     value gradients
   T* K;      // [B][N][m][n]   feedback gains (du = -K dx - k)
   T* k;      // [B][N][m]
+  T* F;      // [B][N][n][n]   closed-loop maps A - B K
+  T* f;      // [B][N][n]      offsets c - B k
   T* dx;     // [B][N1][n]
   T* du;     // [B][N][m]
   T* dv;     // [B][N1][n]
   T* mpart;  // [B][N1][NMP]
-  T* fpart;  // [B][P][NFP]
+  T* fpart;  // [B][N1][NFP]   per-stage slope partials
   T* agg;    // [B][P][3n^2+2n] segment aggregates (A, C, J, b, eta)
   T* fmap;   // [B][P][n^2+n]   segment forward maps (Phi, phi)
   T* sval;   // [B][P][n^2+n]   values (S, s) at segment starts
@@ -198,6 +201,250 @@ __device__ void trsm_upper_t(const T* L, int m, T* G, int ncol, int ld) {
       for (int kk = i + 1; kk < m; ++kk) v -= L[kk * m + i] * G[kk * ld + c];
       G[i * ld + c] = v / L[i * m + i];
     }
+  }
+}
+
+}  // namespace mpxk
+
+namespace mpxk {
+
+// ---------------------------------------------------------------------------
+// FP32 matrix-core GEMM on LDS operands (v_mfma_f32_16x16x4f32, wave64).
+// Operand layouts of the instruction: A (16x4) lane l holds A[l%16][l/16],
+// B (4x16) lane l holds B[l/16][l%16], result lane l holds D[4(l/16)+v][l%16].
+// The product is a sum of up to 3 K-segments (each op(A_s) op(B_s) scaled by
+// sign_s), so rank updates fold into one pass; waves of the block take the
+// 16x16 output tiles round-robin.  With SYM only tiles ti <= tj are formed
+// and every value goes to (i, j) and (j, i) (the result must be symmetric).
+// At store: C = [C +] acc + addend[i*lda_add + j] + (i == j ? diag[i] : 0).
+// ---------------------------------------------------------------------------
+typedef float mpxk_f32x4 __attribute__((ext_vector_type(4)));
+
+struct MSeg {
+  const float* A;
+  const float* B;
+  int lda, ldb, K;
+  bool ta, tb;
+  float sign;
+};
+
+// acc += sign * op(A)[i, :] op(B)[:, j] over one K-segment; operands of four
+// k-steps are loaded before their four MFMAs so LDS latency overlaps.
+__device__ __forceinline__ void mfma_seg(mpxk_f32x4& acc, const MSeg& g, int i, int j, int M, int N, int q) {
+  const bool iok = i < M, jok = j < N;
+  const int sak = g.ta ? g.lda : 1, sai = g.ta ? 1 : g.lda;
+  const int sbk = g.tb ? 1 : g.ldb, sbj = g.tb ? g.ldb : 1;
+  const float* pa = g.A + (iok ? i : 0) * sai;
+  const float* pb = g.B + (jok ? j : 0) * sbj;
+  for (int k0 = 0; k0 < g.K; k0 += 16) {
+    float a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + 4 * u + q;
+      const bool kok = k < g.K;
+      a[u] = (iok && kok) ? pa[k * sak] * g.sign : 0.f;
+      b[u] = (jok && kok) ? pb[k * sbk] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (k0 + 4 * u < g.K) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u], acc, 0, 0, 0);
+  }
+}
+
+// tile index -> (ti, tj); with SYM only the upper triangle of tiles is enumerated
+template <bool SYM>
+__device__ __forceinline__ void tile_of(int tile, int tn, int* ti, int* tj) {
+  if (!SYM) { *ti = tile / tn; *tj = tile - *ti * tn; return; }
+  int r = 0, rem = tile;
+  while (rem >= tn - r) { rem -= tn - r; ++r; }
+  *ti = r; *tj = r + rem;
+}
+
+template <bool SYM>
+__device__ __forceinline__ void mfma_store(const mpxk_f32x4& acc, const float* add, int ti, int tj, int M, int N,
+                                           int q, int r, float* C, int ldc) {
+  const int cj = tj * 16 + r;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int ci = ti * 16 + 4 * q + v;
+    if (ci >= M || cj >= N) continue;
+    if (SYM && ti == tj && ci > cj) continue;
+    const float val = acc[v] + add[v];
+    C[ci * ldc + cj] = val;
+    if (SYM && ci != cj) C[cj * ldc + ci] = val;
+  }
+}
+
+template <bool SYM>
+__device__ __forceinline__ void mfma_addend(float* add, int ti, int tj, int M, int N, int q, int r,
+                                            const float* addend, int ld_add, const float* diag) {
+  const int cj = tj * 16 + r;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int ci = ti * 16 + 4 * q + v;
+    add[v] = 0.f;
+    if (ci < M && cj < N) {
+      if (addend) add[v] = addend[ci * ld_add + cj];
+      if (diag && ci == cj) add[v] += diag[ci];
+    }
+  }
+}
+
+// C = sum_s sign_s op(A_s) op(B_s) + addend + diag, 16x16 tiles, two tiles per
+// wave at a time (independent accumulators keep the matrix pipe busy).
+template <bool SYM>
+__device__ void mfma_gemm(int M, int N, int nseg, MSeg s0, MSeg s1, MSeg s2, float* C, int ldc,
+                          const float* addend, int ld_add, const float* diag) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int tm = (M + 15) >> 4, tn = (N + 15) >> 4;
+  const int ntile = SYM ? tn * (tn + 1) / 2 : tm * tn;
+  const int r = lane & 15, q = lane >> 4;
+  for (int t0 = wave; t0 < ntile; t0 += 2 * nw) {
+    const int t1 = t0 + nw;
+    const bool two = t1 < ntile;
+    int ti0, tj0, ti1 = 0, tj1 = 0;
+    tile_of<SYM>(t0, tn, &ti0, &tj0);
+    if (two) tile_of<SYM>(t1, tn, &ti1, &tj1);
+    float add0[4], add1[4];
+    mfma_addend<SYM>(add0, ti0, tj0, M, N, q, r, addend, ld_add, diag);
+    if (two) mfma_addend<SYM>(add1, ti1, tj1, M, N, q, r, addend, ld_add, diag);
+    mpxk_f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    const int i0 = ti0 * 16 + r, j0 = tj0 * 16 + r, i1 = ti1 * 16 + r, j1 = tj1 * 16 + r;
+    mfma_seg(acc0, s0, i0, j0, M, N, q);
+    if (two) mfma_seg(acc1, s0, i1, j1, M, N, q);
+    if (nseg > 1) {
+      mfma_seg(acc0, s1, i0, j0, M, N, q);
+      if (two) mfma_seg(acc1, s1, i1, j1, M, N, q);
+    }
+    if (nseg > 2) {
+      mfma_seg(acc0, s2, i0, j0, M, N, q);
+      if (two) mfma_seg(acc1, s2, i1, j1, M, N, q);
+    }
+    mfma_store<SYM>(acc0, add0, ti0, tj0, M, N, q, r, C, ldc);
+    if (two) mfma_store<SYM>(acc1, add1, ti1, tj1, M, N, q, r, C, ldc);
+  }
+}
+
+__device__ inline void wave_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+
+// value of v in lane `src` (uniform), for 32- and 64-bit types
+__device__ __forceinline__ float rlane(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ __forceinline__ double rlane(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// In-place inverse of the m x m SPD matrix H (LDS, row-major), m <= MMAX <= 64,
+// by Gauss-Jordan elimination without pivoting (stable for SPD) in wave 0,
+// lane i holding row i of [H | I] in registers; pivot rows are broadcast with
+// readlane.  Other waves return at once; the caller syncs the block.
+template <typename T, int MMAX>
+__device__ void spd_inverse_wave(T* H, int m) {
+  if (threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
+  T a[MMAX], bi[MMAX];
+#pragma unroll
+  for (int k = 0; k < MMAX; ++k) {
+    a[k] = (lane < m && k < m) ? H[lane * m + k] : T(lane == k ? 1 : 0);
+    bi[k] = T(lane == k ? 1 : 0);
+  }
+#pragma unroll
+  for (int k = 0; k < MMAX; ++k) {
+    if (k < m) {
+      const T piv = rlane(a[k], k);
+      const T f = lane == k ? T(0) : a[k] / piv;   // multiple of the pivot row to remove
+      const T sc = T(1) / piv;
+#pragma unroll
+      for (int j = 0; j < MMAX; ++j) {
+        if (j < m) {
+          const T pa = rlane(a[j], k), pb = rlane(bi[j], k);
+          if (lane == k) { a[j] = pa * sc; bi[j] = pb * sc; }
+          else { a[j] -= f * pa; bi[j] -= f * pb; }
+        }
+      }
+    }
+  }
+  if (lane < m) {
+#pragma unroll
+    for (int k = 0; k < MMAX; ++k)
+      if (k < m) H[lane * m + k] = bi[k];
+  }
+}
+
+// Lower Cholesky of the m x m SPD matrix H (LDS, row-major), m <= MMAX <= 64,
+// by wave 0 with the rows in registers (lane i holds row i); the factor is
+// written back to H.  Other waves return at once; the caller syncs the block.
+template <typename T, int MMAX>
+__device__ void chol_wave(T* H, int m) {
+  if (threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
+  T h[MMAX];
+#pragma unroll
+  for (int k = 0; k < MMAX; ++k) h[k] = (lane < m && k < m) ? H[lane * m + k] : T(lane == k ? 1 : 0);
+#pragma unroll
+  for (int j = 0; j < MMAX; ++j) {
+    if (j < m) {
+      const T d = dev_sqrt(rlane(h[j], j));
+      h[j] = lane == j ? d : (lane > j ? h[j] / d : h[j]);
+#pragma unroll
+      for (int k = j + 1; k < MMAX; ++k) {
+        if (k < m) {
+          const T lkj = rlane(h[j], k);
+          if (lane >= k) h[k] -= h[j] * lkj;
+        }
+      }
+    }
+  }
+  if (lane < m) {
+#pragma unroll
+    for (int k = 0; k < MMAX; ++k)
+      if (k < m) H[lane * m + k] = k <= lane ? h[k] : T(0);
+  }
+}
+
+// L Y = G (FWD) or L^T X = Y for the columns of G (m x ncol, ld) in place, by
+// wave 0; each lane owns columns lane, lane+64, ...; L entries are uniform LDS reads.
+template <typename T, int MMAX, bool FWD>
+__device__ void trsm_wave(const T* L, int m, T* G, int ncol, int ld) {
+  if (threadIdx.x >= 64) return;
+  for (int c = threadIdx.x; c < ncol; c += 64) {
+    T y[MMAX];
+#pragma unroll
+    for (int i = 0; i < MMAX; ++i) y[i] = i < m ? G[i * ld + c] : T(0);
+    if (FWD) {
+#pragma unroll
+      for (int i = 0; i < MMAX; ++i) {
+        if (i < m) {
+          T v = y[i];
+#pragma unroll
+          for (int k = 0; k < i; ++k) v -= L[i * m + k] * y[k];
+          y[i] = v / L[i * m + i];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = MMAX - 1; i >= 0; --i) {
+        if (i < m) {
+          T v = y[i];
+#pragma unroll
+          for (int k = i + 1; k < MMAX; ++k)
+            if (k < m) v -= L[k * m + i] * y[k];
+          y[i] = v / L[i * m + i];
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < MMAX; ++i)
+      if (i < m) G[i * ld + c] = y[i];
   }
 }
 

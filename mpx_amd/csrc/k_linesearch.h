@@ -96,7 +96,7 @@ __device__ T merit_item(const Prob<T>& P, const Prm<T>& prm, const Iter<T>& it, 
 }
 
 template <typename T>
-__global__ void k_linesearch(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, LqrIn<T> in, T* o_alpha,
+__global__ void __launch_bounds__(512) k_linesearch(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, LqrIn<T> in, T* o_alpha,
                              T* o_rho, T* o_phi0, T* o_dphi, int* o_status, int* o_trials) {
   const int b = blockIdx.x;
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -111,14 +111,15 @@ __global__ void k_linesearch(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, LqrIn<
     v6[1] += mp[MP_VC];
     v6[2] += mp[MP_CC];
   }
-  for (int s = tid; s < prm.P; s += nt) {
-    const T* fp = ws.fpart + ((size_t)b * prm.P + s) * NFP;
+  for (int s = tid; s < N1; s += nt) {
+    const T* fp = ws.fpart + ((size_t)b * N1 + s) * NFP;
     v6[3] += fp[FP_DF];
     v6[4] += fp[FP_DC];
-    v6[5] += fp[FP_KAP];
+    v6[5] += fp[FP_LC];
   }
   block_sum<T, 6>(v6, 6, sred);
-  const T cost = v6[0], vc = v6[1], cc = v6[2], df = v6[3], dc = v6[4], kap = v6[5];
+  // curvature dz'H dz = sum lam'c - d_f (stationarity of the LQR step)
+  const T cost = v6[0], vc = v6[1], cc = v6[2], df = v6[3], dc = v6[4], kap = v6[5] - v6[3];
   T rho = prm.rho0;
   if (cc > T(0)) {
     const T r2 = T(2) * (df + dc + T(0.5) * kap) / cc;

@@ -6,14 +6,12 @@
 // [rho_c E'D, R + diag(rd) + rho_c E'E]] (the AL rows never materialise).
 //
 // Backward: value V_t(x) = x'S_t x/2 + s_t'x, by sequential Riccati steps
-// (segments = 1) or by the chunked associative scan (segments > 1): every
-// segment but the last reduces its stages to one element (A, b, C, eta, J)
-// with leaf combines, the last runs Riccati from the terminal cost, the suffix
-// values at segment starts follow from value combines, and every other segment
-// then runs Riccati from its suffix value.  Forward: dx_0 = c_init,
-// du = -K dx - k, dx' = A dx + B du + c, costates lam = S dx + s,
-// dV = lam - V; again sequential or chunked (segment maps composed, states at
-// segment starts propagated, then each segment rolled out).
+// (segments = 1) or by the chunked associative scan (segments > 1, k_scan.h).
+// Each step also stores the closed-loop map F_t = A_t - B_t K_t and
+// f_t = c_t - B_t k_t, so the forward pass is the affine recursion
+// dx_{t+1} = F_t dx_t + f_t (sequential, or chunked by segment maps), and the
+// costates lam_t = S_t dx_t + s_t, du_t = -K_t dx_t - k_t, dV_t = lam_t - V_t
+// and the merit slope partials follow in one pass parallel over all stages.
 #pragma once
 #include "common.h"
 
@@ -24,28 +22,31 @@ struct LqrIn {
   View<T> A, Bm;
 };
 
-// LDS carve-up of the Riccati / combine working set.
+// LDS carve-up of the Riccati working set.  A and B of the stage sit side by
+// side in AB (n x (n+m)); the product S [A | B] lands in TB = [T1 | SB].
 template <typename T>
 struct RicSmem {
-  T *S, *A, *T1, *B, *SB, *G, *H, *D, *E, *s, *Sc, *c, *rd, *q, *r, *red;
+  T *S, *A, *T1, *B, *SB, *G, *Kb, *H, *D, *E, *s, *Sc, *c, *rd, *q, *r, *red;
+  int ldab;
 };
 
 template <typename T>
 __host__ __device__ inline size_t ric_smem_elems(int n, int m, int p, int nt) {
-  return (size_t)3 * n * n + 2 * (size_t)n * m + (size_t)m * (n + 1) + (size_t)m * m +
-         (size_t)p * (n + m) + 4 * (size_t)n + 2 * (size_t)m + 2 * (size_t)n + 4 * (size_t)nt + 8;
+  const size_t kb = (size_t)m * (n + 1 > m ? n + 1 : m);  // [K | k], or m x m scratch
+  return (size_t)n * n + 2 * (size_t)n * (n + m) + (size_t)m * (n + 1) + kb + (size_t)m * m +
+         (size_t)p * (n + m) + 6 * (size_t)n + 2 * (size_t)m + 4 * (size_t)nt + 8;
 }
 
 template <typename T>
 __device__ RicSmem<T> ric_carve(T* base, int n, int m, int p, int nt) {
   RicSmem<T> L;
   T* q = base;
+  L.ldab = n + m;
   L.S = q; q += n * n;
-  L.A = q; q += n * n;
-  L.T1 = q; q += n * n;
-  L.B = q; q += n * m;
-  L.SB = q; q += n * m;
+  L.A = q; L.B = q + n; q += n * (n + m);
+  L.T1 = q; L.SB = q + n; q += n * (n + m);
   L.G = q; q += m * (n + 1);
+  L.Kb = q; q += m * (n + 1 > m ? n + 1 : m);
   L.H = q; q += m * m;
   L.D = q; q += p * n;
   L.E = q; q += p * m;
@@ -55,93 +56,170 @@ __device__ RicSmem<T> ric_carve(T* base, int n, int m, int p, int nt) {
   L.q = q; q += n;
   L.rd = q; q += m;
   L.r = q; q += m;
-  q += 2 * n;  // spare vectors (forward pass)
+  q += 2 * n;
   L.red = q;
   return L;
 }
 
-// Load the stage's QP data into LDS (A, B, D, E, c, q, r, rd).
+// Load the stage's QP data into LDS: [A | B], sqrt(rho_c) D, sqrt(rho_c) E, c, q, r, rd.
 template <typename T>
-__device__ void load_stage(const Prob<T>& P, const Ws<T>& ws, const LqrIn<T>& in, int b, int t,
-                           RicSmem<T>& L) {
-  const int n = P.n, m = P.m, p = P.p, N1 = P.N + 1;
-  load_block(L.A, in.A.at(b, t), n, n, n);
-  load_block(L.B, in.Bm.at(b, t), n, m, m);
+__device__ void load_stage(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& ws, const LqrIn<T>& in, int b,
+                           int t, RicSmem<T>& L) {
+  const int n = P.n, m = P.m, p = P.p, N1 = P.N + 1, ldab = L.ldab;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const T* A = in.A.at(b, t);
+  const T* B = in.Bm.at(b, t);
+  for (int i = 0; i < n; ++i) {
+    for (int j = tid; j < ldab; j += nt) L.A[i * ldab + j] = j < n ? A[i * n + j] : B[i * m + (j - n)];
+  }
   if (p > 0) {
-    load_block(L.D, P.D.at(b, t), p, n, n);
-    load_block(L.E, P.E.at(b, t), p, m, m);
+    const T sr = dev_sqrt(prm.rho_c);
+    const T* D = P.D.at(b, t);
+    const T* E = P.E.at(b, t);
+    for (int e = tid; e < p * n; e += nt) L.D[e] = sr * D[e];
+    for (int e = tid; e < p * m; e += nt) L.E[e] = sr * E[e];
   }
   const T* c = ws.cdef + ((size_t)b * N1 + t + 1) * n;
   const T* q = ws.qh + ((size_t)b * N1 + t) * n;
   const T* r = ws.rh + ((size_t)b * P.N + t) * m;
   const T* rd = ws.rd + ((size_t)b * P.N + t) * m;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) { L.c[i] = c[i]; L.q[i] = q[i]; }
-  for (int i = threadIdx.x; i < m; i += blockDim.x) { L.r[i] = r[i]; L.rd[i] = rd[i]; }
+  for (int i = tid; i < n; i += nt) { L.c[i] = c[i]; L.q[i] = q[i]; }
+  for (int i = tid; i < m; i += nt) { L.r[i] = r[i]; L.rd[i] = rd[i]; }
 }
 
-// One backward Riccati step in LDS: (S, s) = V_{t+1} -> V_t, and K_t, k_t in
-// G (m x (n+1); column n holds k).  The stage data must already be loaded.
 template <typename T>
-__device__ void riccati_step(const Prob<T>& P, const Prm<T>& prm, int b, int t, RicSmem<T>& L) {
-  const int n = P.n, m = P.m, p = P.p, ld = n + 1;
+struct IsF32 { static constexpr bool value = false; };
+template <>
+struct IsF32<float> { static constexpr bool value = true; };
+
+// H <- H^{-1} for the SPD m x m matrix in LDS (block-wide; wave 0 registers for m <= 16).
+template <typename T, int MMAX>
+__device__ void spd_inverse(T* H, T* tmp, int m) {
+  if constexpr (MMAX <= 16) {
+    spd_inverse_wave<T, MMAX>(H, m);
+  } else {
+    // Cholesky, then the identity solved through both triangles
+    chol_lds(H, m);
+    for (int e = threadIdx.x; e < m * m; e += blockDim.x) tmp[e] = (e / m == e % m) ? T(1) : T(0);
+    __syncthreads();
+    trsm_lower(H, m, tmp, m, m);
+    __syncthreads();
+    trsm_upper_t(H, m, tmp, m, m);
+    __syncthreads();
+    for (int e = threadIdx.x; e < m * m; e += blockDim.x) H[e] = tmp[e];
+  }
+}
+
+// One backward Riccati step in LDS: (S, s) = V_{t+1} -> V_t; [K | k] in Kb;
+// F_t = A - B K and f_t = c - B k go straight to the workspace.
+//   [T1 | SB] = S [A | B],  Sc = S c + s
+//   H = R + diag(rd) + rho_c E'E + B'SB,  [G | g] = [rho_c E'D + B'T1 | r + B'Sc]
+//   [K | k] = H^{-1} [G | g]
+//   S_t = Q + rho_c D'D + A'T1 - G'K,  s_t = q + A'Sc - G'k
+// FP32 forms the products on the matrix cores (unless flags bit 0), FP64 on VALU.
+template <typename T, int MMAX>
+__device__ void riccati_step(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& ws, int b, int t,
+                             RicSmem<T>& L) {
+  const int n = P.n, m = P.m, p = P.p, N = P.N, ld = n + 1, ldab = L.ldab;
   const int tid = threadIdx.x, nt = blockDim.x;
-  const T rc = prm.rho_c;
-  // Sc = S c + s ; T1 = S A ; SB = S B
+  const T* R = P.R.at(b, t);
+  const T* Q = P.Q.at(b, t);
+  T* Fo = ws.F + ((size_t)b * N + t) * n * n;
+  T* fo = ws.f + ((size_t)b * N + t) * n;
+  bool mfma = false;
+  if constexpr (IsF32<T>::value) mfma = (prm.flags & 1) == 0;
+  // A: Sc = S c + s ; [T1 | SB] = S [A | B]
   for (int i = tid; i < n; i += nt) {
     T v = L.s[i];
     for (int j = 0; j < n; ++j) v += L.S[i * n + j] * L.c[j];
     L.Sc[i] = v;
   }
-  gemm_lds<T, false, false, 0>(n, n, n, L.S, n, L.A, n, L.T1, n);
-  gemm_lds<T, false, false, 0>(n, m, n, L.S, n, L.B, m, L.SB, m);
-  // H = R + diag(rd) + rc E'E ; G = rc E'D ; g = r
-  const T* R = P.R.at(b, t);
-  for (int e = tid; e < m * m; e += nt) {
-    const int i = e / m, j = e - i * m;
-    T v = R[e] + (i == j ? L.rd[i] : T(0));
-    for (int k = 0; k < p; ++k) v += rc * L.E[k * m + i] * L.E[k * m + j];
-    L.H[e] = v;
+  if constexpr (IsF32<T>::value) {
+    if (mfma) {
+      const MSeg sg{L.S, L.A, n, ldab, n, false, false, 1.f};
+      mfma_gemm<false>(n, n + m, 1, sg, sg, sg, L.T1, ldab, nullptr, 0, nullptr);
+    }
   }
-  for (int e = tid; e < m * n; e += nt) {
-    const int i = e / n, j = e - i * n;
-    T v = 0;
-    for (int k = 0; k < p; ++k) v += L.E[k * m + i] * L.D[k * n + j];
-    L.G[i * ld + j] = rc * v;
-  }
-  for (int i = tid; i < m; i += nt) L.G[i * ld + n] = L.r[i];
+  if (!mfma) gemm_lds<T, false, false, 0>(n, n + m, n, L.S, n, L.A, ldab, L.T1, ldab);
   __syncthreads();
-  // H += B'SB ; G += B'T1 ; g += B'Sc
-  gemm_lds<T, true, false, 1>(m, m, n, L.B, m, L.SB, m, L.H, m);
-  gemm_lds<T, true, false, 1>(m, n, n, L.B, m, L.T1, n, L.G, ld);
+  // B: H, [G | g]
+  if constexpr (IsF32<T>::value) {
+    if (mfma) {
+      const MSeg h0{L.B, L.SB, ldab, ldab, n, true, false, 1.f}, h1{L.E, L.E, m, m, p, true, false, 1.f};
+      mfma_gemm<false>(m, m, p > 0 ? 2 : 1, h0, h1, h1, L.H, m, R, m, L.rd);
+      const MSeg g0{L.B, L.T1, ldab, ldab, n, true, false, 1.f}, g1{L.E, L.D, m, n, p, true, false, 1.f};
+      mfma_gemm<false>(m, n, p > 0 ? 2 : 1, g0, g1, g1, L.G, ld, nullptr, 0, nullptr);
+    }
+  }
+  if (!mfma) {
+    for (int e = tid; e < m * m; e += nt) {
+      const int i = e / m, j = e - i * m;
+      T v = R[e] + (i == j ? L.rd[i] : T(0));
+      for (int k = 0; k < p; ++k) v += L.E[k * m + i] * L.E[k * m + j];
+      for (int k = 0; k < n; ++k) v += L.B[k * ldab + i] * L.SB[k * ldab + j];
+      L.H[e] = v;
+    }
+    for (int e = tid; e < m * n; e += nt) {
+      const int i = e / n, j = e - i * n;
+      T v = 0;
+      for (int k = 0; k < p; ++k) v += L.E[k * m + i] * L.D[k * n + j];
+      for (int k = 0; k < n; ++k) v += L.B[k * ldab + i] * L.T1[k * ldab + j];
+      L.G[i * ld + j] = v;
+    }
+  }
   for (int i = tid; i < m; i += nt) {
-    T v = 0;
-    for (int k = 0; k < n; ++k) v += L.B[k * m + i] * L.Sc[k];
-    L.G[i * ld + n] += v;
+    T v = L.r[i];
+    for (int k = 0; k < n; ++k) v += L.B[k * ldab + i] * L.Sc[k];
+    L.G[i * ld + n] = v;
   }
   __syncthreads();
-  chol_lds(L.H, m);
-  trsm_lower(L.H, m, L.G, n + 1, ld);  // Y = L^{-1} [G | g]
+  // C: H <- H^{-1}
+  spd_inverse<T, MMAX>(L.H, L.Kb, m);
   __syncthreads();
-  // S_t = Q + rc D'D + A'T1 - Y'Y (upper triangle mirrored) ; s_t = q + A'Sc - Y'y
-  const T* Q = P.Q.at(b, t);
-  for (int e = tid; e < n * n; e += nt) {
-    const int i = e / n, j = e - i * n;
-    if (i > j) continue;
-    T v = Q[e];
-    for (int k = 0; k < p; ++k) v += rc * L.D[k * n + i] * L.D[k * n + j];
-    for (int k = 0; k < n; ++k) v += L.A[k * n + i] * L.T1[k * n + j];
-    for (int r = 0; r < m; ++r) v -= L.G[r * ld + i] * L.G[r * ld + j];
-    L.S[i * n + j] = v;
-    L.S[j * n + i] = v;
+  // D: [K | k] = H^{-1} [G | g]
+  if constexpr (IsF32<T>::value) {
+    if (mfma) {
+      const MSeg k0{L.H, L.G, m, ld, m, false, false, 1.f};
+      mfma_gemm<false>(m, n + 1, 1, k0, k0, k0, L.Kb, ld, nullptr, 0, nullptr);
+    }
+  }
+  if (!mfma) gemm_lds<T, false, false, 0>(m, n + 1, m, L.H, m, L.G, ld, L.Kb, ld);
+  __syncthreads();
+  // E: S_t, s_t ; F_t = A - B K and f_t = c - B k (to the workspace)
+  if constexpr (IsF32<T>::value) {
+    if (mfma) {
+      const MSeg s0{L.A, L.T1, ldab, ldab, n, true, false, 1.f};
+      const MSeg s1{L.G, L.Kb, ld, ld, m, true, false, -1.f};
+      const MSeg s2{L.D, L.D, n, n, p, true, false, 1.f};
+      mfma_gemm<true>(n, n, p > 0 ? 3 : 2, s0, s1, s2, L.S, n, Q, n, nullptr);
+      const MSeg f0{L.B, L.Kb, ldab, ld, m, false, false, -1.f};
+      mfma_gemm<false>(n, n, 1, f0, f0, f0, Fo, n, L.A, ldab, nullptr);
+    }
+  }
+  if (!mfma) {
+    for (int e = tid; e < n * n; e += nt) {
+      const int i = e / n, j = e - i * n;
+      T fv = L.A[i * ldab + j];
+      for (int r = 0; r < m; ++r) fv -= L.B[i * ldab + r] * L.Kb[r * ld + j];
+      Fo[e] = fv;
+      if (i > j) continue;
+      T v = Q[e];
+      for (int k = 0; k < p; ++k) v += L.D[k * n + i] * L.D[k * n + j];
+      for (int k = 0; k < n; ++k) v += L.A[k * ldab + i] * L.T1[k * ldab + j];
+      for (int r = 0; r < m; ++r) v -= L.G[r * ld + i] * L.Kb[r * ld + j];
+      L.S[i * n + j] = v;
+      L.S[j * n + i] = v;
+    }
   }
   for (int i = tid; i < n; i += nt) {
     T v = L.q[i];
-    for (int k = 0; k < n; ++k) v += L.A[k * n + i] * L.Sc[k];
-    for (int r = 0; r < m; ++r) v -= L.G[r * ld + i] * L.G[r * ld + n];
+    for (int k = 0; k < n; ++k) v += L.A[k * ldab + i] * L.Sc[k];
+    for (int r = 0; r < m; ++r) v -= L.G[r * ld + i] * L.Kb[r * ld + n];
     L.s[i] = v;
+    T w = L.c[i];
+    for (int r = 0; r < m; ++r) w -= L.B[i * ldab + r] * L.Kb[r * ld + n];
+    fo[i] = w;
   }
-  __syncthreads();
-  trsm_upper_t(L.H, m, L.G, n + 1, ld);  // [K | k] = L^{-T} Y
   __syncthreads();
 }
 
@@ -154,123 +232,152 @@ __device__ void store_value_and_gain(const Prob<T>& P, const Ws<T>& ws, int b, i
   if (with_gain) {
     T* K = ws.K + ((size_t)b * P.N + t) * m * n;
     T* k = ws.k + ((size_t)b * P.N + t) * m;
-    for (int e = threadIdx.x; e < m * n; e += blockDim.x) {
-      const int i = e / n, j = e - i * n;
-      K[e] = L.G[i * ld + j];
-    }
-    for (int i = threadIdx.x; i < m; i += blockDim.x) k[i] = L.G[i * ld + n];
+    for (int i = 0; i < m; ++i)
+      for (int j = threadIdx.x; j < n; j += blockDim.x) K[i * n + j] = L.Kb[i * ld + j];
+    for (int i = threadIdx.x; i < m; i += blockDim.x) k[i] = L.Kb[i * ld + n];
   }
 }
 
 // Riccati over stages [t_lo, t_hi) (t_hi <= N) starting from the value already in
 // (L.S, L.s) at index t_hi; stores S, s for t in [t_lo, t_hi) (and t_hi when
-// store_top) and K, k for t < t_hi.
-template <typename T>
+// store_top), K, k, F, f for t < t_hi.
+template <typename T, int MMAX>
 __device__ void riccati_range(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& ws, const LqrIn<T>& in,
                               int b, int t_lo, int t_hi, RicSmem<T>& L, bool store_top) {
   if (store_top) store_value_and_gain(P, ws, b, t_hi, L, false);
   for (int t = t_hi - 1; t >= t_lo; --t) {
-    load_stage(P, ws, in, b, t, L);
+    load_stage(P, prm, ws, in, b, t, L);
     __syncthreads();
-    riccati_step(P, prm, b, t, L);
+    riccati_step<T, MMAX>(P, prm, ws, b, t, L);
     store_value_and_gain(P, ws, b, t, L, true);
   }
   __syncthreads();
 }
 
-// Forward rollout over stages [t_lo, t_hi) from dx_{t_lo} in `x` (LDS, n);
-// writes dx, du, dv for t in [t_lo, t_hi) (and t_hi when last) and accumulates
-// d_f, d_c, kappa partials into acc.  On return `x` holds dx_{t_hi}.
+// dx over stages [t_lo, t_hi) from dx_{t_lo} in x (LDS, n): dx_{t+1} = F_t dx_t + f_t.
+// F of the next stage is fetched into registers while the current one is applied.
+// Writes dx_t for t in [t_lo, t_hi) (and dx_{t_hi} when last); x ends as dx_{t_hi}.
+// F0, F1: LDS buffers of n*n; x2: LDS n.
 template <typename T>
-__device__ void forward_range(const Prob<T>& P, const Prm<T>& prm, const Iter<T>& it, const Ws<T>& ws,
-                              const LqrIn<T>& in, int b, int t_lo, int t_hi, bool last, T* x, T* x2,
-                              T* sdu, T* acc) {
-  const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
+__device__ void fwd_chain(const Prob<T>& P, const Ws<T>& ws, int b, int t_lo, int t_hi, bool last, T* x, T* F0,
+                          T* F1, T* x2) {
+  const int n = P.n, N = P.N, N1 = N + 1;
   const int tid = threadIdx.x, nt = blockDim.x;
-  const T rc = prm.rho_c;
-  const T* Vb = it.V + (size_t)b * N1 * n;
-  for (int t = t_lo; t <= t_hi; ++t) {
-    const bool term = (t == t_hi);
-    if (term && !last) break;
-    const T* St = ws.S + ((size_t)b * N1 + t) * n * n;
-    const T* st = ws.s + ((size_t)b * N1 + t) * n;
-    const T* ct = ws.cdef + ((size_t)b * N1 + t) * n;  // row t pairs with V_t
-    const T* qt = ws.qh + ((size_t)b * N1 + t) * n;
+  const int nn = n * n;
+  constexpr int PF = 16;  // prefetch registers per thread
+  T pre[PF];
+  const bool fast = nn <= PF * nt;
+  auto fetch = [&](int t) {
+    const T* F = ws.F + ((size_t)b * N + t) * nn;
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const int e = tid + u * nt;
+      pre[u] = e < nn ? F[e] : T(0);
+    }
+  };
+  auto spill = [&](T* dst) {
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const int e = tid + u * nt;
+      if (e < nn) dst[e] = pre[u];
+    }
+  };
+  T* cur = F0;
+  T* nxt = F1;
+  if (t_lo < t_hi) {
+    if (fast) { fetch(t_lo); spill(cur); }
+    else load_block(cur, ws.F + ((size_t)b * N + t_lo) * nn, n, n, n);
+  }
+  __syncthreads();
+  for (int t = t_lo; t < t_hi; ++t) {
+    if (fast && t + 1 < t_hi) fetch(t + 1);
+    const T* f = ws.f + ((size_t)b * N + t) * n;
     T* dxo = ws.dx + ((size_t)b * N1 + t) * n;
-    T* dvo = ws.dv + ((size_t)b * N1 + t) * n;
-    // lam = S dx + s ; dV = lam - V ; d_c and the state part of d_f
     for (int i = tid; i < n; i += nt) {
-      T lam = st[i];
-      for (int j = 0; j < n; ++j) lam += St[i * n + j] * x[j];
-      const T v = Vb[(size_t)t * n + i];
-      const T dv = lam - v;
-      dvo[i] = dv;
-      dxo[i] = x[i];
-      acc[FP_DC] += (dv - v) * ct[i];
-      acc[FP_DF] += qt[i] * x[i];
-    }
-    if (term) {
-      const T* QN = P.QN.at(b, 0);
-      for (int i = tid; i < n; i += nt) {
-        T v = 0;
-        for (int j = 0; j < n; ++j) v += QN[i * n + j] * x[j];
-        acc[FP_KAP] += x[i] * v;
-      }
-      break;
-    }
-    // du = -K dx - k
-    const T* Kt = ws.K + ((size_t)b * N + t) * m * n;
-    const T* kt = ws.k + ((size_t)b * N + t) * m;
-    T* duo = ws.du + ((size_t)b * N + t) * m;
-    for (int i = tid; i < m; i += nt) {
-      T v = kt[i];
-      for (int j = 0; j < n; ++j) v += Kt[i * n + j] * x[j];
-      sdu[i] = -v;
-      duo[i] = -v;
-    }
-    __syncthreads();
-    // dx' = A dx + B du + c_t ; curvature and control part of d_f
-    const T* A = in.A.at(b, t);
-    const T* Bm = in.Bm.at(b, t);
-    const T* cn = ws.cdef + ((size_t)b * N1 + t + 1) * n;
-    const T* Q = P.Q.at(b, t);
-    for (int i = tid; i < n; i += nt) {
-      T v = cn[i], qx = 0;
-      for (int j = 0; j < n; ++j) { v += A[i * n + j] * x[j]; qx += Q[i * n + j] * x[j]; }
-      for (int j = 0; j < m; ++j) v += Bm[i * m + j] * sdu[j];
+      T v = f[i];
+      for (int j = 0; j < n; ++j) v += cur[i * n + j] * x[j];
       x2[i] = v;
-      acc[FP_KAP] += x[i] * qx;
+      dxo[i] = x[i];
     }
-    {
-      const T* R = P.R.at(b, t);
-      const T* rt = ws.rh + ((size_t)b * N + t) * m;
-      const T* rdt = ws.rd + ((size_t)b * N + t) * m;
-      for (int i = tid; i < m; i += nt) {
-        T ru = 0;
-        for (int j = 0; j < m; ++j) ru += R[i * m + j] * sdu[j];
-        acc[FP_KAP] += sdu[i] * ru + rdt[i] * sdu[i] * sdu[i];
-        acc[FP_DF] += rt[i] * sdu[i];
-      }
-      if (p > 0) {
-        const T* D = P.D.at(b, t);
-        const T* E = P.E.at(b, t);
-        for (int k = tid; k < p; k += nt) {
-          T hd = 0;
-          for (int j = 0; j < n; ++j) hd += D[k * n + j] * x[j];
-          for (int j = 0; j < m; ++j) hd += E[k * m + j] * sdu[j];
-          acc[FP_KAP] += rc * hd * hd;
-        }
-      }
+    if (t + 1 < t_hi) {
+      if (fast) spill(nxt);
+      else load_block(nxt, ws.F + ((size_t)b * N + t + 1) * nn, n, n, n);
     }
     __syncthreads();
     for (int i = tid; i < n; i += nt) x[i] = x2[i];
+    T* tmp = cur; cur = nxt; nxt = tmp;
     __syncthreads();
+  }
+  if (last)
+    for (int i = tid; i < n; i += nt) ws.dx[((size_t)b * N1 + t_hi) * n + i] = x[i];
+  __syncthreads();
+}
+
+template <typename T>
+__host__ __device__ inline size_t stage_smem_elems(int n, int nt) {
+  return (size_t)n * (n + 1) + (size_t)n + (size_t)NFP * nt + 8;
+}
+
+// Parallel over (stage t, instance b): lam = S dx + s, dV = lam - V,
+// du = -K dx - k, and the partials q'dx + r'du, (dV - V)'c, lam'c of stage t
+// (the line search forms the curvature dz'H dz = sum lam'c - d_f from them).
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd_stage(Prob<T> P, Iter<T> it, Ws<T> ws) {
+  const int t = blockIdx.x, b = blockIdx.y;
+  const int n = P.n, m = P.m, N = P.N, N1 = N + 1;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sS = reinterpret_cast<T*>(smem_raw);   // n x (n+1) (padded rows)
+  T* sx = sS + n * (n + 1);
+  T* red = sx + n;
+  const int lds = n + 1;
+  const T* S = ws.S + ((size_t)b * N1 + t) * n * n;
+  for (int i = 0; i < n; ++i)
+    for (int j = tid; j < n; j += nt) sS[i * lds + j] = S[i * n + j];
+  const T* dx = ws.dx + ((size_t)b * N1 + t) * n;
+  for (int i = tid; i < n; i += nt) sx[i] = dx[i];
+  __syncthreads();
+  T acc[NFP] = {0, 0, 0, 0};
+  const T* s = ws.s + ((size_t)b * N1 + t) * n;
+  const T* V = it.V + ((size_t)b * N1 + t) * n;
+  const T* c = ws.cdef + ((size_t)b * N1 + t) * n;
+  const T* q = ws.qh + ((size_t)b * N1 + t) * n;
+  T* dv = ws.dv + ((size_t)b * N1 + t) * n;
+  for (int i = tid; i < n; i += nt) {
+    T lam = s[i];
+    for (int j = 0; j < n; ++j) lam += sS[i * lds + j] * sx[j];
+    const T v = V[i];
+    const T d = lam - v;
+    dv[i] = d;
+    acc[FP_DC] += (d - v) * c[i];
+    acc[FP_LC] += lam * c[i];
+    acc[FP_DF] += q[i] * sx[i];
+  }
+  if (t < N) {
+    const T* K = ws.K + ((size_t)b * N + t) * m * n;
+    const T* k = ws.k + ((size_t)b * N + t) * m;
+    const T* r = ws.rh + ((size_t)b * N + t) * m;
+    T* du = ws.du + ((size_t)b * N + t) * m;
+    for (int i = tid; i < m; i += nt) {
+      T v = k[i];
+      for (int j = 0; j < n; ++j) v += K[i * n + j] * sx[j];
+      du[i] = -v;
+      acc[FP_DF] -= r[i] * v;
+    }
+  }
+  block_sum<T, NFP>(acc, 3, red);
+  if (tid == 0) {
+    T* fp = ws.fpart + ((size_t)b * N1 + t) * NFP;
+    fp[FP_DF] = acc[FP_DF];
+    fp[FP_DC] = acc[FP_DC];
+    fp[FP_LC] = acc[FP_LC];
+    fp[3] = 0;
   }
 }
 
-// segments == 1: sequential Riccati + forward pass, one block per instance.
-template <typename T>
-__global__ void k_lqr_seq(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, LqrIn<T> in) {
+// segments == 1: sequential Riccati, then the dx chain, one block per instance.
+template <typename T, int MMAX>
+__global__ void __launch_bounds__(512) k_lqr_seq(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, LqrIn<T> in) {
   const int b = blockIdx.x;
   const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -280,23 +387,11 @@ __global__ void k_lqr_seq(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, LqrIn<T> 
   load_block(L.S, P.QN.at(b, 0), n, n, n);
   for (int i = tid; i < n; i += nt) L.s[i] = ws.qh[((size_t)b * N1 + N) * n + i];
   __syncthreads();
-  riccati_range(P, prm, ws, in, b, 0, N, L, true);
-  // forward pass (reuse LDS: x in L.c, x2 in L.q, du in L.r)
-  T acc[NFP] = {T(0), T(0), T(0), T(0)};
-  T* x = L.Sc;
-  T* x2 = L.c;
-  T* sdu = L.r;
-  for (int i = tid; i < n; i += nt) x[i] = ws.cdef[(size_t)b * N1 * n + i];
+  riccati_range<T, MMAX>(P, prm, ws, in, b, 0, N, L, true);
+  // dx chain: F buffers in A/T1 (each >= n*n), x in Sc, x2 in c
+  for (int i = tid; i < n; i += nt) L.Sc[i] = ws.cdef[(size_t)b * N1 * n + i];
   __syncthreads();
-  forward_range(P, prm, it, ws, in, b, 0, N, true, x, x2, sdu, acc);
-  block_sum<T, NFP>(acc, 3, L.red);
-  if (tid == 0) {
-    T* fp = ws.fpart + (size_t)b * prm.P * NFP;
-    fp[FP_DF] = acc[FP_DF];
-    fp[FP_DC] = acc[FP_DC];
-    fp[FP_KAP] = acc[FP_KAP];
-    fp[3] = 0;
-  }
+  fwd_chain(P, ws, b, 0, N, true, L.Sc, L.A, L.T1, L.c);
 }
 
 }  // namespace mpxk

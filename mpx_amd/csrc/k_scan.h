@@ -292,8 +292,8 @@ __device__ void leaf_combine_up(const Prob<T>& P, const Prm<T>& prm, const Ws<T>
   store_block(out + 3 * nn + n, ge, n);
 }
 
-template <typename T>
-__global__ void k_scan_up(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, LqrIn<T> in) {
+template <typename T, int MMAX>
+__global__ void __launch_bounds__(512) k_scan_up(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, LqrIn<T> in) {
   const int s = blockIdx.x, b = blockIdx.y;
   const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -305,7 +305,7 @@ __global__ void k_scan_up(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, LqrIn<T> 
     load_block(L.S, P.QN.at(b, 0), n, n, n);
     for (int i = threadIdx.x; i < n; i += blockDim.x) L.s[i] = ws.qh[((size_t)b * N1 + N) * n + i];
     __syncthreads();
-    riccati_range(P, prm, ws, in, b, a, N, L, true);
+    riccati_range<T, MMAX>(P, prm, ws, in, b, a, N, L, true);
     return;
   }
   leaf_combine_up(P, prm, ws, in, b, a, e, base, ws.agg + ((size_t)b * prm.P + s) * agg_elems<T>(n));
@@ -318,7 +318,7 @@ __host__ __device__ inline size_t mid_smem_elems(int n, int nt) {
 
 // Suffix values at segment starts: v_s = E_s (x) v_{s+1}.
 template <typename T>
-__global__ void k_scan_mid(Prob<T> P, Prm<T> prm, Ws<T> ws) {
+__global__ void __launch_bounds__(512) k_scan_mid(Prob<T> P, Prm<T> prm, Ws<T> ws) {
   const int b = blockIdx.x;
   const int n = P.n, N = P.N, N1 = N + 1, Pn = prm.P;
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -417,8 +417,8 @@ __global__ void k_scan_mid(Prob<T> P, Prm<T> prm, Ws<T> ws) {
   }
 }
 
-template <typename T>
-__global__ void k_scan_down(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, LqrIn<T> in) {
+template <typename T, int MMAX>
+__global__ void __launch_bounds__(512) k_scan_down(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, LqrIn<T> in) {
   const int s = blockIdx.x, b = blockIdx.y;
   const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1, Pn = prm.P;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -436,73 +436,49 @@ __global__ void k_scan_down(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, LqrIn<T
     for (int i = threadIdx.x; i < n; i += blockDim.x) L.s[i] = ws.s[((size_t)b * N1 + e) * n + i];
   }
   __syncthreads();
-  riccati_range(P, prm, ws, in, b, a, e, L, false);
+  riccati_range<T, MMAX>(P, prm, ws, in, b, a, e, L, false);
 }
 
-// forward: segment maps / segment-0 rollout
+// forward: segment 0 runs the dx chain; segments 1..P-2 compose their maps
+// Phi = F_{e-1} ... F_a, phi (dx_e = Phi dx_a + phi) from the stored F_t, f_t.
 template <typename T>
 __host__ __device__ inline size_t fwd_smem_elems(int n, int m, int nt) {
-  return (size_t)3 * n * n + 2 * (size_t)n * m + 6 * (size_t)n + 2 * (size_t)m + 4 * (size_t)nt + 8;
+  return (size_t)3 * n * n + 4 * (size_t)n + 8;
 }
 
 template <typename T>
-__global__ void k_fwd_up(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, LqrIn<T> in) {
+__global__ void __launch_bounds__(512) k_fwd_up(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, LqrIn<T> in) {
   const int s = blockIdx.x, b = blockIdx.y;
-  const int n = P.n, m = P.m, N = P.N, N1 = N + 1, Pn = prm.P;
+  const int n = P.n, N = P.N, N1 = N + 1, Pn = prm.P;
   const int tid = threadIdx.x, nt = blockDim.x;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* q = reinterpret_cast<T*>(smem_raw);
   T* Phi = q; q += n * n;
   T* F = q; q += n * n;
   T* Tm = q; q += n * n;
-  T* sB = q; q += n * m;
-  T* sK = q; q += n * m;
   T* x = q; q += n;
   T* x2 = q; q += n;
   T* phi = q; q += n;
-  T* f = q; q += n;
-  T* w = q; q += 2 * n;
-  T* sk = q; q += m;
-  T* sdu = q; q += m;
-  T* red = q;
+  T* w = q; q += n;
   int a, e;
   seg_bounds(N, Pn, s, &a, &e);
   if (s == 0) {
-    T acc[NFP] = {0, 0, 0, 0};
     for (int i = tid; i < n; i += nt) x[i] = ws.cdef[(size_t)b * N1 * n + i];
     __syncthreads();
-    forward_range(P, prm, it, ws, in, b, 0, e, Pn == 1, x, x2, sdu, acc);
-    block_sum<T, NFP>(acc, 3, red);
-    if (tid == 0) {
-      T* fp = ws.fpart + (size_t)b * Pn * NFP;
-      fp[0] = acc[0]; fp[1] = acc[1]; fp[2] = acc[2]; fp[3] = 0;
-    }
+    fwd_chain(P, ws, b, 0, e, Pn == 1, x, F, Tm, x2);
     for (int i = tid; i < n; i += nt) ws.xs[((size_t)b * Pn + 1) * n + i] = x[i];
     return;
   }
-  if (s == Pn - 1) return;  // its map is never needed
   for (int e2 = tid; e2 < n * n; e2 += nt) Phi[e2] = (e2 / n == e2 % n) ? T(1) : T(0);
   for (int i = tid; i < n; i += nt) phi[i] = 0;
   __syncthreads();
   for (int t = a; t < e; ++t) {
-    load_block(F, in.A.at(b, t), n, n, n);
-    load_block(sB, in.Bm.at(b, t), n, m, m);
-    load_block(sK, ws.K + ((size_t)b * N + t) * m * n, m, n, n);
-    const T* kt = ws.k + ((size_t)b * N + t) * m;
-    const T* ct = ws.cdef + ((size_t)b * N1 + t + 1) * n;
-    for (int i = tid; i < m; i += nt) sk[i] = kt[i];
-    for (int i = tid; i < n; i += nt) f[i] = ct[i];
-    __syncthreads();
-    gemm_lds<T, false, false, 2>(n, n, m, sB, m, sK, n, F, n);  // F = A - B K
-    for (int i = tid; i < n; i += nt) {
-      T v = f[i];
-      for (int k = 0; k < m; ++k) v -= sB[i * m + k] * sk[k];
-      f[i] = v;
-    }
+    load_block(F, ws.F + ((size_t)b * N + t) * n * n, n, n, n);
+    const T* ft = ws.f + ((size_t)b * N + t) * n;
     __syncthreads();
     gemm_lds<T, false, false, 0>(n, n, n, F, n, Phi, n, Tm, n);
     for (int i = tid; i < n; i += nt) {
-      T v = f[i];
+      T v = ft[i];
       for (int k = 0; k < n; ++k) v += F[i * n + k] * phi[k];
       w[i] = v;
     }
@@ -542,27 +518,23 @@ __global__ void k_fwd_mid(Prob<T> P, Prm<T> prm, Ws<T> ws) {
   }
 }
 
+// segments 1..P-1 run their dx chains from the propagated segment starts
 template <typename T>
-__global__ void k_fwd_down(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, LqrIn<T> in) {
+__global__ void __launch_bounds__(512) k_fwd_down(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, LqrIn<T> in) {
   const int s = blockIdx.x + 1, b = blockIdx.y;
-  const int n = P.n, m = P.m, N = P.N, Pn = prm.P;
+  const int n = P.n, N = P.N, Pn = prm.P;
   const int tid = threadIdx.x, nt = blockDim.x;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* x = reinterpret_cast<T*>(smem_raw);
-  T* x2 = x + n;
-  T* sdu = x2 + n;
-  T* red = sdu + m;
+  T* q = reinterpret_cast<T*>(smem_raw);
+  T* F0 = q; q += n * n;
+  T* F1 = q; q += n * n;
+  T* x = q; q += n;
+  T* x2 = q; q += n;
   int a, e;
   seg_bounds(N, Pn, s, &a, &e);
   for (int i = tid; i < n; i += nt) x[i] = ws.xs[((size_t)b * Pn + s) * n + i];
   __syncthreads();
-  T acc[NFP] = {0, 0, 0, 0};
-  forward_range(P, prm, it, ws, in, b, a, e, s == Pn - 1, x, x2, sdu, acc);
-  block_sum<T, NFP>(acc, 3, red);
-  if (tid == 0) {
-    T* fp = ws.fpart + ((size_t)b * Pn + s) * NFP;
-    fp[0] = acc[0]; fp[1] = acc[1]; fp[2] = acc[2]; fp[3] = 0;
-  }
+  fwd_chain(P, ws, b, a, e, s == Pn - 1, x, F0, F1, x2);
 }
 
 }  // namespace mpxk

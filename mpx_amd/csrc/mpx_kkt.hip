@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "mpx_kkt.h"
 #include "common.h"
@@ -86,11 +87,13 @@ size_t carve(const mpxkkt_problem* pr, int P, Ws<T>* ws, char* base) {
   ws->s = take(B * N1 * n);
   ws->K = take(B * N * m * n);
   ws->k = take(B * N * m);
+  ws->F = take(B * N * n * n);
+  ws->f = take(B * N * n);
   ws->dx = take(B * N1 * n);
   ws->du = take(B * N * m);
   ws->dv = take(B * N1 * n);
   ws->mpart = take(B * N1 * NMP);
-  ws->fpart = take(B * P * NFP);
+  ws->fpart = take(B * N1 * NFP);
   ws->agg = take(P > 1 ? B * P * (3 * n * n + 2 * n) : 0);
   ws->fmap = take(P > 1 ? B * P * (n * n + n) : 0);
   ws->sval = take(P > 1 ? B * P * (n * n + n) : 0);
@@ -124,8 +127,8 @@ mpxkkt_status check_problem(const mpxkkt_problem* pr) {
   if (pr->batch < 1 || pr->batch > 65535) return fail(MPXKKT_ERR_ARG, "batch must be in [1, 65535]");
   if (pr->n < 1 || pr->m < 1 || pr->horizon < 1 || pr->n_eq < 0)
     return fail(MPXKKT_ERR_ARG, "dimensions must be positive");
-  if (pr->n > 512 || pr->m > 512 || pr->n_eq > 512 || pr->horizon > 100000)
-    return fail(MPXKKT_ERR_UNSUPPORTED, "dimension too large");
+  if (pr->n > 512 || pr->m > 64 || pr->n_eq > 512 || pr->horizon > 100000)
+    return fail(MPXKKT_ERR_UNSUPPORTED, "dimension too large (n <= 512, m <= 64)");
   mpxkkt_status s;
   const bool eq = pr->n_eq > 0, box = pr->has_box != 0;
   if ((s = check_view(pr->A, "A", true)) != MPXKKT_OK) return s;
@@ -199,6 +202,7 @@ Prm<T> make_prm(const mpxkkt_params* prm, int P) {
   if (prm->limited_memory) g = 1;
   r.group = std::max(1, std::min(g, r.max_trials));
   r.P = P;
+  r.flags = prm->flags;
   return r;
 }
 
@@ -226,22 +230,22 @@ LqrIn<T> make_lqr_in(const mpxkkt_problem* pr, const Ws<T>& ws) {
 
 int k1_threads(int n) { return n <= 64 ? 64 : 128; }
 
-template <typename T>
+template <typename T, int MMAX>
 mpxkkt_status run_scan(hipStream_t st, const mpxkkt_problem* pr, int nt, const Prob<T>& P, const Prm<T>& R,
                        const Iter<T>& I, const Ws<T>& W, const LqrIn<T>& in) {
   const int n = pr->n, m = pr->m, p = pr->n_eq, B = pr->batch, Pn = R.P;
   mpxkkt_status s;
-  const size_t up = std::max(up_smem_elems<T>(n, m, p, nt), ric_smem_elems<T>(n, m, p, nt)) * sizeof(T);
-  if ((s = prep_lds(k_scan_up<T>, up)) != MPXKKT_OK) return s;
-  hipLaunchKernelGGL(k_scan_up<T>, dim3(Pn, B), dim3(nt), up, st, P, R, I, W, in);
+  const size_t ric = ric_smem_elems<T>(n, m, p, nt) * sizeof(T);
+  const size_t up = std::max(up_smem_elems<T>(n, m, p, nt) * sizeof(T), ric);
+  if ((s = prep_lds(k_scan_up<T, MMAX>, up)) != MPXKKT_OK) return s;
+  hipLaunchKernelGGL((k_scan_up<T, MMAX>), dim3(Pn, B), dim3(nt), up, st, P, R, I, W, in);
   if ((s = launch_check("k_scan_up")) != MPXKKT_OK) return s;
   const size_t mid = mid_smem_elems<T>(n, nt) * sizeof(T) + 64;
   if ((s = prep_lds(k_scan_mid<T>, mid)) != MPXKKT_OK) return s;
   hipLaunchKernelGGL(k_scan_mid<T>, dim3(B), dim3(nt), mid, st, P, R, W);
   if ((s = launch_check("k_scan_mid")) != MPXKKT_OK) return s;
-  const size_t down = ric_smem_elems<T>(n, m, p, nt) * sizeof(T);
-  if ((s = prep_lds(k_scan_down<T>, down)) != MPXKKT_OK) return s;
-  hipLaunchKernelGGL(k_scan_down<T>, dim3(Pn - 1, B), dim3(nt), down, st, P, R, I, W, in);
+  if ((s = prep_lds(k_scan_down<T, MMAX>, ric)) != MPXKKT_OK) return s;
+  hipLaunchKernelGGL((k_scan_down<T, MMAX>), dim3(Pn - 1, B), dim3(nt), ric, st, P, R, I, W, in);
   if ((s = launch_check("k_scan_down")) != MPXKKT_OK) return s;
   const size_t fu = fwd_smem_elems<T>(n, m, nt) * sizeof(T);
   if ((s = prep_lds(k_fwd_up<T>, fu)) != MPXKKT_OK) return s;
@@ -249,10 +253,22 @@ mpxkkt_status run_scan(hipStream_t st, const mpxkkt_problem* pr, int nt, const P
   if ((s = launch_check("k_fwd_up")) != MPXKKT_OK) return s;
   hipLaunchKernelGGL(k_fwd_mid<T>, dim3(B), dim3(64), 2 * n * sizeof(T), st, P, R, W);
   if ((s = launch_check("k_fwd_mid")) != MPXKKT_OK) return s;
-  const size_t fd = ((size_t)2 * n + m + 4 * nt + 8) * sizeof(T);
+  const size_t fd = ((size_t)2 * n * n + 2 * n + 8) * sizeof(T);
   if ((s = prep_lds(k_fwd_down<T>, fd)) != MPXKKT_OK) return s;
   hipLaunchKernelGGL(k_fwd_down<T>, dim3(Pn - 1, B), dim3(nt), fd, st, P, R, I, W, in);
   return launch_check("k_fwd_down");
+}
+
+// costates, control steps and slope partials, parallel over stages
+template <typename T>
+mpxkkt_status run_fwd_stage(hipStream_t st, const mpxkkt_problem* pr, const Prob<T>& P, const Iter<T>& I,
+                            const Ws<T>& W) {
+  const int nt = 64;
+  const size_t lds = stage_smem_elems<T>(pr->n, nt) * sizeof(T);
+  mpxkkt_status s = prep_lds(k_fwd_stage<T>, lds);
+  if (s != MPXKKT_OK) return s;
+  hipLaunchKernelGGL(k_fwd_stage<T>, dim3(pr->horizon + 1, pr->batch), dim3(nt), lds, st, P, I, W);
+  return launch_check("k_fwd_stage");
 }
 
 template <typename T>
@@ -267,18 +283,28 @@ mpxkkt_status run_linearize(hipStream_t st, const mpxkkt_problem* pr, const Prob
   return launch_check("k_linearize");
 }
 
+template <typename T, int MMAX>
+mpxkkt_status run_lqr_m(hipStream_t st, const mpxkkt_problem* pr, int nt, const Prob<T>& P, const Prm<T>& R,
+                        const Iter<T>& I, const Ws<T>& W, const LqrIn<T>& in) {
+  mpxkkt_status s;
+  if (R.P > 1) {
+    if ((s = run_scan<T, MMAX>(st, pr, nt, P, R, I, W, in)) != MPXKKT_OK) return s;
+  } else {
+    const size_t lds = ric_smem_elems<T>(pr->n, pr->m, pr->n_eq, nt) * sizeof(T);
+    if ((s = prep_lds(k_lqr_seq<T, MMAX>, lds)) != MPXKKT_OK) return s;
+    hipLaunchKernelGGL((k_lqr_seq<T, MMAX>), dim3(pr->batch), dim3(nt), lds, st, P, R, I, W, in);
+    if ((s = launch_check("k_lqr_seq")) != MPXKKT_OK) return s;
+  }
+  return run_fwd_stage<T>(st, pr, P, I, W);
+}
+
 template <typename T>
 mpxkkt_status run_lqr(hipStream_t st, const mpxkkt_problem* pr, const mpxkkt_params* prm, const Prob<T>& P,
                       const Prm<T>& R, const Iter<T>& I, const Ws<T>& W, const LqrIn<T>& in) {
-  const int nt = prm->block_threads > 0 ? pow2_threads(prm->block_threads, 64, 1024) : 256;
-  const size_t lds = ric_smem_elems<T>(pr->n, pr->m, pr->n_eq, nt) * sizeof(T);
-  mpxkkt_status s;
-  if (R.P <= 1) {
-    if ((s = prep_lds(k_lqr_seq<T>, lds)) != MPXKKT_OK) return s;
-    hipLaunchKernelGGL(k_lqr_seq<T>, dim3(pr->batch), dim3(nt), lds, st, P, R, I, W, in);
-    return launch_check("k_lqr_seq");
-  }
-  return run_scan<T>(st, pr, nt, P, R, I, W, in);
+  const int nt = prm->block_threads > 0 ? pow2_threads(prm->block_threads, 64, 512) : 256;
+  if (pr->m <= 16) return run_lqr_m<T, 16>(st, pr, nt, P, R, I, W, in);
+  if (pr->m <= 32) return run_lqr_m<T, 32>(st, pr, nt, P, R, I, W, in);
+  return run_lqr_m<T, 64>(st, pr, nt, P, R, I, W, in);
 }
 
 template <typename T>
@@ -298,6 +324,17 @@ mpxkkt_status run_linesearch(hipStream_t st, const mpxkkt_problem* pr, const Pro
   return launch_check("k_linesearch");
 }
 
+// phase-timing record (host-side; events on the caller's stream)
+struct PhaseEvents {
+  hipEvent_t ev[4];
+};
+bool g_prof_on = false;
+std::vector<PhaseEvents> g_prof;
+
+void prof_mark(hipStream_t st, PhaseEvents* pe, int k) {
+  if (pe) (void)hipEventRecord(pe->ev[k], st);
+}
+
 template <typename T>
 mpxkkt_status sqp_iterate_t(hipStream_t st, const mpxkkt_problem* pr, const mpxkkt_params* prm,
                             mpxkkt_iterate* it, mpxkkt_stats* so, void* wsp) {
@@ -309,9 +346,22 @@ mpxkkt_status sqp_iterate_t(hipStream_t st, const mpxkkt_problem* pr, const mpxk
   const Iter<T> I = make_iter<T>(it);
   const LqrIn<T> in = make_lqr_in<T>(pr, W);
   mpxkkt_status s;
+  PhaseEvents* pe = nullptr;
+  if (g_prof_on) {
+    PhaseEvents e;
+    for (int k = 0; k < 4; ++k)
+      if (hipEventCreate(&e.ev[k]) != hipSuccess) return fail(MPXKKT_ERR_DEVICE, "hipEventCreate failed");
+    g_prof.push_back(e);
+    pe = &g_prof.back();
+  }
+  prof_mark(st, pe, 0);
   if ((s = run_linearize<T>(st, pr, P, R, I, W)) != MPXKKT_OK) return s;
+  prof_mark(st, pe, 1);
   if ((s = run_lqr<T>(st, pr, prm, P, R, I, W, in)) != MPXKKT_OK) return s;
-  return run_linesearch<T>(st, pr, P, R, I, W, in, so);
+  prof_mark(st, pe, 2);
+  s = run_linesearch<T>(st, pr, P, R, I, W, in, so);
+  prof_mark(st, pe, 3);
+  return s;
 }
 
 template <typename T>
@@ -432,6 +482,28 @@ mpxkkt_status mpxkkt_shift_warmstart(void* stream, const mpxkkt_problem* prob, i
                        make_iter<float>(it), (const float*)u_reset, reset);
   }
   return launch_check("k_shift");
+}
+
+mpxkkt_status mpxkkt_profile_enable(int32_t on) {
+  g_prof_on = on != 0;
+  return MPXKKT_OK;
+}
+
+mpxkkt_status mpxkkt_profile_collect(double ms_per_phase[3], int32_t* calls) {
+  if (!ms_per_phase || !calls) return fail(MPXKKT_ERR_ARG, "null output");
+  ms_per_phase[0] = ms_per_phase[1] = ms_per_phase[2] = 0.0;
+  mpxkkt_status s = MPXKKT_OK;
+  for (auto& e : g_prof) {
+    if (hipEventSynchronize(e.ev[3]) != hipSuccess) s = fail(MPXKKT_ERR_DEVICE, "hipEventSynchronize failed");
+    for (int k = 0; k < 3; ++k) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, e.ev[k], e.ev[k + 1]) == hipSuccess) ms_per_phase[k] += ms;
+    }
+    for (int k = 0; k < 4; ++k) (void)hipEventDestroy(e.ev[k]);
+  }
+  *calls = (int32_t)g_prof.size();
+  g_prof.clear();
+  return s;
 }
 
 const char* mpxkkt_last_error(void) { return g_err.c_str(); }

@@ -1,0 +1,92 @@
+"""ctypes driver of the FP64 C restatement (oracle/pdilqr_c.c).
+
+TEST INFRASTRUCTURE and CPU baseline only: used by tests/ (checked against
+the numpy oracle) and by bench.py's cpu_baseline leg.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import time
+
+import numpy as np
+
+from . import pdilqr as orc
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(_HERE, "_build", "libpdilqr_c.so")
+_lib = None
+
+
+def available():
+    return os.path.exists(LIB)
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        _lib = C.CDLL(LIB)
+        _lib.pdilqr_solve_batch.restype = C.c_int
+    return _lib
+
+
+def _arr(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else C.c_void_p(0)
+
+
+def solve_batch(insts, starts, iters, threads=0, params=None):
+    """iters SQP iterations for same-shaped instances; returns (X, U, V, Yeq, Ybox, alphas)."""
+    prm = dict(orc.DEFAULT_PARAMS)
+    if params:
+        prm.update(params)
+    i0 = insts[0]
+    n, m, N, p = i0["n"], i0["m"], i0["N"], i0["p"]
+    B = len(insts)
+    st = lambda k: _arr(np.stack([d[k] for d in insts]))
+    A, Bm = st("A"), st("B")
+    W = st("W") if i0["model"] == "tanh" else None
+    Q, R = st("Q"), st("R")
+    q_tv, r_tv = int(Q.shape[1] > 1), int(R.shape[1] > 1)
+    D = st("D") if p else None
+    E = st("E") if p else None
+    e = st("e") if p else None
+    box = i0.get("ulo") is not None
+    ulo = st("ulo") if box else None
+    uhi = st("uhi") if box else None
+    X = _arr(np.stack([s[0] for s in starts]))
+    U = _arr(np.stack([s[1] for s in starts]))
+    V = _arr(np.stack([s[2] for s in starts]))
+    Yeq = _arr(np.stack([s[3] for s in starts]))
+    Ybox = _arr(np.stack([s[4] for s in starts]))
+    pv = _arr([prm["rho0"], prm["rho_max"], prm["rho_c"], prm["mu_b"], prm["tau"], prm["beta"],
+               prm["c1"], prm["armijo_slack"], prm["max_trials"]])
+    alphas = np.zeros((B, iters))
+    threads = threads or os.cpu_count() or 1
+    _load().pdilqr_solve_batch(
+        B, n, m, N, p, 0 if i0["model"] == "ltv" else 1, int(box), q_tv, r_tv, C.c_double(i0["h"]),
+        C.c_double(i0["kappa"]), _ptr(A), _ptr(Bm), _ptr(W), _ptr(Q), _ptr(R), _ptr(st("QN")),
+        _ptr(st("xref")), _ptr(st("uref")), _ptr(D), _ptr(E), _ptr(e), _ptr(ulo), _ptr(uhi), _ptr(st("x0")),
+        _ptr(X), _ptr(U), _ptr(V), _ptr(Yeq), _ptr(Ybox), _ptr(pv), iters, _ptr(alphas), threads)
+    return X, U, V, Yeq, Ybox, alphas
+
+
+def bench(insts, starts, iters=3, budget_s=15.0, threads=0):
+    """Instance-iterations per second of the C restatement on all host cores,
+    on a sample of the instances grown until it takes about budget_s."""
+    threads = threads or os.cpu_count() or 1
+    k = max(threads, 8)
+    while True:
+        sample = min(k, len(insts))
+        t0 = time.perf_counter()
+        solve_batch(insts[:sample], starts[:sample], iters, threads)
+        dt = time.perf_counter() - t0
+        if dt > budget_s / 4 or sample == len(insts):
+            break
+        k *= 4
+    return dict(inst_iters_per_s=sample * iters / dt, cores=threads, kind="port",
+                sample=f"{sample} instances x {iters} SQP iterations of the FP64 C restatement "
+                       f"(oracle/pdilqr_c.c, -O3 -march=native, OpenMP), {dt:.2f} s")

@@ -68,10 +68,11 @@ def test_fp64_matches_oracle(wl, shape, iters, segments):
                 assert _rel(g["Ybox"][i], r["Ybox"]) < 1e-9
 
 
-@pytest.mark.parametrize("segments", [1, 4])
-def test_fp32_workload3_shape(segments):
-    """n=36, m=12, N=50, p=8 (the benchmark shape), 3 iterations, FP32."""
-    insts, starts, refs, out, solver, it = _run(3, 4, 3, torch.float32, segments=segments)
+@pytest.mark.parametrize("segments,flags", [(1, 0), (1, 1), (4, 0)])
+def test_fp32_workload3_shape(segments, flags):
+    """n=36, m=12, N=50, p=8 (the benchmark shape), 3 iterations, FP32; flags=1
+    forms the products on VALU instead of the matrix cores."""
+    insts, starts, refs, out, solver, it = _run(3, 4, 3, torch.float32, segments=segments, flags=flags)
     for k in range(3):
         g, st = out[k]
         for i in range(4):
