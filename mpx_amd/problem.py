@@ -116,6 +116,20 @@ class DeviceBatch:
         pr.x0 = _view(t["x0"], n, 0)
         self.problem = pr
 
+    def prefix(self, k):
+        """The first k instances as a batch of their own (same device arrays; the
+        instance-major layout makes any prefix contiguous)."""
+        if not 1 <= k <= self.B:
+            raise ValueError(f"prefix size {k} outside [1, {self.B}]")
+        other = object.__new__(DeviceBatch)
+        other.__dict__.update(self.__dict__)
+        pr = L.Problem()
+        C.pointer(pr)[0] = self.problem
+        pr.batch = k
+        other.problem = pr
+        other.B = k
+        return other
+
     def iterate_from(self, X, U, V, Yeq=None, Ybox=None):
         """Device iterate dict from host or device arrays (copied, contiguous)."""
         def dev(a, shape):
@@ -141,19 +155,25 @@ def _stream():
 
 
 class SQPSolver:
-    """Batched primal-dual SQP iterations on one device batch."""
+    """Batched primal-dual SQP iterations on one device batch.
 
-    def __init__(self, batch: DeviceBatch, params=None, **tune):
+    ``workspace``: optional preallocated uint8 device tensor (shared by solvers
+    that never run concurrently); sized by :meth:`workspace_bytes`.
+    """
+
+    def __init__(self, batch: DeviceBatch, params=None, workspace=None, **tune):
         self.batch = batch
         self.lib = L.load()
         self.params = params if params is not None else L.default_params()
         for k, v in tune.items():
             setattr(self.params, k, v)
-        nbytes = C.c_size_t(0)
-        L.check(self.lib.mpxkkt_workspace_size(C.byref(batch.problem), C.byref(self.params),
-                                               C.byref(nbytes)))
-        self.ws_bytes = int(nbytes.value)
-        self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=batch.device)
+        self.ws_bytes = self.workspace_bytes(batch, self.params)
+        if workspace is not None:
+            if workspace.numel() < self.ws_bytes:
+                raise L.KKTError(f"workspace of {workspace.numel()} bytes < {self.ws_bytes} needed")
+            self.ws = workspace
+        else:
+            self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=batch.device)
         B = batch.B
         self.stats_t = dict(
             alpha=torch.zeros(B, dtype=batch.dtype, device=batch.device),
@@ -166,6 +186,12 @@ class SQPSolver:
         for k, v in self.stats_t.items():
             setattr(st, k, v.data_ptr())
         self._stats = st
+
+    @staticmethod
+    def workspace_bytes(batch, params):
+        nbytes = C.c_size_t(0)
+        L.check(L.load().mpxkkt_workspace_size(C.byref(batch.problem), C.byref(params), C.byref(nbytes)))
+        return int(nbytes.value)
 
     def iterate(self, it):
         """One SQP iteration in place on ``it``; returns the stats tensors."""

@@ -1,0 +1,91 @@
+"""Serving-stream workload: Poisson arrivals of MPC ticks over a mix of shapes.
+
+Each arrival is one tick of an idle controller (a new controller opens when
+none is idle and the HBM budget allows; a controller closes after ``life``
+ticks).  Shapes are drawn from (n, m, N) = (12, 12, 25), (36, 12, 50),
+(76, 32, 100) with probabilities 0.5, 0.35, 0.15 and every tick needs 1-8 SQP
+iterations.  Latency is measured on the host from the scheduled arrival time
+to the moment the tick's first control is back on the host.  The plant of
+every controller is its own model, advanced on the host with the returned
+control.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import synthetic as syn
+from .serving import MPCServer
+
+
+def _plant_step(inst, x, u):
+    return x + inst["h"] * (inst["A"] @ x + inst["B"] @ u + inst["kappa"] * np.tanh(inst["W"] @ x))
+
+
+def run(rate, duration, seed=0, life=200, budget_gb=64.0, max_group=256, n_streams=4, warm=True, log=None):
+    """Serve ``rate`` ticks/s for ``duration`` s; returns a summary dict."""
+    rng = np.random.default_rng(seed)
+    server = MPCServer(budget_gb * 2**30, max_group=max_group, n_streams=n_streams)
+    shapes, probs = syn.SERVING_SHAPES, syn.SERVING_PROBS
+    if warm:
+        # build every bucket (and its solvers) before the clock starts
+        for si, (n, m, N) in enumerate(shapes):
+            inst = syn.make_instance(5, 10**6 + si, seed, n=n, m=m, N=N)
+            cid = server.open(inst)
+            server.request(cid, inst["x0"], 1)
+            server.run_round()
+            server.close(cid)
+    n_arr = rng.poisson(rate * duration)
+    t_rel = np.sort(rng.uniform(0.0, duration, size=n_arr))
+    idle, state = [], {}
+    latencies, by_shape, waits = [], {}, 0
+    t0 = time.perf_counter()
+    i = 0
+    serial = 0
+    while i < n_arr or server.pending:
+        now = time.perf_counter() - t0
+        while i < n_arr and t_rel[i] <= now:
+            if idle:
+                cid = idle.pop()
+            else:
+                k = rng.choice(len(shapes), p=probs)
+                n, m, N = shapes[k]
+                serial += 1
+                inst = syn.make_instance(5, serial, seed, n=n, m=m, N=N)
+                cid = server.open(inst)
+                if cid is None:
+                    waits += 1
+                    i += 1
+                    continue
+                state[cid] = dict(inst=inst, x=inst["x0"].copy(), ticks=0, shape=(n, m, N))
+            server.request(cid, state[cid]["x"], int(rng.integers(1, 9)), t_arrive=t0 + t_rel[i])
+            i += 1
+        if not server.pending:
+            if i < n_arr:
+                time.sleep(max(0.0, min(1e-4, t_rel[i] - (time.perf_counter() - t0))))
+            continue
+        for r in server.run_round():
+            lat = r.t_done - r.t_arrive
+            latencies.append(lat)
+            s = state[r.cid]
+            by_shape.setdefault(s["shape"], []).append(lat)
+            s["x"] = _plant_step(s["inst"], s["x"], np.clip(np.nan_to_num(r.u0), -1.0, 1.0))
+            s["ticks"] += 1
+            if s["ticks"] >= life:
+                server.close(r.cid)
+                del state[r.cid]
+            else:
+                idle.append(r.cid)
+    wall = time.perf_counter() - t0
+    lat = np.array(latencies) * 1e3
+    out = dict(rate=rate, duration=duration, served=len(lat), solves_per_s=len(lat) / wall,
+               p50_ms=float(np.percentile(lat, 50)) if len(lat) else None,
+               p99_ms=float(np.percentile(lat, 99)) if len(lat) else None,
+               rejected_for_budget=waits, controllers_opened=serial,
+               by_shape={f"{k[0]}x{k[1]}x{k[2]}": dict(n=len(v), p50_ms=float(np.percentile(v, 50) * 1e3),
+                                                      p99_ms=float(np.percentile(v, 99) * 1e3))
+                         for k, v in by_shape.items()})
+    if log:
+        log(out)
+    return out
