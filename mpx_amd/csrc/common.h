@@ -82,6 +82,13 @@ __device__ __forceinline__ T dev_sqrt(T x) { return sqrt(x); }
 template <>
 __device__ __forceinline__ float dev_sqrt<float>(float x) { return sqrtf(x); }
 
+// Block barrier that waits only for the block's LDS traffic, not for its global
+// stores: phases whose global writes are read by later kernels (or after a full
+// __syncthreads) need not stall on them.
+__device__ __forceinline__ void lds_sync() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // Block-wide sum of `cnt` values per thread (cnt <= NV); blockDim.x must be a
 // power of two; sred holds NV*blockDim.x elements.  Every thread gets the sums.
 template <typename T, int NV>
@@ -167,16 +174,16 @@ __device__ void chol_lds(T* H, int m) {
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int j = 0; j < m; ++j) {
     if (tid == 0) H[j * m + j] = dev_sqrt(H[j * m + j]);
-    __syncthreads();
+    lds_sync();
     const T d = H[j * m + j];
     for (int i = j + 1 + tid; i < m; i += nt) H[i * m + j] /= d;
-    __syncthreads();
+    lds_sync();
     const int rem = m - j - 1;
     for (int e = tid; e < rem * rem; e += nt) {
       const int i = j + 1 + e / rem, kk = j + 1 + e % rem;
       if (kk <= i) H[i * m + kk] -= H[i * m + j] * H[kk * m + j];
     }
-    __syncthreads();
+    lds_sync();
   }
 }
 
@@ -228,28 +235,40 @@ struct MSeg {
   float sign;
 };
 
-// acc += sign * op(A)[i, :] op(B)[:, j] over one K-segment; operands of four
-// k-steps are loaded before their four MFMAs so LDS latency overlaps.
+// acc += sign * op(A)[i, :] op(B)[:, j] over one K-segment.  Operands of the
+// next four k-steps are loaded while the current four MFMAs issue.
+#define MPXK_MF_LOAD(K0, A0, A1, A2, A3, B0, B1, B2, B3)                       \
+  do {                                                                         \
+    const int k_ = (K0) + q;                                                   \
+    A0 = (iok && k_ < g.K) ? pa[k_ * sak] * g.sign : 0.f;                      \
+    B0 = (jok && k_ < g.K) ? pb[k_ * sbk] : 0.f;                               \
+    A1 = (iok && k_ + 4 < g.K) ? pa[(k_ + 4) * sak] * g.sign : 0.f;            \
+    B1 = (jok && k_ + 4 < g.K) ? pb[(k_ + 4) * sbk] : 0.f;                     \
+    A2 = (iok && k_ + 8 < g.K) ? pa[(k_ + 8) * sak] * g.sign : 0.f;            \
+    B2 = (jok && k_ + 8 < g.K) ? pb[(k_ + 8) * sbk] : 0.f;                     \
+    A3 = (iok && k_ + 12 < g.K) ? pa[(k_ + 12) * sak] * g.sign : 0.f;          \
+    B3 = (jok && k_ + 12 < g.K) ? pb[(k_ + 12) * sbk] : 0.f;                   \
+  } while (0)
+
 __device__ __forceinline__ void mfma_seg(mpxk_f32x4& acc, const MSeg& g, int i, int j, int M, int N, int q) {
   const bool iok = i < M, jok = j < N;
   const int sak = g.ta ? g.lda : 1, sai = g.ta ? 1 : g.lda;
   const int sbk = g.tb ? 1 : g.ldb, sbj = g.tb ? g.ldb : 1;
   const float* pa = g.A + (iok ? i : 0) * sai;
   const float* pb = g.B + (jok ? j : 0) * sbj;
+  float a0, a1, a2, a3, b0, b1, b2, b3;
+  MPXK_MF_LOAD(0, a0, a1, a2, a3, b0, b1, b2, b3);
   for (int k0 = 0; k0 < g.K; k0 += 16) {
-    float a[4], b[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int k = k0 + 4 * u + q;
-      const bool kok = k < g.K;
-      a[u] = (iok && kok) ? pa[k * sak] * g.sign : 0.f;
-      b[u] = (jok && kok) ? pb[k * sbk] : 0.f;
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (k0 + 4 * u < g.K) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u], acc, 0, 0, 0);
+    float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f, d0 = 0.f, d1 = 0.f, d2 = 0.f, d3 = 0.f;
+    if (k0 + 16 < g.K) MPXK_MF_LOAD(k0 + 16, c0, c1, c2, c3, d0, d1, d2, d3);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc, 0, 0, 0);
+    if (k0 + 4 < g.K) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc, 0, 0, 0);
+    if (k0 + 8 < g.K) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, b2, acc, 0, 0, 0);
+    if (k0 + 12 < g.K) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a3, b3, acc, 0, 0, 0);
+    a0 = c0; a1 = c1; a2 = c2; a3 = c3; b0 = d0; b1 = d1; b2 = d2; b3 = d3;
   }
 }
+#undef MPXK_MF_LOAD
 
 // tile index -> (ti, tj); with SYM only the upper triangle of tiles is enumerated
 template <bool SYM>
@@ -290,8 +309,8 @@ __device__ __forceinline__ void mfma_addend(float* add, int ti, int tj, int M, i
   }
 }
 
-// C = sum_s sign_s op(A_s) op(B_s) + addend + diag, 16x16 tiles, two tiles per
-// wave at a time (independent accumulators keep the matrix pipe busy).
+// C = sum_s sign_s op(A_s) op(B_s) + addend + diag, 16x16 tiles taken by the
+// waves of the block round-robin.
 template <bool SYM>
 __device__ void mfma_gemm(int M, int N, int nseg, MSeg s0, MSeg s1, MSeg s2, float* C, int ldc,
                           const float* addend, int ld_add, const float* diag) {
@@ -299,29 +318,17 @@ __device__ void mfma_gemm(int M, int N, int nseg, MSeg s0, MSeg s1, MSeg s2, flo
   const int tm = (M + 15) >> 4, tn = (N + 15) >> 4;
   const int ntile = SYM ? tn * (tn + 1) / 2 : tm * tn;
   const int r = lane & 15, q = lane >> 4;
-  for (int t0 = wave; t0 < ntile; t0 += 2 * nw) {
-    const int t1 = t0 + nw;
-    const bool two = t1 < ntile;
-    int ti0, tj0, ti1 = 0, tj1 = 0;
-    tile_of<SYM>(t0, tn, &ti0, &tj0);
-    if (two) tile_of<SYM>(t1, tn, &ti1, &tj1);
-    float add0[4], add1[4];
-    mfma_addend<SYM>(add0, ti0, tj0, M, N, q, r, addend, ld_add, diag);
-    if (two) mfma_addend<SYM>(add1, ti1, tj1, M, N, q, r, addend, ld_add, diag);
-    mpxk_f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-    const int i0 = ti0 * 16 + r, j0 = tj0 * 16 + r, i1 = ti1 * 16 + r, j1 = tj1 * 16 + r;
-    mfma_seg(acc0, s0, i0, j0, M, N, q);
-    if (two) mfma_seg(acc1, s0, i1, j1, M, N, q);
-    if (nseg > 1) {
-      mfma_seg(acc0, s1, i0, j0, M, N, q);
-      if (two) mfma_seg(acc1, s1, i1, j1, M, N, q);
-    }
-    if (nseg > 2) {
-      mfma_seg(acc0, s2, i0, j0, M, N, q);
-      if (two) mfma_seg(acc1, s2, i1, j1, M, N, q);
-    }
-    mfma_store<SYM>(acc0, add0, ti0, tj0, M, N, q, r, C, ldc);
-    if (two) mfma_store<SYM>(acc1, add1, ti1, tj1, M, N, q, r, C, ldc);
+  for (int t0 = wave; t0 < ntile; t0 += nw) {
+    int ti, tj;
+    tile_of<SYM>(t0, tn, &ti, &tj);
+    float add[4];
+    mfma_addend<SYM>(add, ti, tj, M, N, q, r, addend, ld_add, diag);
+    mpxk_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    const int i = ti * 16 + r, j = tj * 16 + r;
+    mfma_seg(acc, s0, i, j, M, N, q);
+    if (nseg > 1) mfma_seg(acc, s1, i, j, M, N, q);
+    if (nseg > 2) mfma_seg(acc, s2, i, j, M, N, q);
+    mfma_store<SYM>(acc, add, ti, tj, M, N, q, r, C, ldc);
   }
 }
 
@@ -344,11 +351,12 @@ __device__ __forceinline__ double rlane(double v, int src) {
 }
 
 // In-place inverse of the m x m SPD matrix H (LDS, row-major), m <= MMAX <= 64,
-// by Gauss-Jordan elimination without pivoting (stable for SPD) in wave 0,
-// lane i holding row i of [H | I] in registers; pivot rows are broadcast with
-// readlane.  Other waves return at once; the caller syncs the block.
+// by Gauss-Jordan elimination without pivoting (stable for SPD) in wave 0:
+// lane i keeps row i of [H | I] in registers and the pivot row of each step is
+// broadcast through LDS (`piv`, 2*MMAX elements).  Other waves return at once;
+// the caller syncs the block.
 template <typename T, int MMAX>
-__device__ void spd_inverse_wave(T* H, int m) {
+__device__ void spd_inverse_wave(T* H, int m, T* piv) {
   if (threadIdx.x >= 64) return;
   const int lane = threadIdx.x;
   T a[MMAX], bi[MMAX];
@@ -360,17 +368,18 @@ __device__ void spd_inverse_wave(T* H, int m) {
 #pragma unroll
   for (int k = 0; k < MMAX; ++k) {
     if (k < m) {
-      const T piv = rlane(a[k], k);
-      const T f = lane == k ? T(0) : a[k] / piv;   // multiple of the pivot row to remove
-      const T sc = T(1) / piv;
+      if (lane == k) {
+        const T sc = T(1) / a[k];
+#pragma unroll
+        for (int j = 0; j < MMAX; ++j) { a[j] *= sc; bi[j] *= sc; piv[j] = a[j]; piv[MMAX + j] = bi[j]; }
+      }
+      wave_fence();
+      const T f = lane == k ? T(0) : a[k];
 #pragma unroll
       for (int j = 0; j < MMAX; ++j) {
-        if (j < m) {
-          const T pa = rlane(a[j], k), pb = rlane(bi[j], k);
-          if (lane == k) { a[j] = pa * sc; bi[j] = pb * sc; }
-          else { a[j] -= f * pa; bi[j] -= f * pb; }
-        }
+        if (j < m) { a[j] -= f * piv[j]; bi[j] -= f * piv[MMAX + j]; }
       }
+      wave_fence();
     }
   }
   if (lane < m) {

@@ -26,7 +26,7 @@ struct LqrIn {
 // side in AB (n x (n+m)); the product S [A | B] lands in TB = [T1 | SB].
 template <typename T>
 struct RicSmem {
-  T *S, *A, *T1, *B, *SB, *G, *Kb, *H, *D, *E, *s, *Sc, *c, *rd, *q, *r, *red;
+  T *S, *A, *T1, *B, *SB, *G, *Kb, *H, *D, *E, *s, *Sc, *c, *rd, *q, *r, *piv, *red;
   int ldab;
 };
 
@@ -34,7 +34,7 @@ template <typename T>
 __host__ __device__ inline size_t ric_smem_elems(int n, int m, int p, int nt) {
   const size_t kb = (size_t)m * (n + 1 > m ? n + 1 : m);  // [K | k], or m x m scratch
   return (size_t)n * n + 2 * (size_t)n * (n + m) + (size_t)m * (n + 1) + kb + (size_t)m * m +
-         (size_t)p * (n + m) + 6 * (size_t)n + 2 * (size_t)m + 4 * (size_t)nt + 8;
+         (size_t)p * (n + m) + 6 * (size_t)n + 2 * (size_t)m + 128 + 4 * (size_t)nt + 8;
 }
 
 template <typename T>
@@ -57,11 +57,14 @@ __device__ RicSmem<T> ric_carve(T* base, int n, int m, int p, int nt) {
   L.rd = q; q += m;
   L.r = q; q += m;
   q += 2 * n;
+  L.piv = q; q += 128;
   L.red = q;
   return L;
 }
 
 // Load the stage's QP data into LDS: [A | B], sqrt(rho_c) D, sqrt(rho_c) E, c, q, r, rd.
+// Each thread issues its loads eight at a time before storing them, so a stage
+// costs a couple of memory latencies rather than one per element.
 template <typename T>
 __device__ void load_stage(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& ws, const LqrIn<T>& in, int b,
                            int t, RicSmem<T>& L) {
@@ -69,22 +72,44 @@ __device__ void load_stage(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& ws,
   const int tid = threadIdx.x, nt = blockDim.x;
   const T* A = in.A.at(b, t);
   const T* B = in.Bm.at(b, t);
-  for (int i = 0; i < n; ++i) {
-    for (int j = tid; j < ldab; j += nt) L.A[i * ldab + j] = j < n ? A[i * n + j] : B[i * m + (j - n)];
-  }
-  if (p > 0) {
-    const T sr = dev_sqrt(prm.rho_c);
-    const T* D = P.D.at(b, t);
-    const T* E = P.E.at(b, t);
-    for (int e = tid; e < p * n; e += nt) L.D[e] = sr * D[e];
-    for (int e = tid; e < p * m; e += nt) L.E[e] = sr * E[e];
-  }
+  const T* D = p > 0 ? P.D.at(b, t) : nullptr;
+  const T* E = p > 0 ? P.E.at(b, t) : nullptr;
+  const T sr = dev_sqrt(prm.rho_c);
   const T* c = ws.cdef + ((size_t)b * N1 + t + 1) * n;
   const T* q = ws.qh + ((size_t)b * N1 + t) * n;
   const T* r = ws.rh + ((size_t)b * P.N + t) * m;
   const T* rd = ws.rd + ((size_t)b * P.N + t) * m;
-  for (int i = tid; i < n; i += nt) { L.c[i] = c[i]; L.q[i] = q[i]; }
-  for (int i = tid; i < m; i += nt) { L.r[i] = r[i]; L.rd[i] = rd[i]; }
+  const int o1 = n * ldab, o2 = o1 + p * n, o3 = o2 + p * m, o4 = o3 + n, o5 = o4 + n, o6 = o5 + m;
+  const int tot = o6 + m;
+  for (int base = tid; base < tot; base += 8 * nt) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = base + u * nt;
+      T x = 0;
+      if (e < o1) {
+        const int i = e / ldab, j = e - i * ldab;
+        x = j < n ? A[i * n + j] : B[i * m + (j - n)];
+      } else if (e < o2) x = sr * D[e - o1];
+      else if (e < o3) x = sr * E[e - o2];
+      else if (e < o4) x = c[e - o3];
+      else if (e < o5) x = q[e - o4];
+      else if (e < o6) x = r[e - o5];
+      else if (e < tot) x = rd[e - o6];
+      v[u] = x;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = base + u * nt;
+      if (e < o1) L.A[e] = v[u];
+      else if (e < o2) L.D[e - o1] = v[u];
+      else if (e < o3) L.E[e - o2] = v[u];
+      else if (e < o4) L.c[e - o3] = v[u];
+      else if (e < o5) L.q[e - o4] = v[u];
+      else if (e < o6) L.r[e - o5] = v[u];
+      else if (e < tot) L.rd[e - o6] = v[u];
+    }
+  }
 }
 
 template <typename T>
@@ -92,20 +117,20 @@ struct IsF32 { static constexpr bool value = false; };
 template <>
 struct IsF32<float> { static constexpr bool value = true; };
 
-// H <- H^{-1} for the SPD m x m matrix in LDS (block-wide; wave 0 registers for m <= 16).
+// H <- H^{-1} for the SPD m x m matrix in LDS: Cholesky, then the identity
+// solved through both triangles (block-wide).  (A register Gauss-Jordan in one
+// wave, spd_inverse_wave, measured slower here: its unrolled rows spill.)
 template <typename T, int MMAX>
-__device__ void spd_inverse(T* H, T* tmp, int m) {
-  if constexpr (MMAX <= 16) {
-    spd_inverse_wave<T, MMAX>(H, m);
-  } else {
+__device__ void spd_inverse(T* H, T* tmp, T* piv, int m) {
+  {
     // Cholesky, then the identity solved through both triangles
     chol_lds(H, m);
     for (int e = threadIdx.x; e < m * m; e += blockDim.x) tmp[e] = (e / m == e % m) ? T(1) : T(0);
-    __syncthreads();
+    lds_sync();
     trsm_lower(H, m, tmp, m, m);
-    __syncthreads();
+    lds_sync();
     trsm_upper_t(H, m, tmp, m, m);
-    __syncthreads();
+    lds_sync();
     for (int e = threadIdx.x; e < m * m; e += blockDim.x) H[e] = tmp[e];
   }
 }
@@ -131,6 +156,7 @@ __device__ void riccati_step(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& w
   // A: Sc = S c + s ; [T1 | SB] = S [A | B]
   for (int i = tid; i < n; i += nt) {
     T v = L.s[i];
+#pragma unroll 8
     for (int j = 0; j < n; ++j) v += L.S[i * n + j] * L.c[j];
     L.Sc[i] = v;
   }
@@ -141,7 +167,7 @@ __device__ void riccati_step(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& w
     }
   }
   if (!mfma) gemm_lds<T, false, false, 0>(n, n + m, n, L.S, n, L.A, ldab, L.T1, ldab);
-  __syncthreads();
+  lds_sync();
   // B: H, [G | g]
   if constexpr (IsF32<T>::value) {
     if (mfma) {
@@ -169,13 +195,14 @@ __device__ void riccati_step(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& w
   }
   for (int i = tid; i < m; i += nt) {
     T v = L.r[i];
+#pragma unroll 8
     for (int k = 0; k < n; ++k) v += L.B[k * ldab + i] * L.Sc[k];
     L.G[i * ld + n] = v;
   }
-  __syncthreads();
+  lds_sync();
   // C: H <- H^{-1}
-  spd_inverse<T, MMAX>(L.H, L.Kb, m);
-  __syncthreads();
+  spd_inverse<T, MMAX>(L.H, L.Kb, L.piv, m);
+  lds_sync();
   // D: [K | k] = H^{-1} [G | g]
   if constexpr (IsF32<T>::value) {
     if (mfma) {
@@ -184,7 +211,7 @@ __device__ void riccati_step(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& w
     }
   }
   if (!mfma) gemm_lds<T, false, false, 0>(m, n + 1, m, L.H, m, L.G, ld, L.Kb, ld);
-  __syncthreads();
+  lds_sync();
   // E: S_t, s_t ; F_t = A - B K and f_t = c - B k (to the workspace)
   if constexpr (IsF32<T>::value) {
     if (mfma) {
@@ -213,14 +240,19 @@ __device__ void riccati_step(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& w
   }
   for (int i = tid; i < n; i += nt) {
     T v = L.q[i];
+#pragma unroll 8
     for (int k = 0; k < n; ++k) v += L.A[k * ldab + i] * L.Sc[k];
-    for (int r = 0; r < m; ++r) v -= L.G[r * ld + i] * L.Kb[r * ld + n];
-    L.s[i] = v;
     T w = L.c[i];
-    for (int r = 0; r < m; ++r) w -= L.B[i * ldab + r] * L.Kb[r * ld + n];
+#pragma unroll 4
+    for (int r = 0; r < m; ++r) {
+      const T kr = L.Kb[r * ld + n];
+      v -= L.G[r * ld + i] * kr;
+      w -= L.B[i * ldab + r] * kr;
+    }
+    L.s[i] = v;
     fo[i] = w;
   }
-  __syncthreads();
+  lds_sync();
 }
 
 template <typename T>
@@ -247,11 +279,11 @@ __device__ void riccati_range(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& 
   if (store_top) store_value_and_gain(P, ws, b, t_hi, L, false);
   for (int t = t_hi - 1; t >= t_lo; --t) {
     load_stage(P, prm, ws, in, b, t, L);
-    __syncthreads();
+    lds_sync();
     riccati_step<T, MMAX>(P, prm, ws, b, t, L);
     store_value_and_gain(P, ws, b, t, L, true);
   }
-  __syncthreads();
+  lds_sync();
 }
 
 // dx over stages [t_lo, t_hi) from dx_{t_lo} in x (LDS, n): dx_{t+1} = F_t dx_t + f_t.
@@ -265,52 +297,50 @@ __device__ void fwd_chain(const Prob<T>& P, const Ws<T>& ws, int b, int t_lo, in
   const int tid = threadIdx.x, nt = blockDim.x;
   const int nn = n * n;
   constexpr int PF = 16;  // prefetch registers per thread
-  T pre[PF];
   const bool fast = nn <= PF * nt;
-  auto fetch = [&](int t) {
-    const T* F = ws.F + ((size_t)b * N + t) * nn;
-#pragma unroll
-    for (int u = 0; u < PF; ++u) {
-      const int e = tid + u * nt;
-      pre[u] = e < nn ? F[e] : T(0);
-    }
-  };
-  auto spill = [&](T* dst) {
-#pragma unroll
-    for (int u = 0; u < PF; ++u) {
-      const int e = tid + u * nt;
-      if (e < nn) dst[e] = pre[u];
-    }
-  };
   T* cur = F0;
   T* nxt = F1;
-  if (t_lo < t_hi) {
-    if (fast) { fetch(t_lo); spill(cur); }
-    else load_block(cur, ws.F + ((size_t)b * N + t_lo) * nn, n, n, n);
-  }
-  __syncthreads();
+  if (t_lo < t_hi) load_block(cur, ws.F + ((size_t)b * N + t_lo) * nn, n, n, n);
+  lds_sync();
   for (int t = t_lo; t < t_hi; ++t) {
-    if (fast && t + 1 < t_hi) fetch(t + 1);
+    T pre[PF];
+    const bool more = t + 1 < t_hi;
+    if (fast && more) {
+      const T* Fn = ws.F + ((size_t)b * N + t + 1) * nn;
+#pragma unroll
+      for (int u = 0; u < PF; ++u) {
+        const int e = tid + u * nt;
+        pre[u] = e < nn ? Fn[e] : T(0);
+      }
+    }
     const T* f = ws.f + ((size_t)b * N + t) * n;
     T* dxo = ws.dx + ((size_t)b * N1 + t) * n;
     for (int i = tid; i < n; i += nt) {
       T v = f[i];
+#pragma unroll 8
       for (int j = 0; j < n; ++j) v += cur[i * n + j] * x[j];
       x2[i] = v;
       dxo[i] = x[i];
     }
-    if (t + 1 < t_hi) {
-      if (fast) spill(nxt);
-      else load_block(nxt, ws.F + ((size_t)b * N + t + 1) * nn, n, n, n);
+    if (more) {
+      if (fast) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+          const int e = tid + u * nt;
+          if (e < nn) nxt[e] = pre[u];
+        }
+      } else {
+        load_block(nxt, ws.F + ((size_t)b * N + t + 1) * nn, n, n, n);
+      }
     }
-    __syncthreads();
+    lds_sync();
     for (int i = tid; i < n; i += nt) x[i] = x2[i];
     T* tmp = cur; cur = nxt; nxt = tmp;
-    __syncthreads();
+    lds_sync();
   }
   if (last)
     for (int i = tid; i < n; i += nt) ws.dx[((size_t)b * N1 + t_hi) * n + i] = x[i];
-  __syncthreads();
+  lds_sync();
 }
 
 template <typename T>
@@ -345,6 +375,7 @@ __global__ void __launch_bounds__(256) k_fwd_stage(Prob<T> P, Iter<T> it, Ws<T> 
   T* dv = ws.dv + ((size_t)b * N1 + t) * n;
   for (int i = tid; i < n; i += nt) {
     T lam = s[i];
+#pragma unroll 8
     for (int j = 0; j < n; ++j) lam += sS[i * lds + j] * sx[j];
     const T v = V[i];
     const T d = lam - v;
@@ -388,6 +419,7 @@ __global__ void __launch_bounds__(512) k_lqr_seq(Prob<T> P, Prm<T> prm, Iter<T> 
   for (int i = tid; i < n; i += nt) L.s[i] = ws.qh[((size_t)b * N1 + N) * n + i];
   __syncthreads();
   riccati_range<T, MMAX>(P, prm, ws, in, b, 0, N, L, true);
+  __syncthreads();  // F_t, f_t stored by every wave are read below
   // dx chain: F buffers in A/T1 (each >= n*n), x in Sc, x2 in c
   for (int i = tid; i < n; i += nt) L.Sc[i] = ws.cdef[(size_t)b * N1 * n + i];
   __syncthreads();

@@ -218,10 +218,52 @@ class SQPSolver:
                                            _ptr(c)))
         return g, c
 
-    def shift(self, it, shift, u_reset=None):
+    def shift(self, it, shift, u_reset=None, reset=None):
         b = self.batch
-        reset = torch.zeros(b.B, dtype=torch.int32, device=b.device)
+        if reset is None:
+            reset = torch.zeros(b.B, dtype=torch.int32, device=b.device)
         ci = b.c_iterate(it)
         L.check(self.lib.mpxkkt_shift_warmstart(_stream(), C.byref(b.problem), int(shift), C.byref(ci),
                                                 _ptr(u_reset), _ptr(reset)))
         return reset
+
+    def capture(self, it, iters=1, shift=None):
+        """Record ``iters`` SQP iterations on the iterate ``it`` (and, if
+        ``shift`` is given, the warm-start shift after them) as one HIP graph.
+
+        The graph reads and writes the device arrays of ``it`` and of the
+        batch in place: refresh them with ``copy_`` between replays.  Every
+        kernel of the iteration is a plain launch (no memset or host sync), so
+        the captured graph replays the eager launch sequence exactly.
+        """
+        return SQPGraph(self, it, iters, shift)
+
+
+class SQPGraph:
+    """A captured sequence of SQP iterations; ``replay()`` launches it on the
+    current stream.  ``stats`` and ``reset`` are the tensors the last
+    captured iteration and the shift write."""
+
+    def __init__(self, solver, it, iters, shift):
+        dev = solver.batch.device
+        self.it = it
+        self.reset = (torch.zeros(solver.batch.B, dtype=torch.int32, device=dev)
+                      if shift is not None else None)
+        # one eager pass on a copy first, so that one-time set-up (code object
+        # load, LDS attribute) happens outside the capture
+        scratch = {k: v.clone() for k, v in it.items()}
+        solver.iterate(scratch)
+        side = torch.cuda.Stream(device=dev)
+        side.wait_stream(torch.cuda.current_stream(dev))
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph, stream=side):
+            for _ in range(iters):
+                solver.iterate(it)
+            if shift is not None:
+                solver.shift(it, shift, reset=self.reset)
+        torch.cuda.current_stream(dev).wait_stream(side)
+        self.stats = solver.stats_t
+
+    def replay(self):
+        self.graph.replay()
+        return self.stats

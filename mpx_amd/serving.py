@@ -194,35 +194,46 @@ class MPCServer:
         for ev, chunk, u0, reset in launched:
             ev.synchronize()
             t = time.perf_counter()
-            u0h = u0.cpu().numpy()
-            rh = reset.cpu().numpy()
+            u0h = u0.numpy()
+            rh = reset.numpy()
             for j, r in enumerate(chunk):
-                r.u0, r.reset, r.t_done = u0h[j], bool(rh[j]), t
+                r.u0, r.reset, r.t_done = u0h[j].copy(), bool(rh[j]), t
                 self.controllers[r.cid].ticks += 1
                 done.append(r)
         return done
 
     def _launch(self, b, chunk, iters, stream, si):
+        """Gather -> iterate -> shift -> scatter for one group on ``stream``.
+
+        Every device tensor of the launch is allocated while ``stream`` is
+        current, so the caching allocator orders its reuse after this stream's
+        work; host inputs and outputs go through pinned buffers with
+        asynchronous copies (the pinned allocator holds a buffer until its copy
+        has completed)."""
         k = len(chunk)
-        slots = torch.tensor([self.controllers[r.cid].slot for r in chunk], dtype=torch.long, device=self.device)
-        x0 = torch.as_tensor(np.stack([r.x0 for r in chunk]), dtype=self.dtype, device=self.device)
+        slots_h = torch.tensor([self.controllers[r.cid].slot for r in chunk], dtype=torch.long).pin_memory()
+        x0_h = torch.from_numpy(np.stack([r.x0 for r in chunk])).to(self.dtype).pin_memory()
         with torch.cuda.stream(stream):
+            slots = slots_h.to(self.device, non_blocking=True)
             st, st_it = b.stage[si], b.stage_it[si]
             for key in _PROBLEM_KEYS:
                 src = b.slots.t.get(key)
                 if src is not None:
-                    st.t[key][:k].copy_(src.index_select(0, slots))
-            st.t["x0"][:k].copy_(x0)
+                    torch.index_select(src, 0, slots, out=st.t[key][:k])
+            st.t["x0"][:k].copy_(x0_h, non_blocking=True)
             for key in _ITER_KEYS:
-                st_it[key][:k].copy_(b.state[key].index_select(0, slots))
+                torch.index_select(b.state[key], 0, slots, out=st_it[key][:k])
             it = {key: st_it[key][:k] for key in _ITER_KEYS}
             solver = b.solver(k, si)
             for _ in range(iters):
                 solver.iterate(it)
-            u0 = it["U"][:, 0, :].clone()
+            u0_h = torch.empty((k, st.m), dtype=self.dtype, pin_memory=True)
+            u0_h.copy_(it["U"][:, 0, :], non_blocking=True)
             reset = solver.shift(it, self.shift)
+            reset_h = torch.empty(k, dtype=torch.int32, pin_memory=True)
+            reset_h.copy_(reset, non_blocking=True)
             for key in _ITER_KEYS:
                 b.state[key].index_copy_(0, slots, it[key])
             ev = torch.cuda.Event()
             ev.record(stream)
-        return ev, chunk, u0, reset
+        return ev, chunk, u0_h, reset_h

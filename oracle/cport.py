@@ -6,7 +6,11 @@ the numpy oracle) and by bench.py's cpu_baseline leg.
 from __future__ import annotations
 
 import ctypes as C
+import hashlib
 import os
+import shutil
+import subprocess
+import tempfile
 import time
 
 import numpy as np
@@ -15,19 +19,49 @@ from . import pdilqr as orc
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(_HERE, "_build", "libpdilqr_c.so")
+SRC = os.path.join(_HERE, "pdilqr_c.c")
 _lib = None
+_lib_path = None
 
 
 def available():
-    return os.path.exists(LIB)
+    return os.path.exists(LIB) or shutil.which("gcc") is not None
+
+
+def native_build():
+    """Compile the restatement with -march=native for the host this runs on
+    (into a temp folder keyed by the source hash); returns the path or None."""
+    gcc = shutil.which("gcc")
+    if gcc is None or not os.path.exists(SRC):
+        return None
+    with open(SRC, "rb") as f:
+        tag = hashlib.sha1(f.read()).hexdigest()[:12]
+    out_dir = os.path.join(tempfile.gettempdir(), f"mpx_oracle_{os.getuid()}_{tag}")
+    out = os.path.join(out_dir, "libpdilqr_c_native.so")
+    if not os.path.exists(out):
+        os.makedirs(out_dir, exist_ok=True)
+        tmp = out + f".{os.getpid()}.tmp"
+        r = subprocess.run([gcc, "-O3", "-march=native", "-fopenmp", "-fPIC", "-std=c11", "-shared",
+                            "-o", tmp, SRC, "-lm"], capture_output=True)
+        if r.returncode != 0:
+            return None
+        os.replace(tmp, out)
+    return out
 
 
 def _load():
-    global _lib
+    global _lib, _lib_path
     if _lib is None:
-        _lib = C.CDLL(LIB)
+        path = native_build() or LIB
+        _lib = C.CDLL(path)
         _lib.pdilqr_solve_batch.restype = C.c_int
+        _lib_path = path
     return _lib
+
+
+def build_flags():
+    _load()
+    return "-O3 -march=native" if _lib_path != LIB else "-O3 -march=x86-64-v3"
 
 
 def _arr(a):
@@ -89,4 +123,4 @@ def bench(insts, starts, iters=3, budget_s=15.0, threads=0):
         k *= 4
     return dict(inst_iters_per_s=sample * iters / dt, cores=threads, kind="port",
                 sample=f"{sample} instances x {iters} SQP iterations of the FP64 C restatement "
-                       f"(oracle/pdilqr_c.c, -O3 -march=native, OpenMP), {dt:.2f} s")
+                       f"(oracle/pdilqr_c.c, {build_flags()}, OpenMP), {dt:.2f} s")

@@ -84,7 +84,8 @@ def test_mpc_matches_oracle_step(wl, shape):
     for name, got in zip(names, out):
         want = ref[name]
         assert got.shape == want.shape, name
-        assert np.max(np.abs(got - want)) <= 1e-9 * max(1.0, np.max(np.abs(want))), name
+        if want.size:
+            assert np.max(np.abs(got - want)) <= 1e-9 * max(1.0, np.max(np.abs(want))), name
 
 
 @pytest.mark.gpu
