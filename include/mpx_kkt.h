@@ -99,7 +99,9 @@ typedef struct {
   int32_t segments;       /* time-parallel scan segments per instance (1 = sequential Riccati) */
   int32_t block_threads;  /* threads per block of the per-instance kernels (0 = auto) */
   int32_t trial_group;    /* trials evaluated together (0 = all); 1 if limited_memory */
-  int32_t flags;          /* bit 0: FP32 products on VALU instead of the matrix cores */
+  int32_t flags;          /* bit 0: FP32 products on VALU instead of the matrix cores;
+                             bit 1: no register-resident wave LQR kernel (FP32,
+                             n + 1 <= 48, m <= 16, n_eq <= 16, one segment) */
 } mpxkkt_params;
 
 /* Per-instance outputs of one SQP iteration (each may be NULL). */
@@ -147,11 +149,14 @@ mpxkkt_status mpxkkt_shift_warmstart(void* stream, const mpxkkt_problem* prob, i
                                      int32_t* reset);
 
 /* Phase timing.  While enabled, every mpxkkt_sqp_iterate records HIP events on
- * its stream at the phase boundaries (linearise | LQR | line search).
- * mpxkkt_profile_collect waits for the recorded events, returns the summed
- * milliseconds per phase and the number of calls, and clears the record. */
+ * its stream at the phase boundaries: linearise | LQR recursion (backward
+ * Riccati or scan and the forward state chain) | costates and step
+ * assembly | line search.  mpxkkt_profile_collect waits for the recorded
+ * events, returns the summed milliseconds per phase and the number of calls,
+ * and clears the record. */
+#define MPXKKT_PHASES 4
 mpxkkt_status mpxkkt_profile_enable(int32_t on);
-mpxkkt_status mpxkkt_profile_collect(double ms_per_phase[3], int32_t* calls);
+mpxkkt_status mpxkkt_profile_collect(double ms_per_phase[MPXKKT_PHASES], int32_t* calls);
 
 const char* mpxkkt_last_error(void);
 int32_t mpxkkt_version(void);

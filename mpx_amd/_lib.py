@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libmpx_kkt.so")
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_WORKSPACE, ERR_DEVICE = 0, 1, 2, 3, 4
 F32, F64 = 0, 1
 MODEL_LTV, MODEL_TANH = 0, 1
+PHASES = 4          # MPXKKT_PHASES
 
 EXPORTED = (
     "mpxkkt_default_params",
@@ -95,7 +96,7 @@ def load():
     lib.mpxkkt_shift_warmstart.argtypes = [C.c_void_p, P(Problem), C.c_int32, P(Iterate),
                                            C.c_void_p, C.c_void_p]
     lib.mpxkkt_profile_enable.argtypes = [C.c_int32]
-    lib.mpxkkt_profile_collect.argtypes = [P(C.c_double * 3), P(C.c_int32)]
+    lib.mpxkkt_profile_collect.argtypes = [P(C.c_double * PHASES), P(C.c_int32)]
     lib.mpxkkt_last_error.restype = C.c_char_p
     lib.mpxkkt_version.restype = C.c_int32
     for name in EXPORTED[:-2]:
@@ -123,8 +124,9 @@ def profile_enable(on=True):
 
 
 def profile_collect():
-    """(ms per phase [linearise, LQR, line search] summed over calls, calls)."""
-    ms = (C.c_double * 3)()
+    """(ms per phase [linearise, LQR recursion, costates, line search] summed
+    over calls, calls)."""
+    ms = (C.c_double * PHASES)()
     calls = C.c_int32(0)
     check(load().mpxkkt_profile_collect(C.byref(ms), C.byref(calls)))
-    return [ms[0], ms[1], ms[2]], calls.value
+    return list(ms), calls.value

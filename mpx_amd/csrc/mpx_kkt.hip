@@ -13,6 +13,7 @@
 #include "common.h"
 #include "k_linearize.h"
 #include "k_lqr.h"
+#include "k_lqr_wave.h"
 #include "k_scan.h"
 #include "k_linesearch.h"
 #include "k_misc.h"
@@ -283,19 +284,46 @@ mpxkkt_status run_linearize(hipStream_t st, const mpxkkt_problem* pr, const Prob
   return launch_check("k_linearize");
 }
 
+// The register-resident wave kernel covers FP32 on the matrix cores with
+// n + 1 <= 48, m <= 16, p <= 16 and one segment (flags bit 1 turns it off).
+template <typename T>
+bool use_wave_lqr(const mpxkkt_problem* pr, const Prm<T>& R) {
+  if constexpr (!IsF32<T>::value) {
+    return false;
+  } else {
+    return (R.flags & 3) == 0 && R.P <= 1 && pr->n + 1 <= 48 && pr->m <= 16 && pr->n_eq <= 16;
+  }
+}
+
+template <typename T>
+mpxkkt_status run_lqr_wave(hipStream_t st, const mpxkkt_problem* pr, const Prob<T>& P, const Prm<T>& R,
+                           const Iter<T>& I, const Ws<T>& W, const LqrIn<T>& in) {
+  if constexpr (IsF32<T>::value) {
+    const int NT = (pr->n + 1 + 15) / 16;
+    if (NT == 1) hipLaunchKernelGGL(k_lqr_wave<1>, dim3(pr->batch), dim3(64), 0, st, P, R, I, W, in);
+    else if (NT == 2) hipLaunchKernelGGL(k_lqr_wave<2>, dim3(pr->batch), dim3(64), 0, st, P, R, I, W, in);
+    else hipLaunchKernelGGL(k_lqr_wave<3>, dim3(pr->batch), dim3(64), 0, st, P, R, I, W, in);
+    return launch_check("k_lqr_wave");
+  } else {
+    return fail(MPXKKT_ERR_UNSUPPORTED, "wave LQR kernel is FP32 only");
+  }
+}
+
 template <typename T, int MMAX>
 mpxkkt_status run_lqr_m(hipStream_t st, const mpxkkt_problem* pr, int nt, const Prob<T>& P, const Prm<T>& R,
                         const Iter<T>& I, const Ws<T>& W, const LqrIn<T>& in) {
   mpxkkt_status s;
   if (R.P > 1) {
     if ((s = run_scan<T, MMAX>(st, pr, nt, P, R, I, W, in)) != MPXKKT_OK) return s;
+  } else if (use_wave_lqr<T>(pr, R)) {
+    return run_lqr_wave(st, pr, P, R, I, W, in);
   } else {
     const size_t lds = ric_smem_elems<T>(pr->n, pr->m, pr->n_eq, nt) * sizeof(T);
     if ((s = prep_lds(k_lqr_seq<T, MMAX>, lds)) != MPXKKT_OK) return s;
     hipLaunchKernelGGL((k_lqr_seq<T, MMAX>), dim3(pr->batch), dim3(nt), lds, st, P, R, I, W, in);
     if ((s = launch_check("k_lqr_seq")) != MPXKKT_OK) return s;
   }
-  return run_fwd_stage<T>(st, pr, P, I, W);
+  return MPXKKT_OK;
 }
 
 template <typename T>
@@ -326,7 +354,7 @@ mpxkkt_status run_linesearch(hipStream_t st, const mpxkkt_problem* pr, const Pro
 
 // phase-timing record (host-side; events on the caller's stream)
 struct PhaseEvents {
-  hipEvent_t ev[4];
+  hipEvent_t ev[MPXKKT_PHASES + 1];
 };
 bool g_prof_on = false;
 std::vector<PhaseEvents> g_prof;
@@ -349,7 +377,7 @@ mpxkkt_status sqp_iterate_t(hipStream_t st, const mpxkkt_problem* pr, const mpxk
   PhaseEvents* pe = nullptr;
   if (g_prof_on) {
     PhaseEvents e;
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k <= MPXKKT_PHASES; ++k)
       if (hipEventCreate(&e.ev[k]) != hipSuccess) return fail(MPXKKT_ERR_DEVICE, "hipEventCreate failed");
     g_prof.push_back(e);
     pe = &g_prof.back();
@@ -359,8 +387,10 @@ mpxkkt_status sqp_iterate_t(hipStream_t st, const mpxkkt_problem* pr, const mpxk
   prof_mark(st, pe, 1);
   if ((s = run_lqr<T>(st, pr, prm, P, R, I, W, in)) != MPXKKT_OK) return s;
   prof_mark(st, pe, 2);
-  s = run_linesearch<T>(st, pr, P, R, I, W, in, so);
+  if ((s = run_fwd_stage<T>(st, pr, P, I, W)) != MPXKKT_OK) return s;
   prof_mark(st, pe, 3);
+  s = run_linesearch<T>(st, pr, P, R, I, W, in, so);
+  prof_mark(st, pe, 4);
   return s;
 }
 
@@ -489,17 +519,18 @@ mpxkkt_status mpxkkt_profile_enable(int32_t on) {
   return MPXKKT_OK;
 }
 
-mpxkkt_status mpxkkt_profile_collect(double ms_per_phase[3], int32_t* calls) {
+mpxkkt_status mpxkkt_profile_collect(double ms_per_phase[MPXKKT_PHASES], int32_t* calls) {
   if (!ms_per_phase || !calls) return fail(MPXKKT_ERR_ARG, "null output");
-  ms_per_phase[0] = ms_per_phase[1] = ms_per_phase[2] = 0.0;
+  for (int k = 0; k < MPXKKT_PHASES; ++k) ms_per_phase[k] = 0.0;
   mpxkkt_status s = MPXKKT_OK;
   for (auto& e : g_prof) {
-    if (hipEventSynchronize(e.ev[3]) != hipSuccess) s = fail(MPXKKT_ERR_DEVICE, "hipEventSynchronize failed");
-    for (int k = 0; k < 3; ++k) {
+    if (hipEventSynchronize(e.ev[MPXKKT_PHASES]) != hipSuccess)
+      s = fail(MPXKKT_ERR_DEVICE, "hipEventSynchronize failed");
+    for (int k = 0; k < MPXKKT_PHASES; ++k) {
       float ms = 0.f;
       if (hipEventElapsedTime(&ms, e.ev[k], e.ev[k + 1]) == hipSuccess) ms_per_phase[k] += ms;
     }
-    for (int k = 0; k < 4; ++k) (void)hipEventDestroy(e.ev[k]);
+    for (int k = 0; k <= MPXKKT_PHASES; ++k) (void)hipEventDestroy(e.ev[k]);
   }
   *calls = (int32_t)g_prof.size();
   g_prof.clear();

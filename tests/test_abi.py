@@ -44,3 +44,9 @@ def test_argument_errors_without_device():
                                    ctypes.byref(nbytes))
     assert st == _lib.ERR_ARG
     assert b"batch" in lib.mpxkkt_last_error()
+
+
+def test_phase_count_matches_header():
+    src = open(os.path.join(ROOT, "include", "mpx_kkt.h")).read()
+    m = re.search(r"#define\s+MPXKKT_PHASES\s+(\d+)", src)
+    assert m and int(m.group(1)) == _lib.PHASES

@@ -127,3 +127,27 @@ def test_shift_and_nan_fallback():
             assert np.array_equal(it[k][i].cpu().numpy(), orc.shift_warmstart(before[k][i], 2))
     assert np.all(it["X"][1].cpu().numpy() == insts[1]["x0"][None])
     assert np.all(it["U"][1].cpu().numpy() == 0) and np.all(it["V"][1].cpu().numpy() == 0)
+
+
+@pytest.mark.parametrize("wl,shape,iters", [
+    (1, dict(n=12, m=12, N=20), 1),            # LTV, one tile (n + 1 <= 16)
+    (2, dict(n=15, m=16, N=14), 3),            # box, n + 1 = 16 exactly, m = 16
+    (3, dict(n=16, m=3, N=11, p=16), 2),       # two tiles, p = 16
+    (3, dict(n=31, m=7, N=9, p=5), 2),         # n + 1 = 32
+    (4, dict(n=47, m=16, N=6, p=16), 4),       # three tiles full, box and rows
+])
+def test_fp32_register_wave_kernel(wl, shape, iters):
+    """FP32 with the register-resident wave LQR kernel (default for these
+    shapes) against the FP64 oracle, and against the LDS block kernel
+    (flags bit 1) on the same inputs."""
+    insts, starts, refs, out, _, _ = _run(wl, 3, iters, torch.float32, shape=shape)
+    _, _, _, out_blk, _, _ = _run(wl, 3, iters, torch.float32, shape=shape, flags=2)
+    for k in range(iters):
+        g, st = out[k]
+        gb, _ = out_blk[k]
+        for i in range(3):
+            r = refs[i][k]
+            assert st["status"][i] == 0
+            for key, tol in (("X", TOL32_X), ("U", 10 * TOL32_X), ("V", 10 * TOL32_X)):
+                assert _rel(g[key][i], r[key]) < tol, (k, i, key, _rel(g[key][i], r[key]))
+                assert _rel(g[key][i], gb[key][i]) < tol, (k, i, key, _rel(g[key][i], gb[key][i]))
