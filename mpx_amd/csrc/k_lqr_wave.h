@@ -13,26 +13,32 @@
 // Homogeneous coordinates fold the affine parts into the matrices: with
 // x~ = [x; 1] (n + 1 <= 16 NT),
 //   A~ = [[A, c], [0, 1]],  B~ = [B; 0],  S~ = [[S, s], [s', 0]],
-//   Q~ = [[Q + rho_c D'D, q], [q', 0]],   G~ = [G | g],  K~ = [K | k],
+//   Q~ = [[Q, q], [q', 0]],  G~ = [G | g],  K~ = [K | k],
 // and one Riccati step
 //   H  = R + diag(rd) + rho_c E'E + B~' S~ B~
 //   G~ = B~' S~ A~ + rho_c E' [D | 0] + [0 | r]
 //   K~ = H^{-1} G~
-//   S~ <- Q~ + A~' S~ A~ - G~' K~ ,   F~ = A~ - B~ K~ = [[A - B K, c - B k], [0, 1]]
+//   S~ <- Q~ + rho_c [D|0]'[D|0] + A~' S~ A~ - G~' K~ ,
+//   F~ = A~ - B~ K~ = [[A - B K, c - B k], [0, 1]]
 // is the affine step of k_lqr.h (S_t = Q + rho_c D'D + A'SA - G'K,
 // s_t = q + A'(S c + s) - G'k, F_t = A - BK, f_t = c - Bk) with the vectors
 // riding in column n.  The corner of S~ is reset to 0 every step.
 //
 // The m x m inverse is an in-place Gauss-Jordan sweep in one tile (H is SPD:
 // no pivoting), with ds_bpermute broadcasts of the pivot row and column; the
-// symmetric tiles are mirrored through a 1 KB LDS transpose.
-// The forward chain x~_{t+1} = F~_t x~_t runs on the matrix cores too, with x~
-// as column 0 of a B-operand tile: the result tile is again in that layout.
+// symmetric tiles are mirrored through a 1 KB LDS transpose.  The forward chain
+// x~_{t+1} = F~_t x~_t runs on the matrix cores too, with x~ as column 0 of a
+// B-operand tile: the result tile is again in that layout.
 //
-// Memory goes through raw buffer descriptors (one per array, in SGPRs) with a
-// 32-bit offset per lane: an offset past the descriptor's record count reads 0
-// and drops a store, which masks the ragged tile edges without branches or
-// 64-bit addresses.
+// Memory goes through raw buffer descriptors.  A tile row (fixed v) of a
+// row-major matrix is one descriptor whose base is the row 16 it + v and whose
+// record count ends at the matrix's last row; each lane adds a loop-invariant
+// offset (4 g ld + col) * 4 that is set past every record count for a column
+// outside the matrix.  Out-of-range loads return 0 and out-of-range stores are
+// dropped, so the ragged tile edges cost no branches and no per-load address
+// arithmetic.  Each stage's data (including Q~) is requested before the
+// previous stage's results are stored: vmcnt counts loads and stores in issue
+// order, so waiting for the data never waits for the stores.
 #pragma once
 
 #include "common.h"
@@ -49,20 +55,17 @@ constexpr int kOOB = 0x40000000;
 __device__ __forceinline__ rsrc_t make_rsrc(const float* p, int nelem) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, nelem * 4, 0x00020000);
 }
+// rows [r0, nrows) of a row-major matrix with leading dimension ld
+__device__ __forceinline__ rsrc_t rows_rsrc(const float* M, int ld, int r0, int nrows) {
+  const int rem = nrows - r0;
+  return make_rsrc(M + (long long)r0 * ld, rem > 0 ? rem * ld : 0);
+}
 // (the b32 buffer builtins move raw 32-bit words: bit-cast, never convert)
-__device__ __forceinline__ float bload(rsrc_t r, int idx, bool ok) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, ok ? idx * 4 : kOOB, 0, 0));
+__device__ __forceinline__ float rload(rsrc_t r, int voff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0));
 }
-__device__ __forceinline__ void bstore(rsrc_t r, int idx, bool ok, float v) {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, ok ? idx * 4 : kOOB, 0, 0);
-}
-
-// An opaque copy: offsets computed from it inside a loop stay inside the loop
-// (loop-invariant per-lane offsets hoisted out of the stage loop would hold
-// hundreds of VGPRs for its whole length).
-__device__ __forceinline__ int fresh(int x) {
-  asm volatile("" : "+v"(x));
-  return x;
+__device__ __forceinline__ void rstore(rsrc_t r, int voff, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, 0, 0);
 }
 
 __device__ __forceinline__ f4 mma(float a, float b, f4 c) {
@@ -107,92 +110,63 @@ __device__ __forceinline__ void gj_inverse(f4& h, int m, int g, int c) {
   }
 }
 
-// Vector in tile-row layout: lane (g, c) holds v[16 it + 4 g + k] (0 past len).
+// Loop-invariant lane offsets (bytes) of one instance's shapes.
 template <int NT>
-__device__ __forceinline__ void load_rows(f4 (&out)[NT], rsrc_t r, int g) {
+struct LaneOff {
+  int nn[NT];   // ld n, column 16 jt + c < n   : (4 g n + 16 jt + c) * 4
+  int mm;       // ld m, column c < m           : (4 g m + c) * 4
+  int bt[4];    // B' tiles, column 4 g + v < m : (c m + 4 g + v) * 4
+  int g16;      // vector in tile-row layout    : 16 g
+  int c4;       // vector along the columns     : 4 c
+  int hom;      // column n to its vector       : 16 g on lane c == n % 16
+  int col0;     // column 0 of a B operand      : 16 g on lane c == 0
+  int cn;       // n % 16
+};
+
+template <int NT>
+__device__ __forceinline__ LaneOff<NT> lane_offsets(int n, int m, int g, int c) {
+  LaneOff<NT> L;
 #pragma unroll
-  for (int it = 0; it < NT; ++it)
+  for (int jt = 0; jt < NT; ++jt) L.nn[jt] = (16 * jt + c < n) ? (4 * g * n + 16 * jt + c) * 4 : kOOB;
+  L.mm = (c < m) ? (4 * g * m + c) * 4 : kOOB;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) out[it][k] = bload(r, 16 * it + 4 * g + k, true);
+  for (int v = 0; v < 4; ++v) L.bt[v] = (4 * g + v < m) ? (c * m + 4 * g + v) * 4 : kOOB;
+  L.g16 = 16 * g;
+  L.c4 = 4 * c;
+  L.cn = n & 15;
+  L.hom = (c == L.cn) ? 16 * g : kOOB;
+  L.col0 = (c == 0) ? 16 * g : kOOB;
+  return L;
 }
 
-// C-layout tile (it, jt) of an n x n row-major matrix (descriptor of n*n records;
-// rows past n fall outside it), with `col_n` added in column n and `corner` at
-// (n, n): the homogeneous A~ = [[A, c], [0, 1]] and Q~ = [[Q, q], [q', 0]].
-__device__ __forceinline__ f4 load_hom(rsrc_t M, const f4& col_n, float row_n, float corner, int n, int it, int jt,
-                                       int g, int c) {
+// C-layout tile (it, jt) of a row-major matrix with ld n and nrows rows.
+template <int NT>
+__device__ __forceinline__ f4 tile_nn(const float* M, int n, int nrows, int it, int jt, const LaneOff<NT>& L) {
   f4 x;
-  const int col = 16 * jt + c;
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int row = 16 * it + 4 * g + v;
-    const float a = bload(M, row * n + col, col < n);
-    const float e = (col == n && row < n) ? col_n[v] : 0.f;
-    const float f = (row == n) ? (col < n ? row_n : (col == n ? corner : 0.f)) : 0.f;
-    x[v] = a + e + f;
-  }
+  for (int v = 0; v < 4; ++v) x[v] = rload(rows_rsrc(M, n, 16 * it + v, nrows), L.nn[jt]);
   return x;
 }
 
-// Store a C-layout tile of an (n+1)-wide homogeneous matrix: the leading n
-// columns to M (row-major, ld n) and column n to vec; rows past the
-// descriptors' record counts are dropped.
-__device__ __forceinline__ void store_hom(const f4& x, rsrc_t M, rsrc_t vec, int n, int it, int jt, int g, int c) {
-  const int col = 16 * jt + c;
+// vector entries v[16 it + 4 g + k] (0 past len) in tile-row layout
+template <int NT>
+__device__ __forceinline__ f4 rows_vec(const float* vec, int len, int it, const LaneOff<NT>& L) {
+  f4 x;
 #pragma unroll
-  for (int v = 0; v < 4; ++v) bstore(M, (16 * it + 4 * g + v) * n + col, col < n, x[v]);
+  for (int k = 0; k < 4; ++k) x[k] = rload(rows_rsrc(vec, 1, 16 * it + k, len), L.g16);
+  return x;
+}
+
+// Store tile (it, jt) of an (n+1)-wide homogeneous matrix: the first n columns
+// to M (ld n, nrows rows), column n to vec.
+template <int NT>
+__device__ __forceinline__ void store_hom(const f4& x, float* M, float* vec, int n, int nrows, int it, int jt,
+                                          const LaneOff<NT>& L) {
+#pragma unroll
+  for (int v = 0; v < 4; ++v) rstore(rows_rsrc(M, n, 16 * it + v, nrows), L.nn[jt], x[v]);
   if (jt == (n >> 4)) {   // the tile holding column n (uniform)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) bstore(vec, 16 * it + 4 * g + v, col == n, x[v]);
-  }
-}
-
-template <int NT>
-__device__ __forceinline__ void load_qbar(f4 (&S)[NT][NT], const float* Q, const float* q, int n, int g, int c,
-                                          bool upper_only) {
-  const rsrc_t rQ = make_rsrc(Q, n * n), rq = make_rsrc(q, n);
-  f4 qrow[NT];
-  load_rows<NT>(qrow, rq, g);
-#pragma unroll
-  for (int i = 0; i < NT; ++i)
-#pragma unroll
-    for (int j = 0; j < NT; ++j)
-      if (!upper_only || j >= i) {
-        const float qc = bload(rq, 16 * j + c, true);
-        S[i][j] = load_hom(rQ, qrow[i], qc, 0.f, n, i, j, g, c);
-      }
-}
-
-template <int NT>
-__device__ __forceinline__ void store_value_wave(const f4 (&S)[NT][NT], const Ws<float>& ws, int b, int t, int N1,
-                                                 int n, int g, int c) {
-  const rsrc_t rS = make_rsrc(ws.S + ((size_t)b * N1 + t) * n * n, n * n);
-  const rsrc_t rs = make_rsrc(ws.s + ((size_t)b * N1 + t) * n, n);
-#pragma unroll
-  for (int it = 0; it < NT; ++it)
-#pragma unroll
-    for (int jt = 0; jt < NT; ++jt) store_hom(S[it][jt], rS, rs, n, it, jt, g, c);
-}
-
-// A-operand tiles of F~ for one forward step: lane (g, c) of tile (i, k) holds
-// F~[16 i + c][16 k + 4 g + s] in register s.
-template <int NT>
-__device__ __forceinline__ void load_fbar_t(f4 (&Fa)[NT][NT], const Ws<float>& ws, int b, int t, int N, int n,
-                                            int g, int c) {
-  const rsrc_t rF = make_rsrc(ws.F + ((size_t)b * N + t) * n * n, n * n);
-  const rsrc_t rf = make_rsrc(ws.f + ((size_t)b * N + t) * n, n);
-#pragma unroll
-  for (int i = 0; i < NT; ++i) {
-    const int row = 16 * i + c;
-    const float fr = bload(rf, row, true);
-#pragma unroll
-    for (int k = 0; k < NT; ++k)
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const int col = 16 * k + 4 * g + s;
-        const float e = bload(rF, row * n + col, col < n);
-        Fa[i][k][s] = e + ((row < n && col == n) ? fr : 0.f) + ((row == n && col == n) ? 1.f : 0.f);
-      }
+    for (int v = 0; v < 4; ++v) rstore(rows_rsrc(vec, 1, 16 * it + v, nrows), L.hom, x[v]);
   }
 }
 
@@ -200,6 +174,7 @@ __device__ __forceinline__ void load_fbar_t(f4 (&Fa)[NT][NT], const Ws<float>& w
 template <int NT>
 struct WaveStage {
   f4 A[NT][NT];   // A~ = [[A, c], [0, 1]]
+  f4 Q[NT][NT];   // Q~ = [[Q, q], [q', 0]], tiles i <= j
   f4 B[NT];       // B~ tiles (i, 0)
   f4 BT[NT];      // B~' tiles (0, i): lane (g, c) holds B[16 i + c][4 g + v]
   f4 D[NT];       // sqrt(rho_c) [D | 0]
@@ -208,49 +183,127 @@ struct WaveStage {
   f4 r;           // r: lane (g, c) holds r[4 g + v]
 };
 
+// Q~ tiles (i <= j, or all) of Q (ld n) and q
 template <int NT>
-__device__ __forceinline__ void load_wave_stage(WaveStage<NT>& d, const Prob<float>& P, const Ws<float>& ws,
-                                                const LqrIn<float>& in, int b, int t, float sr, int g, int c) {
-  const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
-  const rsrc_t rA = make_rsrc(in.A.at(b, t), n * n);
-  const rsrc_t rB = make_rsrc(in.Bm.at(b, t), n * m);
-  {
-    f4 cvec[NT];
-    load_rows<NT>(cvec, make_rsrc(ws.cdef + ((size_t)b * N1 + t + 1) * n, n), g);
-#pragma unroll
-    for (int i = 0; i < NT; ++i)
-#pragma unroll
-      for (int j = 0; j < NT; ++j) d.A[i][j] = load_hom(rA, cvec[i], 0.f, 1.f, n, i, j, g, c);
-  }
+__device__ __forceinline__ void load_qbar(f4 (&S)[NT][NT], const float* Q, const float* q, int n, int g, int c,
+                                          bool upper_only, const LaneOff<NT>& L) {
+  const int tn = n >> 4;
 #pragma unroll
   for (int i = 0; i < NT; ++i)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      d.B[i][v] = bload(rB, (16 * i + 4 * g + v) * m + c, c < m);
-      d.BT[i][v] = bload(rB, (16 * i + c) * m + 4 * g + v, 4 * g + v < m);
+    for (int j = 0; j < NT; ++j) {
+      if (upper_only && j < i) continue;
+      f4 x = tile_nn<NT>(Q, n, n, i, j, L);
+      if (j == tn) {   // column n: q[16 i + 4 g + v]
+        const f4 qr = rows_vec<NT>(q, n, i, L);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) x[v] += (c == L.cn) ? qr[v] : 0.f;
+      }
+      if (i == tn) {   // row n: q[16 j + c]
+        const float qc = rload(make_rsrc(q + 16 * j, n - 16 * j > 0 ? n - 16 * j : 0), L.c4);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) x[v] += (4 * g + v == L.cn) ? qc : 0.f;
+      }
+      S[i][j] = x;
     }
+}
+
+template <int NT>
+__device__ __forceinline__ void load_wave_stage(WaveStage<NT>& d, const Prob<float>& P, const Ws<float>& ws,
+                                                const LqrIn<float>& in, int b, int t, float sr, int g, int c,
+                                                const LaneOff<NT>& L) {
+  const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1, tn = n >> 4;
+  const float* A = in.A.at(b, t);
+  const float* Bm = in.Bm.at(b, t);
+  const float* cd = ws.cdef + ((size_t)b * N1 + t + 1) * n;
+#pragma unroll
+  for (int i = 0; i < NT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      f4 x = tile_nn<NT>(A, n, n, i, j, L);
+      if (j == tn) {   // column n: c ; corner 1
+        const f4 cv = rows_vec<NT>(cd, n, i, L);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          x[v] += (c == L.cn) ? cv[v] : 0.f;
+          if (i == tn) x[v] += (c == L.cn && 4 * g + v == L.cn) ? 1.f : 0.f;
+        }
+      }
+      d.A[i][j] = x;
+    }
+  load_qbar<NT>(d.Q, P.Q.at(b, t), ws.qh + ((size_t)b * N1 + t) * n, n, g, c, true, L);
+#pragma unroll
+  for (int i = 0; i < NT; ++i) {
+    const rsrc_t rbt = rows_rsrc(Bm, m, 16 * i, n);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      d.B[i][v] = rload(rows_rsrc(Bm, m, 16 * i + v, n), L.mm);
+      d.BT[i][v] = rload(rbt, L.bt[v]);
+    }
+  }
   if (p > 0) {
-    const rsrc_t rD = make_rsrc(P.D.at(b, t), p * n), rE = make_rsrc(P.E.at(b, t), p * m);
+    const float* D = P.D.at(b, t);
+    const float* E = P.E.at(b, t);
 #pragma unroll
-    for (int j = 0; j < NT; ++j)
+    for (int j = 0; j < NT; ++j) d.D[j] = sr * tile_nn<NT>(D, n, p, 0, j, L);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) d.D[j][v] = sr * bload(rD, (4 * g + v) * n + 16 * j + c, 16 * j + c < n);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) d.E[v] = sr * bload(rE, (4 * g + v) * m + c, c < m);
+    for (int v = 0; v < 4; ++v) d.E[v] = sr * rload(rows_rsrc(E, m, v, p), L.mm);
   } else {
 #pragma unroll
     for (int j = 0; j < NT; ++j) d.D[j] = zero4();
     d.E = zero4();
   }
-  const rsrc_t rR = make_rsrc(P.R.at(b, t), m * m);
-  const rsrc_t rd = make_rsrc(ws.rd + ((size_t)b * N + t) * m, m);
-  const rsrc_t rr = make_rsrc(ws.rh + ((size_t)b * N + t) * m, m);
+  const float* R = P.R.at(b, t);
+  const float* rd = ws.rd + ((size_t)b * N + t) * m;
+  const float* r = ws.rh + ((size_t)b * N + t) * m;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int row = 4 * g + v;
-    const bool in_m = row < m && c < m;
-    d.H[v] = bload(rR, row * m + c, c < m) + bload(rd, row, row == c) + ((!in_m && row == c) ? 1.f : 0.f);
-    d.r[v] = bload(rr, row, true);
+    const float dv = rload(rows_rsrc(rd, 1, v, m), L.g16);
+    d.H[v] = rload(rows_rsrc(R, m, v, m), L.mm) + (row == c ? (row < m ? dv : 1.f) : 0.f);
+    d.r[v] = rload(rows_rsrc(r, 1, v, m), L.g16);
+  }
+}
+
+template <int NT>
+__device__ __forceinline__ void store_value_wave(const f4 (&S)[NT][NT], const Ws<float>& ws, int b, int t, int N1,
+                                                 int n, const LaneOff<NT>& L) {
+  float* So = ws.S + ((size_t)b * N1 + t) * n * n;
+  float* so = ws.s + ((size_t)b * N1 + t) * n;
+#pragma unroll
+  for (int it = 0; it < NT; ++it)
+#pragma unroll
+    for (int jt = 0; jt < NT; ++jt) store_hom<NT>(S[it][jt], So, so, n, n, it, jt, L);
+}
+
+// A-operand tiles of F~ for one forward step: lane (g, c) of tile (i, k) holds
+// F~[16 i + c][16 k + 4 g + s] in register s.  off[k][s] is the lane's byte
+// offset of column 16 k + 4 g + s from row 16 i (kOOB past column n - 1).
+template <int NT>
+__device__ __forceinline__ void load_fbar_t(f4 (&Fa)[NT][NT], const Ws<float>& ws, int b, int t, int N, int n,
+                                            int g, int c, const int (&off)[NT][4], const LaneOff<NT>& L) {
+  const float* F = ws.F + ((size_t)b * N + t) * n * n;
+  const float* f = ws.f + ((size_t)b * N + t) * n;
+  const int tn = n >> 4;
+#pragma unroll
+  for (int i = 0; i < NT; ++i) {
+    const rsrc_t rF = rows_rsrc(F, n, 16 * i, n);
+#pragma unroll
+    for (int k = 0; k < NT; ++k)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) Fa[i][k][s] = rload(rF, off[k][s]);
+    // column n: f[16 i + c] ; corner 1
+    const float fr = rload(rows_rsrc(f, 1, 16 * i, n), L.c4);
+#pragma unroll
+    for (int k = 0; k < NT; ++k)
+      if (k == tn) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const bool coln = 4 * g + s == L.cn;
+          Fa[i][k][s] += coln ? fr : 0.f;
+          if (i == tn) Fa[i][k][s] += (coln && c == L.cn) ? 1.f : 0.f;
+        }
+      }
   }
 }
 
@@ -265,15 +318,15 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
   const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
   __shared__ float tbuf[16 * 17];
   const float sr = sqrtf(prm.rho_c);
+  const LaneOff<NT> L = lane_offsets<NT>(n, m, g, c);
 
   f4 S[NT][NT];
-  load_qbar<NT>(S, P.QN.at(b, 0), ws.qh + ((size_t)b * N1 + N) * n, n, g, c, false);
-  store_value_wave<NT>(S, ws, b, N, N1, n, g, c);
+  load_qbar<NT>(S, P.QN.at(b, 0), ws.qh + ((size_t)b * N1 + N) * n, n, g, c, false, L);
+  store_value_wave<NT>(S, ws, b, N, N1, n, L);
 
   WaveStage<NT> cur;
-  load_wave_stage<NT>(cur, P, ws, in, b, N - 1, sr, g, c);
+  load_wave_stage<NT>(cur, P, ws, in, b, N - 1, sr, g, c, L);
   for (int t = N - 1; t >= 0; --t) {
-    const int g = fresh(lane >> 4), c = fresh(lane & 15);
     // SB = S~ B~ ; H = R + diag(rd) + rho_c E'E + B~' SB
     f4 SB[NT];
 #pragma unroll
@@ -290,8 +343,8 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
     for (int k = 0; k < NT; ++k)
 #pragma unroll
       for (int s = 0; s < 4; ++s) H = mma(cur.B[k][s], SB[k][s], H);
-    // G~ = [0 | r] + rho_c E'[D | 0] + B~' S~ A~ ; Pm = A~' S~ A~ (tiles i <= j),
-    // one column tile of S~ A~ at a time
+    // G~ = [0 | r] + rho_c E'[D | 0] + B~' S~ A~ ; Pm = Q~ + A~' S~ A~ (tiles
+    // i <= j), one column tile of S~ A~ at a time
     f4 G[NT];
 #pragma unroll
     for (int j = 0; j < NT; ++j)
@@ -324,7 +377,7 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
         for (int s = 0; s < 4; ++s) G[j] = mma(cur.B[k][s], SAc[k][s], G[j]);
 #pragma unroll
       for (int i = 0; i <= j; ++i) {
-        f4 acc = zero4();
+        f4 acc = cur.Q[i][j];
 #pragma unroll
         for (int k = 0; k < NT; ++k)
 #pragma unroll
@@ -350,39 +403,36 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
       K[j] = acc;
     }
 
-    // S~ <- Q~ + rho_c [D|0]'[D|0] + A~' S~ A~ - G~' K~ (tiles i <= j, then mirrored)
-    {
-      f4 Qb[NT][NT];
-      load_qbar<NT>(Qb, P.Q.at(b, t), ws.qh + ((size_t)b * N1 + t) * n, n, g, c, true);
+    // S~ <- Pm + rho_c [D|0]'[D|0] - G~' K~ (tiles i <= j, then mirrored)
 #pragma unroll
-      for (int i = 0; i < NT; ++i)
+    for (int i = 0; i < NT; ++i)
 #pragma unroll
-        for (int j = i; j < NT; ++j) {
-          f4 acc = Pm[i][j] + Qb[i][j];
+      for (int j = i; j < NT; ++j) {
+        f4 acc = Pm[i][j];
 #pragma unroll
-          for (int s = 0; s < 4; ++s) acc = mma(-G[i][s], K[j][s], acc);
-          if (p > 0) {
+        for (int s = 0; s < 4; ++s) acc = mma(-G[i][s], K[j][s], acc);
+        if (p > 0) {
 #pragma unroll
-            for (int s = 0; s < 4; ++s) acc = mma(cur.D[i][s], cur.D[j][s], acc);
-          }
-          S[i][j] = acc;
+          for (int s = 0; s < 4; ++s) acc = mma(cur.D[i][s], cur.D[j][s], acc);
         }
-#pragma unroll
-      for (int i = 0; i < NT; ++i) {
-        const f4 d = tile_transpose(S[i][i], tbuf, g, c);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) S[i][i][v] = 0.5f * (S[i][i][v] + d[v]);
-#pragma unroll
-        for (int j = i + 1; j < NT; ++j) S[j][i] = tile_transpose(S[i][j], tbuf, g, c);
+        S[i][j] = acc;
       }
-      // corner of S~ (the constant of the value function) is not carried
-      const int it_n = n >> 4, cn = n & 15;
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+      const f4 d = tile_transpose(S[i][i], tbuf, g, c);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) S[i][i][v] = 0.5f * (S[i][i][v] + d[v]);
+#pragma unroll
+      for (int j = i + 1; j < NT; ++j) S[j][i] = tile_transpose(S[i][j], tbuf, g, c);
+    }
+    {  // the corner of S~ (the constant of the value function) is not carried
+      const int tn = n >> 4;
 #pragma unroll
       for (int i = 0; i < NT; ++i)
-        if (i == it_n) {
+        if (i == tn) {
 #pragma unroll
           for (int v = 0; v < 4; ++v)
-            if (4 * g + v == cn && c == cn) S[i][i][v] = 0.f;
+            if (4 * g + v == L.cn && c == L.cn) S[i][i][v] = 0.f;
         }
     }
 
@@ -397,48 +447,51 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
         cur.A[i][j] = acc;
       }
 
-    // the next stage's data is requested before this stage's stores, so that
-    // waiting for it never waits for the stores (vmcnt counts both, in order)
+    // the next stage's data is requested before this stage's stores
     WaveStage<NT> nxt;
-    load_wave_stage<NT>(nxt, P, ws, in, b, t > 0 ? t - 1 : 0, sr, g, c);
+    load_wave_stage<NT>(nxt, P, ws, in, b, t > 0 ? t - 1 : 0, sr, g, c, L);
 
     // stores: S_t, s_t ; K_t, k_t ; F_t, f_t
-    store_value_wave<NT>(S, ws, b, t, N1, n, g, c);
+    store_value_wave<NT>(S, ws, b, t, N1, n, L);
     {
-      const rsrc_t rK = make_rsrc(ws.K + ((size_t)b * N + t) * m * n, m * n);
-      const rsrc_t rk = make_rsrc(ws.k + ((size_t)b * N + t) * m, m);
+      float* Ko = ws.K + ((size_t)b * N + t) * m * n;
+      float* ko = ws.k + ((size_t)b * N + t) * m;
 #pragma unroll
-      for (int j = 0; j < NT; ++j) store_hom(K[j], rK, rk, n, 0, j, g, c);
-      const rsrc_t rF = make_rsrc(ws.F + ((size_t)b * N + t) * n * n, n * n);
-      const rsrc_t rf = make_rsrc(ws.f + ((size_t)b * N + t) * n, n);
+      for (int j = 0; j < NT; ++j) store_hom<NT>(K[j], Ko, ko, n, m, 0, j, L);
+      float* Fo = ws.F + ((size_t)b * N + t) * n * n;
+      float* fo = ws.f + ((size_t)b * N + t) * n;
 #pragma unroll
       for (int i = 0; i < NT; ++i)
 #pragma unroll
-        for (int j = 0; j < NT; ++j) store_hom(cur.A[i][j], rF, rf, n, i, j, g, c);
+        for (int j = 0; j < NT; ++j) store_hom<NT>(cur.A[i][j], Fo, fo, n, n, i, j, L);
     }
     cur = nxt;
   }
   __syncthreads();  // F_t, f_t stored above are read back below
 
   // dx chain: x~ in column 0 of B-operand tiles; A operand = F~ read transposed,
-  // the next stage's tiles fetched while the current one is applied
+  // the next stage's tiles fetched before this stage's dx is stored
+  int off[NT][4];
+#pragma unroll
+  for (int k = 0; k < NT; ++k)
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int col = 16 * k + 4 * g + s;
+      off[k][s] = col < n ? (c * n + col) * 4 : kOOB;
+    }
   f4 x[NT];
   {
-    const rsrc_t r0 = make_rsrc(ws.cdef + (size_t)b * N1 * n, n);
+    const float* x0 = ws.cdef + (size_t)b * N1 * n;
 #pragma unroll
     for (int k = 0; k < NT; ++k)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = 16 * k + 4 * g + v;
-        x[k][v] = bload(r0, row, c == 0) + ((c == 0 && row == n) ? 1.f : 0.f);
-      }
+      for (int v = 0; v < 4; ++v)
+        x[k][v] = rload(rows_rsrc(x0, 1, 16 * k + v, n), L.col0) +
+                  ((c == 0 && 16 * k + 4 * g + v == n) ? 1.f : 0.f);
   }
   f4 Fa[NT][NT];
-  load_fbar_t<NT>(Fa, ws, b, 0, N, n, g, c);
+  load_fbar_t<NT>(Fa, ws, b, 0, N, n, g, c, off, L);
   for (int t = 0; t < N; ++t) {
-    const int g = fresh(lane >> 4), c = fresh(lane & 15);
-    f4 Fn[NT][NT];
-    if (t + 1 < N) load_fbar_t<NT>(Fn, ws, b, t + 1, N, n, g, c);
     f4 y[NT];
 #pragma unroll
     for (int i = 0; i < NT; ++i) {
@@ -449,24 +502,21 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
         for (int s = 0; s < 4; ++s) acc = mma(Fa[i][k][s], x[k][s], acc);
       y[i] = acc;
     }
-    const rsrc_t rdx = make_rsrc(ws.dx + ((size_t)b * N1 + t) * n, n);
+    load_fbar_t<NT>(Fa, ws, b, t + 1 < N ? t + 1 : t, N, n, g, c, off, L);
+    float* dxo = ws.dx + ((size_t)b * N1 + t) * n;
 #pragma unroll
     for (int k = 0; k < NT; ++k) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) bstore(rdx, 16 * k + 4 * g + v, c == 0, x[k][v]);
+      for (int v = 0; v < 4; ++v) rstore(rows_rsrc(dxo, 1, 16 * k + v, n), L.col0, x[k][v]);
       x[k] = y[k];
     }
-#pragma unroll
-    for (int i = 0; i < NT; ++i)
-#pragma unroll
-      for (int k = 0; k < NT; ++k) Fa[i][k] = Fn[i][k];
   }
   {
-    const rsrc_t rdx = make_rsrc(ws.dx + ((size_t)b * N1 + N) * n, n);
+    float* dxo = ws.dx + ((size_t)b * N1 + N) * n;
 #pragma unroll
     for (int k = 0; k < NT; ++k)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) bstore(rdx, 16 * k + 4 * g + v, c == 0, x[k][v]);
+      for (int v = 0; v < 4; ++v) rstore(rows_rsrc(dxo, 1, 16 * k + v, n), L.col0, x[k][v]);
   }
 }
 
