@@ -217,7 +217,7 @@ __device__ __forceinline__ void load_wave_stage(WaveStage<NT>& d, const Prob<flo
   const float* Bm = in.Bm.at(b, t);
   const float* cd = ws.cdef + ((size_t)b * N1 + t + 1) * n;
 #pragma unroll
-  for (int i = 0; i < NT; ++i)
+  for (int i = 0; i < NT; ++i) {
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
       f4 x = tile_nn<NT>(A, n, n, i, j, L);
@@ -231,6 +231,7 @@ __device__ __forceinline__ void load_wave_stage(WaveStage<NT>& d, const Prob<flo
       }
       d.A[i][j] = x;
     }
+  }
   load_qbar<NT>(d.Q, P.Q.at(b, t), ws.qh + ((size_t)b * N1 + t) * n, n, g, c, true, L);
 #pragma unroll
   for (int i = 0; i < NT; ++i) {

@@ -14,6 +14,7 @@
 #include "k_linearize.h"
 #include "k_lqr.h"
 #include "k_lqr_wave.h"
+#include "k_inst.h"
 #include "k_scan.h"
 #include "k_linesearch.h"
 #include "k_misc.h"
@@ -229,6 +230,18 @@ LqrIn<T> make_lqr_in(const mpxkkt_problem* pr, const Ws<T>& ws) {
   return in;
 }
 
+// The instance-per-block kernels (k_inst.h) take n, m, n_eq <= 64 when their
+// LDS plan fits the device.
+template <typename T>
+bool use_inst(const mpxkkt_problem* pr, bool with_v) {
+  if (pr->n > 64 || pr->m > 64 || pr->n_eq > 64) return false;
+  const InstLds L = inst_lds(pr->n, pr->m, pr->horizon, pr->model == MPXKKT_MODEL_TANH, pr->Q.stage_stride == 0,
+                             pr->R.stage_stride == 0, with_v);
+  int lim = 0;
+  if (lds_limit(&lim) != MPXKKT_OK) return false;
+  return (size_t)L.total * sizeof(T) <= (size_t)lim;
+}
+
 int k1_threads(int n) { return n <= 64 ? 64 : 128; }
 
 template <typename T, int MMAX>
@@ -275,6 +288,15 @@ mpxkkt_status run_fwd_stage(hipStream_t st, const mpxkkt_problem* pr, const Prob
 template <typename T>
 mpxkkt_status run_linearize(hipStream_t st, const mpxkkt_problem* pr, const Prob<T>& P, const Prm<T>& R,
                             const Iter<T>& I, const Ws<T>& W) {
+  if (use_inst<T>(pr, false)) {
+    const InstLds L = inst_lds(pr->n, pr->m, pr->horizon, pr->model == MPXKKT_MODEL_TANH, pr->Q.stage_stride == 0,
+                               pr->R.stage_stride == 0, false);
+    const size_t lds = (size_t)L.total * sizeof(T);
+    mpxkkt_status s = prep_lds(k_linearize_inst<T>, lds);
+    if (s != MPXKKT_OK) return s;
+    hipLaunchKernelGGL(k_linearize_inst<T>, dim3(pr->batch), dim3(64 * kInstWaves), lds, st, P, R, I, W);
+    return launch_check("k_linearize_inst");
+  }
   const int nt = k1_threads(pr->n);
   const size_t lds = ((size_t)2 * pr->n + pr->m + pr->n_eq + NMP * nt) * sizeof(T);
   mpxkkt_status s = prep_lds(k_linearize<T>, lds);
@@ -338,6 +360,22 @@ mpxkkt_status run_lqr(hipStream_t st, const mpxkkt_problem* pr, const mpxkkt_par
 template <typename T>
 mpxkkt_status run_linesearch(hipStream_t st, const mpxkkt_problem* pr, const Prob<T>& P, const Prm<T>& R,
                              const Iter<T>& I, const Ws<T>& W, const LqrIn<T>& in, const mpxkkt_stats* so) {
+  if (use_inst<T>(pr, true)) {
+    const InstLds L = inst_lds(pr->n, pr->m, pr->horizon, pr->model == MPXKKT_MODEL_TANH, pr->Q.stage_stride == 0,
+                               pr->R.stage_stride == 0, true);
+    const size_t lds = (size_t)L.total * sizeof(T);
+    mpxkkt_status s = prep_lds(k_linesearch_inst<T>, lds);
+    if (s != MPXKKT_OK) return s;
+    T* a = so ? static_cast<T*>(so->alpha) : nullptr;
+    T* r = so ? static_cast<T*>(so->rho) : nullptr;
+    T* f0 = so ? static_cast<T*>(so->merit0) : nullptr;
+    T* df = so ? static_cast<T*>(so->dmerit) : nullptr;
+    int* stt = so ? so->status : nullptr;
+    int* tr = so ? so->trials : nullptr;
+    hipLaunchKernelGGL(k_linesearch_inst<T>, dim3(pr->batch), dim3(64 * kInstWaves), lds, st, P, R, I, W, a, r, f0,
+                       df, stt, tr);
+    return launch_check("k_linesearch_inst");
+  }
   const int nt = 256;
   const size_t lds = (size_t)MAX_TRIALS * nt * sizeof(T);
   mpxkkt_status s = prep_lds(k_linesearch<T>, lds);
