@@ -89,6 +89,14 @@ __device__ __forceinline__ void lds_sync() {
   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
+// Sum over the 64 lanes of a wave (every lane gets the result).
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
 // Block-wide sum of `cnt` values per thread (cnt <= NV); blockDim.x must be a
 // power of two; sred holds NV*blockDim.x elements.  Every thread gets the sums.
 template <typename T, int NV>

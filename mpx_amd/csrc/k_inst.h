@@ -24,13 +24,6 @@ namespace mpxk {
 constexpr int kInstWaves = 4;
 constexpr int kInstScratch = 4 * 64;   // per wave: x - xr, u - ur, mu + rho_c h, sech^2(W x)
 
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-
 // LDS written and read by lanes of one wave: wait for the wave's LDS traffic
 // and keep the compiler from moving memory operations across.
 __device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }

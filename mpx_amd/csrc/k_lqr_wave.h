@@ -26,9 +26,12 @@
 //
 // The m x m inverse is an in-place Gauss-Jordan sweep in one tile (H is SPD:
 // no pivoting), with ds_bpermute broadcasts of the pivot row and column; the
-// symmetric tiles are mirrored through a 1 KB LDS transpose.  The forward chain
-// x~_{t+1} = F~_t x~_t runs on the matrix cores too, with x~ as column 0 of a
-// B-operand tile: the result tile is again in that layout.
+// symmetric tiles are mirrored through a 1 KB LDS transpose.  The forward pass
+// (k_fwd_wave) runs on the matrix cores too, with x~ as column 0 of a
+// B-operand tile: every product's result tile is again in that layout, so
+// x~_{t+1} = A~ x~ + B~ du and du = -K~ x~ chain from registers and
+// F = A - B K is never formed; the costates lam = S~ x~ follow in parallel over
+// the stages (k_costate_wave).
 //
 // Memory goes through raw buffer descriptors.  A tile row (fixed v) of a
 // row-major matrix is one descriptor whose base is the row 16 it + v and whose
@@ -176,7 +179,6 @@ struct WaveStage {
   f4 A[NT][NT];   // A~ = [[A, c], [0, 1]]
   f4 Q[NT][NT];   // Q~ = [[Q, q], [q', 0]], tiles i <= j
   f4 B[NT];       // B~ tiles (i, 0)
-  f4 BT[NT];      // B~' tiles (0, i): lane (g, c) holds B[16 i + c][4 g + v]
   f4 D[NT];       // sqrt(rho_c) [D | 0]
   f4 E;           // sqrt(rho_c) E
   f4 H;           // R + diag(rd), identity in the padding
@@ -234,14 +236,9 @@ __device__ __forceinline__ void load_wave_stage(WaveStage<NT>& d, const Prob<flo
   }
   load_qbar<NT>(d.Q, P.Q.at(b, t), ws.qh + ((size_t)b * N1 + t) * n, n, g, c, true, L);
 #pragma unroll
-  for (int i = 0; i < NT; ++i) {
-    const rsrc_t rbt = rows_rsrc(Bm, m, 16 * i, n);
+  for (int i = 0; i < NT; ++i)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      d.B[i][v] = rload(rows_rsrc(Bm, m, 16 * i + v, n), L.mm);
-      d.BT[i][v] = rload(rbt, L.bt[v]);
-    }
-  }
+    for (int v = 0; v < 4; ++v) d.B[i][v] = rload(rows_rsrc(Bm, m, 16 * i + v, n), L.mm);
   if (p > 0) {
     const float* D = P.D.at(b, t);
     const float* E = P.E.at(b, t);
@@ -277,39 +274,8 @@ __device__ __forceinline__ void store_value_wave(const f4 (&S)[NT][NT], const Ws
     for (int jt = 0; jt < NT; ++jt) store_hom<NT>(S[it][jt], So, so, n, n, it, jt, L);
 }
 
-// A-operand tiles of F~ for one forward step: lane (g, c) of tile (i, k) holds
-// F~[16 i + c][16 k + 4 g + s] in register s.  off[k][s] is the lane's byte
-// offset of column 16 k + 4 g + s from row 16 i (kOOB past column n - 1).
-template <int NT>
-__device__ __forceinline__ void load_fbar_t(f4 (&Fa)[NT][NT], const Ws<float>& ws, int b, int t, int N, int n,
-                                            int g, int c, const int (&off)[NT][4], const LaneOff<NT>& L) {
-  const float* F = ws.F + ((size_t)b * N + t) * n * n;
-  const float* f = ws.f + ((size_t)b * N + t) * n;
-  const int tn = n >> 4;
-#pragma unroll
-  for (int i = 0; i < NT; ++i) {
-    const rsrc_t rF = rows_rsrc(F, n, 16 * i, n);
-#pragma unroll
-    for (int k = 0; k < NT; ++k)
-#pragma unroll
-      for (int s = 0; s < 4; ++s) Fa[i][k][s] = rload(rF, off[k][s]);
-    // column n: f[16 i + c] ; corner 1
-    const float fr = rload(rows_rsrc(f, 1, 16 * i, n), L.c4);
-#pragma unroll
-    for (int k = 0; k < NT; ++k)
-      if (k == tn) {
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const bool coln = 4 * g + s == L.cn;
-          Fa[i][k][s] += coln ? fr : 0.f;
-          if (i == tn) Fa[i][k][s] += (coln && c == L.cn) ? 1.f : 0.f;
-        }
-      }
-  }
-}
-
-// segments == 1, FP32, n + 1 <= 16 NT, m <= 16, p <= 16: backward Riccati and
-// the dx chain of one instance per wave.
+// segments == 1, FP32, n + 1 <= 16 NT, m <= 16, p <= 16: backward Riccati of
+// one instance per wave; stores S~_t and K~_t for k_fwd_wave.
 template <int NT>
 __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> prm, Iter<float> it_unused,
                                                     Ws<float> ws, LqrIn<float> in) {
@@ -437,42 +403,70 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
         }
     }
 
-    // F~ = A~ - B~ K~ (in place of A~)
-#pragma unroll
-    for (int i = 0; i < NT; ++i)
-#pragma unroll
-      for (int j = 0; j < NT; ++j) {
-        f4 acc = cur.A[i][j];
-#pragma unroll
-        for (int s = 0; s < 4; ++s) acc = mma(-cur.BT[i][s], K[j][s], acc);
-        cur.A[i][j] = acc;
-      }
-
     // the next stage's data is requested before this stage's stores
     WaveStage<NT> nxt;
     load_wave_stage<NT>(nxt, P, ws, in, b, t > 0 ? t - 1 : 0, sr, g, c, L);
 
-    // stores: S_t, s_t ; K_t, k_t ; F_t, f_t
+    // stores: S_t, s_t ; K_t, k_t
     store_value_wave<NT>(S, ws, b, t, N1, n, L);
     {
       float* Ko = ws.K + ((size_t)b * N + t) * m * n;
       float* ko = ws.k + ((size_t)b * N + t) * m;
 #pragma unroll
       for (int j = 0; j < NT; ++j) store_hom<NT>(K[j], Ko, ko, n, m, 0, j, L);
-      float* Fo = ws.F + ((size_t)b * N + t) * n * n;
-      float* fo = ws.f + ((size_t)b * N + t) * n;
-#pragma unroll
-      for (int i = 0; i < NT; ++i)
-#pragma unroll
-        for (int j = 0; j < NT; ++j) store_hom<NT>(cur.A[i][j], Fo, fo, n, n, i, j, L);
     }
     cur = nxt;
   }
-  __syncthreads();  // F_t, f_t stored above are read back below
+}
 
-  // dx chain: x~ in column 0 of B-operand tiles; A operand = F~ read transposed,
-  // the next stage's tiles fetched before this stage's dx is stored
-  int off[NT][4];
+// A-operand tiles of an (n+1)-column homogeneous matrix with row-major block M
+// (ld n, nrows rows) and column n = colv: lane (g, c) of tile (i, k) holds
+// M~[16 i + c][16 k + 4 g + s] in register s, with 1 at (n, n) when corner.
+// off[k][s] is the lane's byte offset of column 16 k + 4 g + s within a row
+// (kOOB past column n - 1).
+template <int NT, int RT>
+__device__ __forceinline__ void load_aop(f4 (&out)[RT][NT], const float* M, const float* colv, int n, int nrows,
+                                         bool corner, int g, int c, const int (&off)[NT][4], const LaneOff<NT>& L) {
+#pragma unroll
+  for (int i = 0; i < RT; ++i) {
+    const rsrc_t rM = rows_rsrc(M, n, 16 * i, nrows);
+#pragma unroll
+    for (int k = 0; k < NT; ++k)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) out[i][k][s] = rload(rM, off[k][s]);
+    const float cv = rload(rows_rsrc(colv, 1, 16 * i, nrows), L.c4);   // colv[16 i + c]
+    const float cc = (corner && 16 * i + c == n) ? 1.f : 0.f;
+    // column n by a per-lane test (a test on the tile index alone lets the
+    // compiler turn the update into a run-time index into the tile array)
+#pragma unroll
+    for (int k = 0; k < NT; ++k)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) out[i][k][s] += (16 * k + 4 * g + s == n) ? cv + cc : 0.f;
+  }
+}
+
+// One forward step's A-operand tiles: A~_t, K~_t and B~_t.
+template <int NT>
+__device__ __forceinline__ void load_chain_stage(f4 (&A)[NT][NT], f4 (&K)[1][NT], f4 (&BT)[NT], const Ws<float>& ws,
+                                                 const LqrIn<float>& in, int b, int t, int N, int n, int m, int g,
+                                                 int c, const int (&off)[NT][4], const LaneOff<NT>& L) {
+  const int N1 = N + 1;
+  load_aop<NT, NT>(A, in.A.at(b, t), ws.cdef + ((size_t)b * N1 + t + 1) * n, n, n, true, g, c, off, L);
+  load_aop<NT, 1>(K, ws.K + ((size_t)b * N + t) * m * n, ws.k + ((size_t)b * N + t) * m, n, m, false, g, c, off,
+                  L);
+  const float* Bm = in.Bm.at(b, t);
+#pragma unroll
+  for (int i = 0; i < NT; ++i) {
+    const rsrc_t rbt = rows_rsrc(Bm, m, 16 * i, n);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) BT[i][s] = rload(rbt, L.bt[s]);   // B[16 i + c][4 g + s]
+  }
+}
+
+// lane's byte offsets of columns 16 k + 4 g + s within a row of ld n (kOOB
+// past column n - 1), for A-operand tiles
+template <int NT>
+__device__ __forceinline__ void aop_offsets(int (&off)[NT][4], int n, int g, int c) {
 #pragma unroll
   for (int k = 0; k < NT; ++k)
 #pragma unroll
@@ -480,6 +474,21 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
       const int col = 16 * k + 4 * g + s;
       off[k][s] = col < n ? (c * n + col) * 4 : kOOB;
     }
+}
+
+// Forward pass of the LQR solve, one wave per instance (the gains K~_t of
+// k_lqr_wave): with x~_t = [dx_t; 1] as column 0 of B-operand tiles,
+//   du_t = -K~_t x~_t,  x~_{t+1} = A~_t x~_t + B~_t du_t
+// on the matrix cores; writes dx and du.  The next stage's tiles are requested
+// before this stage's stores.
+template <int NT>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 3))) k_fwd_wave(Prob<float> P, Ws<float> ws, LqrIn<float> in) {
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
+  const int n = P.n, m = P.m, N = P.N, N1 = N + 1;
+  const LaneOff<NT> L = lane_offsets<NT>(n, m, g, c);
+  int off[NT][4];
+  aop_offsets<NT>(off, n, g, c);
   f4 x[NT];
   {
     const float* x0 = ws.cdef + (size_t)b * N1 * n;
@@ -490,9 +499,15 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
         x[k][v] = rload(rows_rsrc(x0, 1, 16 * k + v, n), L.col0) +
                   ((c == 0 && 16 * k + 4 * g + v == n) ? 1.f : 0.f);
   }
-  f4 Fa[NT][NT];
-  load_fbar_t<NT>(Fa, ws, b, 0, N, n, g, c, off, L);
+  f4 cA[NT][NT], cK[1][NT], cB[NT];
+  load_chain_stage<NT>(cA, cK, cB, ws, in, b, 0, N, n, m, g, c, off, L);
   for (int t = 0; t < N; ++t) {
+    // K~ x~ (tile 0, column 0), then x~_{t+1} = A~ x~ - B~ (K~ x~)
+    f4 kx = zero4();
+#pragma unroll
+    for (int k = 0; k < NT; ++k)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) kx = mma(cK[0][k][s], x[k][s], kx);
     f4 y[NT];
 #pragma unroll
     for (int i = 0; i < NT; ++i) {
@@ -500,24 +515,110 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
 #pragma unroll
       for (int k = 0; k < NT; ++k)
 #pragma unroll
-        for (int s = 0; s < 4; ++s) acc = mma(Fa[i][k][s], x[k][s], acc);
+        for (int s = 0; s < 4; ++s) acc = mma(cA[i][k][s], x[k][s], acc);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) acc = mma(-cB[i][s], kx[s], acc);
       y[i] = acc;
     }
-    load_fbar_t<NT>(Fa, ws, b, t + 1 < N ? t + 1 : t, N, n, g, c, off, L);
-    float* dxo = ws.dx + ((size_t)b * N1 + t) * n;
+    f4 nA[NT][NT], nK[1][NT], nB[NT];
+    load_chain_stage<NT>(nA, nK, nB, ws, in, b, t + 1 < N ? t + 1 : t, N, n, m, g, c, off, L);
+    {
+      const rsrc_t rdx = make_rsrc(ws.dx + ((size_t)b * N1 + t) * n, n);
+#pragma unroll
+      for (int k = 0; k < NT; ++k)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) rstore(rdx, L.col0 + (16 * k + v) * 4, x[k][v]);
+      const rsrc_t rdu = make_rsrc(ws.du + ((size_t)b * N + t) * m, m);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) rstore(rdu, L.col0 + v * 4, -kx[v]);
+    }
 #pragma unroll
     for (int k = 0; k < NT; ++k) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) rstore(rows_rsrc(dxo, 1, 16 * k + v, n), L.col0, x[k][v]);
       x[k] = y[k];
+      cK[0][k] = nK[0][k];
+      cB[k] = nB[k];
+#pragma unroll
+      for (int i = 0; i < NT; ++i) cA[i][k] = nA[i][k];
     }
   }
   {
-    float* dxo = ws.dx + ((size_t)b * N1 + N) * n;
+    const rsrc_t rdx = make_rsrc(ws.dx + ((size_t)b * N1 + N) * n, n);
 #pragma unroll
     for (int k = 0; k < NT; ++k)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) rstore(rows_rsrc(dxo, 1, 16 * k + v, n), L.col0, x[k][v]);
+      for (int v = 0; v < 4; ++v) rstore(rdx, L.col0 + (16 * k + v) * 4, x[k][v]);
+  }
+}
+
+// Costates and slope partials, one wave per (stage t, instance b):
+//   lam_t = (S~_t x~_t)[:n] on the matrix cores (x~_t = [dx_t; 1] as column 0 of
+//   B-operand tiles), dv_t = lam_t - V_t, and the partials q_t'dx_t + r_t'du_t,
+//   (dv_t - V_t)'c_t, lam_t'c_t of k_fwd_stage (c_t is the constraint row paired
+//   with V_t; c_0 = x0 - X_0).
+template <int NT>
+__global__ void __launch_bounds__(64) k_costate_wave(Prob<float> P, Iter<float> it, Ws<float> ws) {
+  const int t = blockIdx.x, b = blockIdx.y;
+  const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
+  const int n = P.n, m = P.m, N = P.N, N1 = N + 1;
+  const LaneOff<NT> L = lane_offsets<NT>(n, m, g, c);
+  int off[NT][4];
+  aop_offsets<NT>(off, n, g, c);
+  f4 Sa[NT][NT];
+  load_aop<NT, NT>(Sa, ws.S + ((size_t)b * N1 + t) * n * n, ws.s + ((size_t)b * N1 + t) * n, n, n, false, g, c,
+                   off, L);
+  f4 x[NT], qv[NT], vv[NT], cv[NT];
+  const float* dxp = ws.dx + ((size_t)b * N1 + t) * n;
+#pragma unroll
+  for (int k = 0; k < NT; ++k) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      x[k][v] = rload(rows_rsrc(dxp, 1, 16 * k + v, n), L.col0) + ((c == 0 && 16 * k + 4 * g + v == n) ? 1.f : 0.f);
+    qv[k] = rows_vec<NT>(ws.qh + ((size_t)b * N1 + t) * n, n, k, L);
+    vv[k] = rows_vec<NT>(it.V + ((size_t)b * N1 + t) * n, n, k, L);
+    cv[k] = rows_vec<NT>(ws.cdef + ((size_t)b * N1 + t) * n, n, k, L);
+  }
+  float df = 0.f, dc = 0.f, lc = 0.f;
+  const bool col0 = c == 0;
+  if (t < N) {   // r_t'du_t
+    const rsrc_t rr = make_rsrc(ws.rh + ((size_t)b * N + t) * m, m);
+    const rsrc_t ru = make_rsrc(ws.du + ((size_t)b * N + t) * m, m);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const float rv = rload(rr, (4 * g + v) * 4), uv = rload(ru, (4 * g + v) * 4);
+      df += col0 ? rv * uv : 0.f;
+    }
+  }
+  f4 lam[NT];
+#pragma unroll
+  for (int i = 0; i < NT; ++i) {
+    f4 acc = zero4();
+#pragma unroll
+    for (int k = 0; k < NT; ++k)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) acc = mma(Sa[i][k][s], x[k][s], acc);
+    lam[i] = acc;
+  }
+  const rsrc_t rdv = make_rsrc(ws.dv + ((size_t)b * N1 + t) * n, n);
+#pragma unroll
+  for (int k = 0; k < NT; ++k)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const bool row_ok = col0 && 16 * k + 4 * g + v < n;
+      const float dv = lam[k][v] - vv[k][v];
+      df += row_ok ? qv[k][v] * x[k][v] : 0.f;
+      dc += row_ok ? (dv - vv[k][v]) * cv[k][v] : 0.f;
+      lc += row_ok ? lam[k][v] * cv[k][v] : 0.f;
+      rstore(rdv, L.col0 + (16 * k + v) * 4, dv);
+    }
+  df = wave_sum(df);
+  dc = wave_sum(dc);
+  lc = wave_sum(lc);
+  if (lane == 0) {
+    float* fp = ws.fpart + ((size_t)b * N1 + t) * NFP;
+    fp[FP_DF] = df;
+    fp[FP_DC] = dc;
+    fp[FP_LC] = lc;
+    fp[3] = 0.f;
   }
 }
 

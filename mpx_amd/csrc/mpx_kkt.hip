@@ -273,10 +273,32 @@ mpxkkt_status run_scan(hipStream_t st, const mpxkkt_problem* pr, int nt, const P
   return launch_check("k_fwd_down");
 }
 
-// costates, control steps and slope partials, parallel over stages
 template <typename T>
-mpxkkt_status run_fwd_stage(hipStream_t st, const mpxkkt_problem* pr, const Prob<T>& P, const Iter<T>& I,
-                            const Ws<T>& W) {
+bool use_wave_lqr(const mpxkkt_problem* pr, const Prm<T>& R);
+
+// costates, control steps and slope partials: on the wave path the forward
+// pass of the LQR solve with them (k_fwd_wave), else parallel over stages
+// after the dx chain (k_fwd_stage)
+template <typename T>
+mpxkkt_status run_fwd_stage(hipStream_t st, const mpxkkt_problem* pr, const Prob<T>& P, const Prm<T>& R,
+                            const Iter<T>& I, const Ws<T>& W, const LqrIn<T>& in) {
+  if constexpr (IsF32<T>::value) {
+    if (use_wave_lqr<T>(pr, R)) {
+      const int NT = (pr->n + 1 + 15) / 16;
+      const dim3 gc(pr->horizon + 1, pr->batch);
+      if (NT == 1) {
+        hipLaunchKernelGGL(k_fwd_wave<1>, dim3(pr->batch), dim3(64), 0, st, P, W, in);
+        hipLaunchKernelGGL(k_costate_wave<1>, gc, dim3(64), 0, st, P, I, W);
+      } else if (NT == 2) {
+        hipLaunchKernelGGL(k_fwd_wave<2>, dim3(pr->batch), dim3(64), 0, st, P, W, in);
+        hipLaunchKernelGGL(k_costate_wave<2>, gc, dim3(64), 0, st, P, I, W);
+      } else {
+        hipLaunchKernelGGL(k_fwd_wave<3>, dim3(pr->batch), dim3(64), 0, st, P, W, in);
+        hipLaunchKernelGGL(k_costate_wave<3>, gc, dim3(64), 0, st, P, I, W);
+      }
+      return launch_check("k_fwd_wave / k_costate_wave");
+    }
+  }
   const int nt = 64;
   const size_t lds = stage_smem_elems<T>(pr->n, nt) * sizeof(T);
   mpxkkt_status s = prep_lds(k_fwd_stage<T>, lds);
@@ -425,7 +447,7 @@ mpxkkt_status sqp_iterate_t(hipStream_t st, const mpxkkt_problem* pr, const mpxk
   prof_mark(st, pe, 1);
   if ((s = run_lqr<T>(st, pr, prm, P, R, I, W, in)) != MPXKKT_OK) return s;
   prof_mark(st, pe, 2);
-  if ((s = run_fwd_stage<T>(st, pr, P, I, W)) != MPXKKT_OK) return s;
+  if ((s = run_fwd_stage<T>(st, pr, P, R, I, W, in)) != MPXKKT_OK) return s;
   prof_mark(st, pe, 3);
   s = run_linesearch<T>(st, pr, P, R, I, W, in, so);
   prof_mark(st, pe, 4);
