@@ -33,15 +33,17 @@
 // F = A - B K is never formed; the costates lam = S~ x~ follow in parallel over
 // the stages (k_costate_wave).
 //
-// Memory goes through raw buffer descriptors.  A tile row (fixed v) of a
-// row-major matrix is one descriptor whose base is the row 16 it + v and whose
-// record count ends at the matrix's last row; each lane adds a loop-invariant
-// offset (4 g ld + col) * 4 that is set past every record count for a column
-// outside the matrix.  Out-of-range loads return 0 and out-of-range stores are
-// dropped, so the ragged tile edges cost no branches and no per-load address
-// arithmetic.  Each stage's data (including Q~) is requested before the
-// previous stage's results are stored: vmcnt counts loads and stores in issue
-// order, so waiting for the data never waits for the stores.
+// Memory goes through raw buffer descriptors, one per matrix and stage, whose
+// record count ends at the matrix's last row.  The lane's byte offset of
+// element (row, col) is (row ld + col) * 4: a row past the matrix lands past
+// the record count, and a column past it gets a lane part set past every
+// record count.  Out-of-range loads return 0 and out-of-range stores are
+// dropped, so the ragged tile edges cost no branches.  The lane parts are
+// recomputed every stage from an opaque copy of the lane id: hoisted out of
+// the stage loop, the hundreds of loop-invariant offsets would occupy VGPRs
+// for its whole length.  Each stage's data (including Q~) is requested before
+// the previous stage's results are stored: vmcnt counts loads and stores in
+// issue order, so waiting for the data never waits for the stores.
 #pragma once
 
 #include "common.h"
@@ -58,10 +60,10 @@ constexpr int kOOB = 0x40000000;
 __device__ __forceinline__ rsrc_t make_rsrc(const float* p, int nelem) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, nelem * 4, 0x00020000);
 }
-// rows [r0, nrows) of a row-major matrix with leading dimension ld
-__device__ __forceinline__ rsrc_t rows_rsrc(const float* M, int ld, int r0, int nrows) {
-  const int rem = nrows - r0;
-  return make_rsrc(M + (long long)r0 * ld, rem > 0 ? rem * ld : 0);
+// An opaque copy: offsets computed from it inside a loop stay inside the loop.
+__device__ __forceinline__ int fresh(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
 }
 // (the b32 buffer builtins move raw 32-bit words: bit-cast, never convert)
 __device__ __forceinline__ float rload(rsrc_t r, int voff) {
@@ -142,34 +144,34 @@ __device__ __forceinline__ LaneOff<NT> lane_offsets(int n, int m, int g, int c) 
   return L;
 }
 
-// C-layout tile (it, jt) of a row-major matrix with ld n and nrows rows.
+// C-layout tile (it, jt) of a row-major matrix with ld n (descriptor rM).
 template <int NT>
-__device__ __forceinline__ f4 tile_nn(const float* M, int n, int nrows, int it, int jt, const LaneOff<NT>& L) {
+__device__ __forceinline__ f4 tile_nn(rsrc_t rM, int n, int it, int jt, const LaneOff<NT>& L) {
   f4 x;
 #pragma unroll
-  for (int v = 0; v < 4; ++v) x[v] = rload(rows_rsrc(M, n, 16 * it + v, nrows), L.nn[jt]);
+  for (int v = 0; v < 4; ++v) x[v] = rload(rM, L.nn[jt] + (16 * it + v) * n * 4);
   return x;
 }
 
-// vector entries v[16 it + 4 g + k] (0 past len) in tile-row layout
+// vector entries v[16 it + 4 g + k] (0 past the descriptor) in tile-row layout
 template <int NT>
-__device__ __forceinline__ f4 rows_vec(const float* vec, int len, int it, const LaneOff<NT>& L) {
+__device__ __forceinline__ f4 rows_vec(rsrc_t r, int it, const LaneOff<NT>& L) {
   f4 x;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) x[k] = rload(rows_rsrc(vec, 1, 16 * it + k, len), L.g16);
+  for (int k = 0; k < 4; ++k) x[k] = rload(r, L.g16 + (16 * it + k) * 4);
   return x;
 }
 
 // Store tile (it, jt) of an (n+1)-wide homogeneous matrix: the first n columns
-// to M (ld n, nrows rows), column n to vec.
+// to rM (ld n), column n to rvec; rows past the descriptors are dropped.
 template <int NT>
-__device__ __forceinline__ void store_hom(const f4& x, float* M, float* vec, int n, int nrows, int it, int jt,
+__device__ __forceinline__ void store_hom(const f4& x, rsrc_t rM, rsrc_t rvec, int n, int it, int jt,
                                           const LaneOff<NT>& L) {
 #pragma unroll
-  for (int v = 0; v < 4; ++v) rstore(rows_rsrc(M, n, 16 * it + v, nrows), L.nn[jt], x[v]);
+  for (int v = 0; v < 4; ++v) rstore(rM, L.nn[jt] + (16 * it + v) * n * 4, x[v]);
   if (jt == (n >> 4)) {   // the tile holding column n (uniform)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) rstore(rows_rsrc(vec, 1, 16 * it + v, nrows), L.hom, x[v]);
+    for (int v = 0; v < 4; ++v) rstore(rvec, L.hom + (16 * it + v) * 4, x[v]);
   }
 }
 
@@ -190,19 +192,20 @@ template <int NT>
 __device__ __forceinline__ void load_qbar(f4 (&S)[NT][NT], const float* Q, const float* q, int n, int g, int c,
                                           bool upper_only, const LaneOff<NT>& L) {
   const int tn = n >> 4;
+  const rsrc_t rQ = make_rsrc(Q, n * n), rq = make_rsrc(q, n);
 #pragma unroll
   for (int i = 0; i < NT; ++i)
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
       if (upper_only && j < i) continue;
-      f4 x = tile_nn<NT>(Q, n, n, i, j, L);
+      f4 x = tile_nn<NT>(rQ, n, i, j, L);
       if (j == tn) {   // column n: q[16 i + 4 g + v]
-        const f4 qr = rows_vec<NT>(q, n, i, L);
+        const f4 qr = rows_vec<NT>(rq, i, L);
 #pragma unroll
         for (int v = 0; v < 4; ++v) x[v] += (c == L.cn) ? qr[v] : 0.f;
       }
       if (i == tn) {   // row n: q[16 j + c]
-        const float qc = rload(make_rsrc(q + 16 * j, n - 16 * j > 0 ? n - 16 * j : 0), L.c4);
+        const float qc = rload(rq, L.c4 + 64 * j);
 #pragma unroll
         for (int v = 0; v < 4; ++v) x[v] += (4 * g + v == L.cn) ? qc : 0.f;
       }
@@ -215,16 +218,16 @@ __device__ __forceinline__ void load_wave_stage(WaveStage<NT>& d, const Prob<flo
                                                 const LqrIn<float>& in, int b, int t, float sr, int g, int c,
                                                 const LaneOff<NT>& L) {
   const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1, tn = n >> 4;
-  const float* A = in.A.at(b, t);
-  const float* Bm = in.Bm.at(b, t);
-  const float* cd = ws.cdef + ((size_t)b * N1 + t + 1) * n;
+  const rsrc_t rA = make_rsrc(in.A.at(b, t), n * n);
+  const rsrc_t rB = make_rsrc(in.Bm.at(b, t), n * m);
+  const rsrc_t rc = make_rsrc(ws.cdef + ((size_t)b * N1 + t + 1) * n, n);
 #pragma unroll
   for (int i = 0; i < NT; ++i) {
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
-      f4 x = tile_nn<NT>(A, n, n, i, j, L);
+      f4 x = tile_nn<NT>(rA, n, i, j, L);
       if (j == tn) {   // column n: c ; corner 1
-        const f4 cv = rows_vec<NT>(cd, n, i, L);
+        const f4 cv = rows_vec<NT>(rc, i, L);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           x[v] += (c == L.cn) ? cv[v] : 0.f;
@@ -238,40 +241,41 @@ __device__ __forceinline__ void load_wave_stage(WaveStage<NT>& d, const Prob<flo
 #pragma unroll
   for (int i = 0; i < NT; ++i)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) d.B[i][v] = rload(rows_rsrc(Bm, m, 16 * i + v, n), L.mm);
+    for (int v = 0; v < 4; ++v) d.B[i][v] = rload(rB, L.mm + (16 * i + v) * m * 4);
   if (p > 0) {
-    const float* D = P.D.at(b, t);
-    const float* E = P.E.at(b, t);
+    const rsrc_t rD = make_rsrc(P.D.at(b, t), p * n), rE = make_rsrc(P.E.at(b, t), p * m);
 #pragma unroll
-    for (int j = 0; j < NT; ++j) d.D[j] = sr * tile_nn<NT>(D, n, p, 0, j, L);
+    for (int j = 0; j < NT; ++j)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) d.E[v] = sr * rload(rows_rsrc(E, m, v, p), L.mm);
+      for (int v = 0; v < 4; ++v) d.D[j][v] = sr * rload(rD, L.nn[j] + v * n * 4);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) d.E[v] = sr * rload(rE, L.mm + v * m * 4);
   } else {
 #pragma unroll
     for (int j = 0; j < NT; ++j) d.D[j] = zero4();
     d.E = zero4();
   }
-  const float* R = P.R.at(b, t);
-  const float* rd = ws.rd + ((size_t)b * N + t) * m;
-  const float* r = ws.rh + ((size_t)b * N + t) * m;
+  const rsrc_t rR = make_rsrc(P.R.at(b, t), m * m);
+  const rsrc_t rd = make_rsrc(ws.rd + ((size_t)b * N + t) * m, m);
+  const rsrc_t rr = make_rsrc(ws.rh + ((size_t)b * N + t) * m, m);
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int row = 4 * g + v;
-    const float dv = rload(rows_rsrc(rd, 1, v, m), L.g16);
-    d.H[v] = rload(rows_rsrc(R, m, v, m), L.mm) + (row == c ? (row < m ? dv : 1.f) : 0.f);
-    d.r[v] = rload(rows_rsrc(r, 1, v, m), L.g16);
+    const float dv = rload(rd, L.g16 + v * 4);
+    d.H[v] = rload(rR, L.mm + v * m * 4) + (row == c ? (row < m ? dv : 1.f) : 0.f);
+    d.r[v] = rload(rr, L.g16 + v * 4);
   }
 }
 
 template <int NT>
 __device__ __forceinline__ void store_value_wave(const f4 (&S)[NT][NT], const Ws<float>& ws, int b, int t, int N1,
                                                  int n, const LaneOff<NT>& L) {
-  float* So = ws.S + ((size_t)b * N1 + t) * n * n;
-  float* so = ws.s + ((size_t)b * N1 + t) * n;
+  const rsrc_t rS = make_rsrc(ws.S + ((size_t)b * N1 + t) * n * n, n * n);
+  const rsrc_t rs = make_rsrc(ws.s + ((size_t)b * N1 + t) * n, n);
 #pragma unroll
   for (int it = 0; it < NT; ++it)
 #pragma unroll
-    for (int jt = 0; jt < NT; ++jt) store_hom<NT>(S[it][jt], So, so, n, n, it, jt, L);
+    for (int jt = 0; jt < NT; ++jt) store_hom<NT>(S[it][jt], rS, rs, n, it, jt, L);
 }
 
 // segments == 1, FP32, n + 1 <= 16 NT, m <= 16, p <= 16: backward Riccati of
@@ -294,6 +298,7 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
   WaveStage<NT> cur;
   load_wave_stage<NT>(cur, P, ws, in, b, N - 1, sr, g, c, L);
   for (int t = N - 1; t >= 0; --t) {
+    const LaneOff<NT> Ls = lane_offsets<NT>(n, m, fresh(g), fresh(c));
     // SB = S~ B~ ; H = R + diag(rd) + rho_c E'E + B~' SB
     f4 SB[NT];
 #pragma unroll
@@ -405,15 +410,17 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
 
     // the next stage's data is requested before this stage's stores
     WaveStage<NT> nxt;
-    load_wave_stage<NT>(nxt, P, ws, in, b, t > 0 ? t - 1 : 0, sr, g, c, L);
+    // (the stage index is pinned to a scalar register: selected into a vector
+    // register it makes every descriptor built from it look divergent)
+    load_wave_stage<NT>(nxt, P, ws, in, b, __builtin_amdgcn_readfirstlane(t > 0 ? t - 1 : 0), sr, g, c, Ls);
 
     // stores: S_t, s_t ; K_t, k_t
-    store_value_wave<NT>(S, ws, b, t, N1, n, L);
+    store_value_wave<NT>(S, ws, b, t, N1, n, Ls);
     {
-      float* Ko = ws.K + ((size_t)b * N + t) * m * n;
-      float* ko = ws.k + ((size_t)b * N + t) * m;
+      const rsrc_t rK = make_rsrc(ws.K + ((size_t)b * N + t) * m * n, m * n);
+      const rsrc_t rk = make_rsrc(ws.k + ((size_t)b * N + t) * m, m);
 #pragma unroll
-      for (int j = 0; j < NT; ++j) store_hom<NT>(K[j], Ko, ko, n, m, 0, j, L);
+      for (int j = 0; j < NT; ++j) store_hom<NT>(K[j], rK, rk, n, 0, j, Ls);
     }
     cur = nxt;
   }
@@ -427,14 +434,14 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
 template <int NT, int RT>
 __device__ __forceinline__ void load_aop(f4 (&out)[RT][NT], const float* M, const float* colv, int n, int nrows,
                                          bool corner, int g, int c, const int (&off)[NT][4], const LaneOff<NT>& L) {
+  const rsrc_t rM = make_rsrc(M, nrows * n), rv = make_rsrc(colv, nrows);
 #pragma unroll
   for (int i = 0; i < RT; ++i) {
-    const rsrc_t rM = rows_rsrc(M, n, 16 * i, nrows);
 #pragma unroll
     for (int k = 0; k < NT; ++k)
 #pragma unroll
-      for (int s = 0; s < 4; ++s) out[i][k][s] = rload(rM, off[k][s]);
-    const float cv = rload(rows_rsrc(colv, 1, 16 * i, nrows), L.c4);   // colv[16 i + c]
+      for (int s = 0; s < 4; ++s) out[i][k][s] = rload(rM, off[k][s] + 16 * i * n * 4);
+    const float cv = rload(rv, L.c4 + 64 * i);   // colv[16 i + c]
     const float cc = (corner && 16 * i + c == n) ? 1.f : 0.f;
     // column n by a per-lane test (a test on the tile index alone lets the
     // compiler turn the update into a run-time index into the tile array)
@@ -454,13 +461,11 @@ __device__ __forceinline__ void load_chain_stage(f4 (&A)[NT][NT], f4 (&K)[1][NT]
   load_aop<NT, NT>(A, in.A.at(b, t), ws.cdef + ((size_t)b * N1 + t + 1) * n, n, n, true, g, c, off, L);
   load_aop<NT, 1>(K, ws.K + ((size_t)b * N + t) * m * n, ws.k + ((size_t)b * N + t) * m, n, m, false, g, c, off,
                   L);
-  const float* Bm = in.Bm.at(b, t);
+  const rsrc_t rB = make_rsrc(in.Bm.at(b, t), n * m);
 #pragma unroll
-  for (int i = 0; i < NT; ++i) {
-    const rsrc_t rbt = rows_rsrc(Bm, m, 16 * i, n);
+  for (int i = 0; i < NT; ++i)
 #pragma unroll
-    for (int s = 0; s < 4; ++s) BT[i][s] = rload(rbt, L.bt[s]);   // B[16 i + c][4 g + s]
-  }
+    for (int s = 0; s < 4; ++s) BT[i][s] = rload(rB, L.bt[s] + 16 * i * m * 4);   // B[16 i + c][4 g + s]
 }
 
 // lane's byte offsets of columns 16 k + 4 g + s within a row of ld n (kOOB
@@ -491,13 +496,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 3)))
   aop_offsets<NT>(off, n, g, c);
   f4 x[NT];
   {
-    const float* x0 = ws.cdef + (size_t)b * N1 * n;
+    const rsrc_t r0 = make_rsrc(ws.cdef + (size_t)b * N1 * n, n);
 #pragma unroll
     for (int k = 0; k < NT; ++k)
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        x[k][v] = rload(rows_rsrc(x0, 1, 16 * k + v, n), L.col0) +
-                  ((c == 0 && 16 * k + 4 * g + v == n) ? 1.f : 0.f);
+        x[k][v] = rload(r0, L.col0 + (16 * k + v) * 4) + ((c == 0 && 16 * k + 4 * g + v == n) ? 1.f : 0.f);
   }
   f4 cA[NT][NT], cK[1][NT], cB[NT];
   load_chain_stage<NT>(cA, cK, cB, ws, in, b, 0, N, n, m, g, c, off, L);
@@ -521,7 +525,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 3)))
       y[i] = acc;
     }
     f4 nA[NT][NT], nK[1][NT], nB[NT];
-    load_chain_stage<NT>(nA, nK, nB, ws, in, b, t + 1 < N ? t + 1 : t, N, n, m, g, c, off, L);
+    {
+      const int gf = fresh(g), cf = fresh(c);
+      const LaneOff<NT> Ls = lane_offsets<NT>(n, m, gf, cf);
+      int offs[NT][4];
+      aop_offsets<NT>(offs, n, gf, cf);
+      load_chain_stage<NT>(nA, nK, nB, ws, in, b, __builtin_amdgcn_readfirstlane(t + 1 < N ? t + 1 : t), N, n, m, gf,
+                           cf, offs, Ls);
+    }
     {
       const rsrc_t rdx = make_rsrc(ws.dx + ((size_t)b * N1 + t) * n, n);
 #pragma unroll
@@ -567,15 +578,18 @@ __global__ void __launch_bounds__(64) k_costate_wave(Prob<float> P, Iter<float> 
   load_aop<NT, NT>(Sa, ws.S + ((size_t)b * N1 + t) * n * n, ws.s + ((size_t)b * N1 + t) * n, n, n, false, g, c,
                    off, L);
   f4 x[NT], qv[NT], vv[NT], cv[NT];
-  const float* dxp = ws.dx + ((size_t)b * N1 + t) * n;
+  const rsrc_t rx = make_rsrc(ws.dx + ((size_t)b * N1 + t) * n, n);
+  const rsrc_t rq = make_rsrc(ws.qh + ((size_t)b * N1 + t) * n, n);
+  const rsrc_t rV = make_rsrc(it.V + ((size_t)b * N1 + t) * n, n);
+  const rsrc_t rc = make_rsrc(ws.cdef + ((size_t)b * N1 + t) * n, n);
 #pragma unroll
   for (int k = 0; k < NT; ++k) {
 #pragma unroll
     for (int v = 0; v < 4; ++v)
-      x[k][v] = rload(rows_rsrc(dxp, 1, 16 * k + v, n), L.col0) + ((c == 0 && 16 * k + 4 * g + v == n) ? 1.f : 0.f);
-    qv[k] = rows_vec<NT>(ws.qh + ((size_t)b * N1 + t) * n, n, k, L);
-    vv[k] = rows_vec<NT>(it.V + ((size_t)b * N1 + t) * n, n, k, L);
-    cv[k] = rows_vec<NT>(ws.cdef + ((size_t)b * N1 + t) * n, n, k, L);
+      x[k][v] = rload(rx, L.col0 + (16 * k + v) * 4) + ((c == 0 && 16 * k + 4 * g + v == n) ? 1.f : 0.f);
+    qv[k] = rows_vec<NT>(rq, k, L);
+    vv[k] = rows_vec<NT>(rV, k, L);
+    cv[k] = rows_vec<NT>(rc, k, L);
   }
   float df = 0.f, dc = 0.f, lc = 0.f;
   const bool col0 = c == 0;
