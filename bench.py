@@ -42,6 +42,21 @@ def make_batch(B, seed, dtype):
     return insts, starts, batch, it
 
 
+def traffic_per_launch(kernel, batch):
+    """HBM bytes per launch of the kernel from the committed counter summary
+    (profiles/traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the
+    same counters' reading for a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    if not rec or rec.get("batch") != batch:
+        return None
+    return rec["bytes_per_launch"]
+
+
 def cpu_baseline(insts, starts, budget_s):
     """FP64 CPU restatement timed on a bounded sample (instance-iterations/s)."""
     try:
@@ -73,6 +88,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--tune", default="", help="segments,block_threads,trial_group: skip the autotuner")
+    ap.add_argument("--serve", type=float, default=3.0,
+                    help="seconds of the mixed serving stream after the batched run (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="serving arrivals per second")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="MiB: only copy a buffer of this size twice (reference for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -80,6 +100,15 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.calibrate_copy:
+        nbytes = args.calibrate_copy << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_read": nbytes, "calibrate_copy_bytes_written": nbytes}))
+        return
     from mpx_amd import _lib as L
     from mpx_amd import autotune, flops
     from mpx_amd.problem import DeviceBatch, SQPSolver
@@ -127,15 +156,22 @@ def main():
     status_bad = int((solver.stats_t["status"] != 0).sum().item())
     per_phase = [x / max(calls, 1) for x in phase_ms]
     log(f"timed {args.steps} steps: {ms_step:.3f} ms/step (wall {wall:.3f} s); phases ms "
-        f"lin={per_phase[0]:.3f} lqr={per_phase[1]:.3f} ls={per_phase[2]:.3f}; trials={trials:.2f}")
+        f"lin={per_phase[0]:.3f} lqr={per_phase[1]:.3f} costate={per_phase[2]:.3f} ls={per_phase[3]:.3f}; "
+        f"trials={trials:.2f}")
 
-    # roofline of the dominant kernel (the LQR phase)
-    lqr_f = flops.lqr_flops(n, m, N, p, tune["segments"]) * B
+    # roofline of the dominant kernel: the LQR recursion (phase 1 events)
+    wave = tune["segments"] == 1 and n + 1 <= 48 and m <= 16 and p <= 16
+    if wave:
+        kname, lqr_f = "k_lqr_wave", flops.riccati_bwd_flops(n, m, N, p) * B
+    elif tune["segments"] == 1:
+        kname, lqr_f = "k_lqr_seq", flops.lqr_flops(n, m, N, p, 1) * B
+    else:
+        kname, lqr_f = "scan kernels", flops.lqr_flops(n, m, N, p, tune["segments"]) * B
     lqr_s = per_phase[1] / 1e3
     achieved = lqr_f / lqr_s
-    roofline = {"bound": "mfma", "kernel": "k_lqr_seq" if tune["segments"] == 1 else "scan kernels",
+    roofline = {"bound": "mfma", "kernel": kname,
                 "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP32, "traffic": None,
+                "frac": achieved / PEAK_FP32, "traffic": traffic_per_launch(kname, B),
                 "flops_per_launch": lqr_f, "ms_per_launch": per_phase[1]}
 
     # plain path: one instance per launch sequence, sample of instances
@@ -169,6 +205,15 @@ def main():
                "inst_iters_per_s": c["inst_iters_per_s"], "cores": c["cores"], "kind": c["kind"],
                "sample": c["sample"]}
 
+    serving = None
+    if args.serve > 0:
+        from mpx_amd import serve_bench
+        sv = serve_bench.run(args.serve_rate, args.serve, seed=args.seed, max_wall=args.serve + 20.0)
+        serving = {k: sv[k] for k in ("rate", "duration", "served", "solves_per_s", "p50_ms", "p99_ms",
+                                      "backlog", "by_shape")}
+        log(f"serving at {args.serve_rate:.0f}/s: p50 {sv['p50_ms']:.2f} ms, p99 {sv['p99_ms']:.2f} ms, "
+            f"{sv['solves_per_s']:.0f} solves/s")
+
     out = {
         "metric": "SQP iters/sec at B=4096 n=36 N=50 vs plain; p50/p99 MPC tick latency",
         "value": value,
@@ -185,10 +230,12 @@ def main():
         "instance_iters_per_s": value * B,
         "plain_iters_per_s": plain_iters_per_s,
         "batched_vs_plain": value / plain_iters_per_s,
-        "phase_ms": {"linearize": per_phase[0], "lqr": per_phase[1], "linesearch": per_phase[2]},
+        "phase_ms": {"linearize": per_phase[0], "lqr": per_phase[1], "costates": per_phase[2],
+                     "linesearch": per_phase[3]},
         "mean_trials": trials, "nonzero_status": status_bad,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out))
 

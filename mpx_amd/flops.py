@@ -23,6 +23,17 @@ def lqr_flops(n, m, N, p, segments=1):
     return total
 
 
+def riccati_bwd_flops(n, m, N, p):
+    """The backward Riccati recursion alone (the k_lqr_wave kernel), per instance:
+    per stage S A and A'(S A) (4n^3), S B, B'(S A) and G'K (6n^2 m), B'(S B) and
+    H^{-1} G (4n m^2), the m x m factorisation (m^3/3), the equality-row terms
+    D'D, E'D, E'E (2p(n+m)^2) and the affine vector terms (S c + s, A'(.),
+    B'(.), G'k, H^{-1} g: 4n^2 + 4nm + 2m^2 + 2p(n+m))."""
+    per = (4 * n**3 + 6 * n**2 * m + 4 * n * m**2 + m**3 / 3.0 + 2 * p * (n + m) ** 2
+           + 4 * n**2 + 4 * n * m + 2 * m**2 + 2 * p * (n + m))
+    return N * per
+
+
 def linearize_flops(n, m, N, p):
     return N * (9 * n**2 + 2 * n * m + 2 * m**2 + 4 * p * (n + m)) + 2 * n**2
 

@@ -23,8 +23,11 @@ def _plant_step(inst, x, u):
     return x + inst["h"] * (inst["A"] @ x + inst["B"] @ u + inst["kappa"] * np.tanh(inst["W"] @ x))
 
 
-def run(rate, duration, seed=0, life=200, budget_gb=64.0, max_group=256, n_streams=4, warm=True, log=None):
-    """Serve ``rate`` ticks/s for ``duration`` s; returns a summary dict."""
+def run(rate, duration, seed=0, life=200, budget_gb=64.0, max_group=256, n_streams=4, warm=True, log=None,
+        max_wall=None):
+    """Serve ``rate`` ticks/s for ``duration`` s; returns a summary dict.  With
+    ``max_wall`` the run stops after that many seconds even if arrivals are
+    still queued; they are reported as ``backlog`` (an overloaded server)."""
     rng = np.random.default_rng(seed)
     server = MPCServer(budget_gb * 2**30, max_group=max_group, n_streams=n_streams)
     shapes, probs = syn.SERVING_SHAPES, syn.SERVING_PROBS
@@ -45,6 +48,8 @@ def run(rate, duration, seed=0, life=200, budget_gb=64.0, max_group=256, n_strea
     serial = 0
     while i < n_arr or server.pending:
         now = time.perf_counter() - t0
+        if max_wall is not None and now > max_wall:
+            break
         while i < n_arr and t_rel[i] <= now:
             if idle:
                 cid = idle.pop()
@@ -79,7 +84,8 @@ def run(rate, duration, seed=0, life=200, budget_gb=64.0, max_group=256, n_strea
                 idle.append(r.cid)
     wall = time.perf_counter() - t0
     lat = np.array(latencies) * 1e3
-    out = dict(rate=rate, duration=duration, served=len(lat), solves_per_s=len(lat) / wall,
+    backlog = (n_arr - i) + len(server.pending)
+    out = dict(rate=rate, duration=duration, served=len(lat), solves_per_s=len(lat) / wall, backlog=backlog,
                p50_ms=float(np.percentile(lat, 50)) if len(lat) else None,
                p99_ms=float(np.percentile(lat, 99)) if len(lat) else None,
                rejected_for_budget=waits, controllers_opened=serial,
