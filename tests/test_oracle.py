@@ -223,3 +223,48 @@ def test_model_evaluator_and_shift():
     Z = np.arange(12.0).reshape(4, 3)
     S = orc.shift_warmstart(Z, 1)
     assert np.array_equal(S[:3], Z[1:]) and np.array_equal(S[3], Z[3])
+
+
+def test_kkt_residual_matches_lagrangian_fd():
+    """D8 pinned independently: the stationarity part of kkt_residual equals the
+    central finite-difference gradient of the Lagrangian
+    L = sum_t [l_t + V_{t+1}'(f(x_t,u_t) - x_{t+1}) + mu_t'h_t + nu_hi'(u - hi) + nu_lo'(lo - u)]
+        + l_N + V_0'(x0 - X_0),
+    and the residual norm is |(grad L, c, h, nu s - mu_b)|."""
+    inst = syn.make_instance(4, 2, n=5, m=3, N=4, p=2)
+    N, n, m, p = inst["N"], inst["n"], inst["m"], inst["p"]
+    rng = np.random.default_rng(11)
+    X = 0.3 * rng.standard_normal((N + 1, n))
+    U = np.clip(0.5 * rng.standard_normal((N, m)), -0.9, 0.9)
+    V = 0.2 * rng.standard_normal((N + 1, n))
+    Yeq = 0.1 * rng.standard_normal((N, p))
+    Ybox = np.abs(0.2 * rng.standard_normal((N, 2 * m))) + 0.01
+    lo, hi = inst["ulo"], inst["uhi"]
+
+    def lagrangian(Xv, Uv):
+        L = 0.0
+        for t in range(N):
+            L += orc.stage_cost(inst, Xv[t], Uv[t], t)
+            L += V[t + 1] @ (orc.dynamics(inst, Xv[t], Uv[t], t) - Xv[t + 1])
+            L += Yeq[t] @ orc.eq_rows(inst, Xv[t], Uv[t], t)
+            L += Ybox[t][:m] @ (Uv[t] - hi) + Ybox[t][m:] @ (lo - Uv[t])
+        L += orc.stage_cost(inst, Xv[N], np.zeros(m), N)
+        L += V[0] @ (inst["x0"] - Xv[0])
+        return L
+
+    eps = 1e-6
+    gX = np.zeros_like(X)
+    gU = np.zeros_like(U)
+    for idx in np.ndindex(X.shape):
+        E = np.zeros_like(X); E[idx] = eps
+        gX[idx] = (lagrangian(X + E, U) - lagrangian(X - E, U)) / (2 * eps)
+    for idx in np.ndindex(U.shape):
+        E = np.zeros_like(U); E[idx] = eps
+        gU[idx] = (lagrangian(X, U + E) - lagrangian(X, U - E)) / (2 * eps)
+    c = orc.constraints(inst, X, U)
+    h = np.array([orc.eq_rows(inst, X[t], U[t], t) for t in range(N)])
+    s = np.hstack([hi - U, U - lo])
+    comp = Ybox * s - orc.DEFAULT_PARAMS["mu_b"]
+    r = np.concatenate([gX.ravel(), gU.ravel(), c.ravel(), h.ravel(), comp.ravel()])
+    kkt = orc.kkt_residual(inst, X, U, V, Yeq, Ybox)
+    assert abs(kkt - np.sqrt(r @ r)) < 1e-6 * max(1.0, kkt)
