@@ -101,7 +101,9 @@ typedef struct {
   int32_t trial_group;    /* trials evaluated together (0 = all); 1 if limited_memory */
   int32_t flags;          /* bit 0: FP32 products on VALU instead of the matrix cores;
                              bit 1: no register-resident wave LQR kernel (FP32,
-                             n + 1 <= 48, m <= 16, n_eq <= 16, one segment) */
+                             n + 1 <= 48, m <= 16, n_eq <= 16, one segment);
+                             bit 2: single-kernel line search (one block per
+                             instance) instead of the stage-parallel one */
 } mpxkkt_params;
 
 /* Per-instance outputs of one SQP iteration (each may be NULL). */
@@ -111,7 +113,7 @@ typedef struct {
   void* merit0;    /* phi(0)                   [batch] dtype */
   void* dmerit;    /* phi'(0)                  [batch] dtype */
   int32_t* status; /* 0 ok, 1 non-finite, 2 line search exhausted [batch] */
-  int32_t* trials; /* trials evaluated         [batch] */
+  int32_t* trials; /* trials examined: backtracking in groups of trial_group [batch] */
 } mpxkkt_stats;
 
 /* Defaults of the oracle's decisions D3-D6. */

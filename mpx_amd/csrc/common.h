@@ -65,6 +65,9 @@ struct Ws {
   T* sval;   // [B][P][n^2+n]   values (S, s) at segment starts
   T* xs;     // [B][P][n]       forward states at segment starts
   T* kpart;  // [B][N1]         squared KKT residual per stage
+  T* lsp;    // [B][8]          line-search scalars (rho, phi(0), phi'(0), step caps, slack)
+  T* tpart;  // [B][N1][2*MAX_TRIALS] per-stage parts of phi at each trial step
+  T* hq;     // [B][N][2p]      equality rows at the iterate and their slopes along the step
 };
 
 template <typename T>
@@ -88,6 +91,10 @@ __device__ __forceinline__ float dev_sqrt<float>(float x) { return sqrtf(x); }
 __device__ __forceinline__ void lds_sync() {
   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
+
+// LDS written and read by lanes of one wave: wait for the wave's LDS traffic
+// and keep the compiler from moving memory operations across.
+__device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
 // Sum over the 64 lanes of a wave (every lane gets the result).
 template <typename T>

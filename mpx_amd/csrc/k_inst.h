@@ -24,10 +24,6 @@ namespace mpxk {
 constexpr int kInstWaves = 4;
 constexpr int kInstScratch = 4 * 64;   // per wave: x - xr, u - ur, mu + rho_c h, sech^2(W x)
 
-// LDS written and read by lanes of one wave: wait for the wave's LDS traffic
-// and keep the compiler from moving memory operations across.
-__device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
-
 // Offsets (elements) of the instance LDS plan.
 struct InstLds {
   int A, W, B, Q, R;      // -1 when not staged

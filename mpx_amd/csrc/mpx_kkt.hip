@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -17,6 +18,7 @@
 #include "k_inst.h"
 #include "k_scan.h"
 #include "k_linesearch.h"
+#include "k_trials.h"
 #include "k_misc.h"
 
 using namespace mpxk;
@@ -101,6 +103,9 @@ size_t carve(const mpxkkt_problem* pr, int P, Ws<T>* ws, char* base) {
   ws->sval = take(P > 1 ? B * P * (n * n + n) : 0);
   ws->xs = take(P > 1 ? B * P * n : 0);
   ws->kpart = take(B * N1);
+  ws->lsp = take(B * NLS);
+  ws->tpart = take(B * N1 * 2 * MAX_TRIALS);
+  ws->hq = take(B * N * 2 * (size_t)pr->n_eq);
   return off + 256;
 }
 
@@ -379,9 +384,77 @@ mpxkkt_status run_lqr(hipStream_t st, const mpxkkt_problem* pr, const mpxkkt_par
   return run_lqr_m<T, 64>(st, pr, nt, P, R, I, W, in);
 }
 
+// Rows of a view of row length L start 16-byte aligned (null views pass).
+bool rows16(const mpxkkt_view& v, long long L) {
+  return v.ptr == nullptr || ((reinterpret_cast<uintptr_t>(v.ptr) & 15) == 0 && v.batch_stride % 4 == 0 &&
+                              v.stage_stride % 4 == 0 && L % 4 == 0);
+}
+
+template <typename T>
+bool trial_rows16(const mpxkkt_problem* pr) {
+  if constexpr (!IsF32<T>::value) {
+    return false;
+  } else {
+    const long long n = pr->n, m = pr->m;
+    return rows16(pr->A, n) && rows16(pr->W, n) && rows16(pr->B, m) && rows16(pr->Q, n) && rows16(pr->R, m) &&
+           rows16(pr->QN, n) && (pr->n_eq == 0 || (rows16(pr->D, n) && rows16(pr->E, m)));
+  }
+}
+
+template <typename T, int RPL>
+mpxkkt_status launch_trials(hipStream_t st, bool v4, dim3 grid, size_t lds, const Prob<T>& P, const Prm<T>& R,
+                            const Iter<T>& I, const Ws<T>& W, int j0, int j1) {
+  mpxkkt_status s;
+  if constexpr (IsF32<T>::value) {
+    if (v4) {
+      if ((s = prep_lds(k_trials<T, RPL, true>, lds)) != MPXKKT_OK) return s;
+      hipLaunchKernelGGL((k_trials<T, RPL, true>), grid, dim3(64 * kTrialWaves), lds, st, P, R, I, W, j0, j1);
+      return launch_check("k_trials");
+    }
+  }
+  if ((s = prep_lds(k_trials<T, RPL, false>, lds)) != MPXKKT_OK) return s;
+  hipLaunchKernelGGL((k_trials<T, RPL, false>), grid, dim3(64 * kTrialWaves), lds, st, P, R, I, W, j0, j1);
+  return launch_check("k_trials");
+}
+
+template <typename T>
+mpxkkt_status run_trials(hipStream_t st, const mpxkkt_problem* pr, bool v4, dim3 grid, size_t lds,
+                         const Prob<T>& P, const Prm<T>& R, const Iter<T>& I, const Ws<T>& W, int j0, int j1) {
+  const int rows = std::max(pr->n, pr->n_eq);
+  if (rows <= 64) return launch_trials<T, 1>(st, v4, grid, lds, P, R, I, W, j0, j1);
+  if (rows <= 128) return launch_trials<T, 2>(st, v4, grid, lds, P, R, I, W, j0, j1);
+  if (rows <= 256) return launch_trials<T, 4>(st, v4, grid, lds, P, R, I, W, j0, j1);
+  return launch_trials<T, 8>(st, v4, grid, lds, P, R, I, W, j0, j1);
+}
+
+// K4: k_ls_setup, k_trials over [0, group) and then [group, max_trials) for the
+// instances that accepted none of the first group, k_ls_accept.  flags bit 2
+// selects the single-kernel line search of k_inst.h / k_linesearch.h instead.
 template <typename T>
 mpxkkt_status run_linesearch(hipStream_t st, const mpxkkt_problem* pr, const Prob<T>& P, const Prm<T>& R,
                              const Iter<T>& I, const Ws<T>& W, const LqrIn<T>& in, const mpxkkt_stats* so) {
+  if ((R.flags & 4) == 0) {
+    mpxkkt_status s;
+    const int B = pr->batch, N1 = pr->horizon + 1;
+    const size_t setup_lds = (size_t)6 * 256 * sizeof(T);
+    hipLaunchKernelGGL(k_ls_setup<T>, dim3(B), dim3(256), setup_lds, st, P, R, I, W);
+    if ((s = launch_check("k_ls_setup")) != MPXKKT_OK) return s;
+    const dim3 grid((N1 + kTrialWaves - 1) / kTrialWaves, B);
+    const size_t lds = (size_t)kTrialWaves * trial_wave_elems(pr->n, pr->m) * sizeof(T);
+    const bool v4 = trial_rows16<T>(pr);
+    if ((s = run_trials<T>(st, pr, v4, grid, lds, P, R, I, W, 0, R.group)) != MPXKKT_OK) return s;
+    if (R.group < R.max_trials &&
+        (s = run_trials<T>(st, pr, v4, grid, lds, P, R, I, W, R.group, R.max_trials)) != MPXKKT_OK)
+      return s;
+    T* a = so ? static_cast<T*>(so->alpha) : nullptr;
+    T* r = so ? static_cast<T*>(so->rho) : nullptr;
+    T* f0 = so ? static_cast<T*>(so->merit0) : nullptr;
+    T* df = so ? static_cast<T*>(so->dmerit) : nullptr;
+    int* stt = so ? so->status : nullptr;
+    int* tr = so ? so->trials : nullptr;
+    hipLaunchKernelGGL(k_ls_accept<T>, grid, dim3(64 * kTrialWaves), 0, st, P, R, I, W, a, r, f0, df, stt, tr);
+    return launch_check("k_ls_accept");
+  }
   if (use_inst<T>(pr, true)) {
     const InstLds L = inst_lds(pr->n, pr->m, pr->horizon, pr->model == MPXKKT_MODEL_TANH, pr->Q.stage_stride == 0,
                                pr->R.stage_stride == 0, true);

@@ -151,3 +151,40 @@ def test_fp32_register_wave_kernel(wl, shape, iters):
             for key, tol in (("X", TOL32_X), ("U", 10 * TOL32_X), ("V", 10 * TOL32_X)):
                 assert _rel(g[key][i], r[key]) < tol, (k, i, key, _rel(g[key][i], r[key]))
                 assert _rel(g[key][i], gb[key][i]) < tol, (k, i, key, _rel(g[key][i], gb[key][i]))
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("group", [1, 3, 0])
+@pytest.mark.parametrize("wl,shape,iters,start", [
+    (2, dict(n=8, m=4, N=12), 4, "zero"),            # box, cold start: several trials
+    (3, dict(n=36, m=12, N=10, p=8), 2, None),       # rows 16-byte aligned: float4 products
+    (4, dict(n=60, m=9, N=6, p=66), 3, None),        # two equality-row slots per lane
+    (4, dict(n=76, m=32, N=5, p=12), 3, "zero"),     # two state rows per lane (FP32: FP64 LQR LDS too large)
+])
+def test_stage_parallel_line_search(wl, shape, iters, start, group, dtype):
+    """The stage-parallel line search (k_ls_setup / k_trials / k_ls_accept,
+    the default) against the single-kernel line search (flags bit 2) on the
+    same inputs, and both against the oracle: the same accepted step and trial
+    count, iterates equal to rounding."""
+    if dtype == "f64" and shape["n"] > 64:
+        pytest.skip("the FP64 sequential LQR kernel needs more LDS than a block has at n > 64")
+    dt = torch.float64 if dtype == "f64" else torch.float32
+    kw = dict(shape=shape, start=start, trial_group=group)
+    insts, starts, refs, out, _, _ = _run(wl, 3, iters, dt, **kw)
+    _, _, _, old, _, _ = _run(wl, 3, iters, dt, flags=4, **kw)
+    tol = 1e-10 if dtype == "f64" else TOL32_X
+    for k in range(iters):
+        (g, st), (go, sto) = out[k], old[k]
+        for i in range(3):
+            r = refs[i][k]
+            assert st["status"][i] == sto["status"][i], (k, i)
+            assert st["trials"][i] == sto["trials"][i], (k, i, st["trials"][i], sto["trials"][i])
+            assert st["alpha"][i] == pytest.approx(sto["alpha"][i], rel=1e-6), (k, i)
+            if dtype == "f64":
+                assert abs(st["alpha"][i] - r["alpha"]) < 1e-12, (k, i)
+            for key in ("X", "U", "V", "Yeq", "Ybox"):
+                if g[key].size == 0:
+                    continue
+                assert _rel(g[key][i], go[key][i]) < 10 * tol, (k, i, key, _rel(g[key][i], go[key][i]))
+                if dtype == "f64" or key == "X":
+                    assert _rel(g[key][i], r[key]) < 10 * tol, (k, i, key, _rel(g[key][i], r[key]))
