@@ -365,111 +365,54 @@ __device__ __forceinline__ double rlane(double v, int src) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// In-place inverse of the m x m SPD matrix H (LDS, row-major), m <= MMAX <= 64,
-// by Gauss-Jordan elimination without pivoting (stable for SPD) in wave 0:
-// lane i keeps row i of [H | I] in registers and the pivot row of each step is
-// broadcast through LDS (`piv`, 2*MMAX elements).  Other waves return at once;
-// the caller syncs the block.
-template <typename T, int MMAX>
-__device__ void spd_inverse_wave(T* H, int m, T* piv) {
+// Wave 0 alone, no block barrier inside (other waves return at once): the
+// upper Cholesky factor U of the SPD m x m matrix H (H = U'U, m <= MM <= 64)
+// into the upper triangle of H, then Y = U^{-1} (upper triangular, zero below)
+// into Y.  Lane k keeps column k in registers: elimination step j takes row j
+// of U from the lanes by readlane and updates the trailing columns in place,
+// so the factorisation touches LDS only to publish U; the back substitution
+// solves U y = e_k with the U entries read back as LDS broadcasts.
+template <typename T, int MM>
+__device__ void chol_upper_inv_wave(T* H, T* Y, int m) {
   if (threadIdx.x >= 64) return;
-  const int lane = threadIdx.x;
-  T a[MMAX], bi[MMAX];
+  const int k = threadIdx.x;
+  const bool own = k < m;
+  T h[MM];
 #pragma unroll
-  for (int k = 0; k < MMAX; ++k) {
-    a[k] = (lane < m && k < m) ? H[lane * m + k] : T(lane == k ? 1 : 0);
-    bi[k] = T(lane == k ? 1 : 0);
-  }
+  for (int i = 0; i < MM; ++i) h[i] = (own && i < m) ? H[i * m + k] : T(0);
+  wave_lds_sync();
 #pragma unroll
-  for (int k = 0; k < MMAX; ++k) {
-    if (k < m) {
-      if (lane == k) {
-        const T sc = T(1) / a[k];
-#pragma unroll
-        for (int j = 0; j < MMAX; ++j) { a[j] *= sc; bi[j] *= sc; piv[j] = a[j]; piv[MMAX + j] = bi[j]; }
-      }
-      wave_fence();
-      const T f = lane == k ? T(0) : a[k];
-#pragma unroll
-      for (int j = 0; j < MMAX; ++j) {
-        if (j < m) { a[j] -= f * piv[j]; bi[j] -= f * piv[MMAX + j]; }
-      }
-      wave_fence();
-    }
-  }
-  if (lane < m) {
-#pragma unroll
-    for (int k = 0; k < MMAX; ++k)
-      if (k < m) H[lane * m + k] = bi[k];
-  }
-}
-
-// Lower Cholesky of the m x m SPD matrix H (LDS, row-major), m <= MMAX <= 64,
-// by wave 0 with the rows in registers (lane i holds row i); the factor is
-// written back to H.  Other waves return at once; the caller syncs the block.
-template <typename T, int MMAX>
-__device__ void chol_wave(T* H, int m) {
-  if (threadIdx.x >= 64) return;
-  const int lane = threadIdx.x;
-  T h[MMAX];
-#pragma unroll
-  for (int k = 0; k < MMAX; ++k) h[k] = (lane < m && k < m) ? H[lane * m + k] : T(lane == k ? 1 : 0);
-#pragma unroll
-  for (int j = 0; j < MMAX; ++j) {
+  for (int j = 0; j < MM; ++j) {
     if (j < m) {
-      const T d = dev_sqrt(rlane(h[j], j));
-      h[j] = lane == j ? d : (lane > j ? h[j] / d : h[j]);
+      const T hj = h[j];                                          // H[j][k]
+      const T d = dev_sqrt(rlane(hj, j));
+      const T lk = (own && k > j) ? hj / d : T(0);                // U[j][k], k > j
+      if (own && k >= j) H[j * m + k] = k == j ? d : lk;
 #pragma unroll
-      for (int k = j + 1; k < MMAX; ++k) {
-        if (k < m) {
-          const T lkj = rlane(h[j], k);
-          if (lane >= k) h[k] -= h[j] * lkj;
-        }
-      }
+      for (int i = j + 1; i < MM; ++i)
+        if (i < m) h[i] -= rlane(lk, i) * lk;
     }
   }
-  if (lane < m) {
+  wave_lds_sync();
+  T y[MM];
 #pragma unroll
-    for (int k = 0; k < MMAX; ++k)
-      if (k < m) H[lane * m + k] = k <= lane ? h[k] : T(0);
-  }
-}
-
-// L Y = G (FWD) or L^T X = Y for the columns of G (m x ncol, ld) in place, by
-// wave 0; each lane owns columns lane, lane+64, ...; L entries are uniform LDS reads.
-template <typename T, int MMAX, bool FWD>
-__device__ void trsm_wave(const T* L, int m, T* G, int ncol, int ld) {
-  if (threadIdx.x >= 64) return;
-  for (int c = threadIdx.x; c < ncol; c += 64) {
-    T y[MMAX];
+  for (int i = MM - 1; i >= 0; --i) {
+    if (i < m) {
+      T v = k == i ? T(1) : T(0);
 #pragma unroll
-    for (int i = 0; i < MMAX; ++i) y[i] = i < m ? G[i * ld + c] : T(0);
-    if (FWD) {
-#pragma unroll
-      for (int i = 0; i < MMAX; ++i) {
-        if (i < m) {
-          T v = y[i];
-#pragma unroll
-          for (int k = 0; k < i; ++k) v -= L[i * m + k] * y[k];
-          y[i] = v / L[i * m + i];
-        }
-      }
+      for (int q = i + 1; q < MM; ++q)
+        if (q < m) v -= H[i * m + q] * y[q];
+      y[i] = v / H[i * m + i];
     } else {
-#pragma unroll
-      for (int i = MMAX - 1; i >= 0; --i) {
-        if (i < m) {
-          T v = y[i];
-#pragma unroll
-          for (int k = i + 1; k < MMAX; ++k)
-            if (k < m) v -= L[k * m + i] * y[k];
-          y[i] = v / L[i * m + i];
-        }
-      }
+      y[i] = T(0);
     }
-#pragma unroll
-    for (int i = 0; i < MMAX; ++i)
-      if (i < m) G[i * ld + c] = y[i];
   }
+  if (own) {
+#pragma unroll
+    for (int i = 0; i < MM; ++i)
+      if (i < m) Y[i * m + k] = i <= k ? y[i] : T(0);
+  }
+  wave_lds_sync();
 }
 
 }  // namespace mpxk

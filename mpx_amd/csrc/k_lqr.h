@@ -117,13 +117,20 @@ struct IsF32 { static constexpr bool value = false; };
 template <>
 struct IsF32<float> { static constexpr bool value = true; };
 
-// H <- H^{-1} for the SPD m x m matrix in LDS: Cholesky, then the identity
-// solved through both triangles (block-wide).  (A register Gauss-Jordan in one
-// wave, spd_inverse_wave, measured slower here: its unrolled rows spill.)
+// H <- H^{-1} for the SPD m x m matrix in LDS.  m > 16: wave 0 factors
+// H = U'U and forms Y = U^{-1} in tmp with the columns in registers
+// (chol_upper_inv_wave), then the block forms H^{-1} = Y Y' on VALU (m^3
+// flops; one more matrix-core product site here pushes the FP32 kernels into
+// register spills).  m <= 16: Cholesky and the identity solved through both
+// triangles, block-wide in LDS (the register version spills at that template
+// size inside the Riccati kernels).
 template <typename T, int MMAX>
-__device__ void spd_inverse(T* H, T* tmp, T* piv, int m) {
-  {
-    // Cholesky, then the identity solved through both triangles
+__device__ void spd_inverse(T* H, T* tmp, int m) {
+  if constexpr (MMAX > 16) {
+    chol_upper_inv_wave<T, MMAX>(H, tmp, m);
+    lds_sync();
+    gemm_lds<T, false, true, 0>(m, m, m, tmp, m, tmp, m, H, m);
+  } else {
     chol_lds(H, m);
     for (int e = threadIdx.x; e < m * m; e += blockDim.x) tmp[e] = (e / m == e % m) ? T(1) : T(0);
     lds_sync();
@@ -201,7 +208,7 @@ __device__ void riccati_step(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& w
   }
   lds_sync();
   // C: H <- H^{-1}
-  spd_inverse<T, MMAX>(L.H, L.Kb, L.piv, m);
+  spd_inverse<T, MMAX>(L.H, L.Kb, m);
   lds_sync();
   // D: [K | k] = H^{-1} [G | g]
   if constexpr (IsF32<T>::value) {
