@@ -23,6 +23,10 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The serving leg runs one HIP stream per (shape bucket, staging area); with
+# enough hardware queues for all of them a small shape's launch never queues
+# behind a large shape's kernels (read when HIP initialises, so set first).
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "12")
 
 PEAK_FP32 = 157.3e12     # MI355X dense FP32 (matrix) FLOP/s
 PEAK_HBM = 8.0e12        # B/s

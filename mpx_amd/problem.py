@@ -121,13 +121,26 @@ class DeviceBatch:
         instance-major layout makes any prefix contiguous)."""
         if not 1 <= k <= self.B:
             raise ValueError(f"prefix size {k} outside [1, {self.B}]")
+        return self.slice(0, k)
+
+    def slice(self, a, b):
+        """Instances a..b-1 as a batch of their own: every view's pointer moves by
+        a batch strides (same device arrays; ``t`` still names the whole batch)."""
+        if not 0 <= a < b <= self.B:
+            raise ValueError(f"slice [{a}, {b}) outside [0, {self.B})")
         other = object.__new__(DeviceBatch)
         other.__dict__.update(self.__dict__)
         pr = L.Problem()
         C.pointer(pr)[0] = self.problem
-        pr.batch = k
+        es = 8 if self.dtype == torch.float64 else 4
+        for name, typ in L.Problem._fields_:
+            if typ is L.View:
+                v = getattr(pr, name)
+                if v.ptr:
+                    v.ptr = v.ptr + a * v.batch_stride * es
+        pr.batch = b - a
         other.problem = pr
-        other.B = k
+        other.B = b - a
         return other
 
     def iterate_from(self, X, U, V, Yeq=None, Ybox=None):
@@ -152,6 +165,17 @@ class DeviceBatch:
 
 def _stream():
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def shift_warmstart(batch, it, shift, u_reset=None, reset=None):
+    """Warm-start shift with the per-instance NaN fallback on ``batch`` (any
+    slice) and its iterate ``it``; returns the int32 reset flags."""
+    if reset is None:
+        reset = torch.zeros(batch.B, dtype=torch.int32, device=batch.device)
+    ci = batch.c_iterate(it)
+    L.check(L.load().mpxkkt_shift_warmstart(_stream(), C.byref(batch.problem), int(shift), C.byref(ci),
+                                            _ptr(u_reset), _ptr(reset)))
+    return reset
 
 
 class SQPSolver:
@@ -219,13 +243,7 @@ class SQPSolver:
         return g, c
 
     def shift(self, it, shift, u_reset=None, reset=None):
-        b = self.batch
-        if reset is None:
-            reset = torch.zeros(b.B, dtype=torch.int32, device=b.device)
-        ci = b.c_iterate(it)
-        L.check(self.lib.mpxkkt_shift_warmstart(_stream(), C.byref(b.problem), int(shift), C.byref(ci),
-                                                _ptr(u_reset), _ptr(reset)))
-        return reset
+        return shift_warmstart(self.batch, it, shift, u_reset, reset)
 
     def capture(self, it, iters=1, shift=None):
         """Record ``iters`` SQP iterations on the iterate ``it`` (and, if

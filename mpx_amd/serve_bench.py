@@ -1,11 +1,13 @@
 """Serving-stream workload: Poisson arrivals of MPC ticks over a mix of shapes.
 
-Each arrival is one tick of an idle controller (a new controller opens when
-none is idle and the HBM budget allows; a controller closes after ``life``
-ticks).  Shapes are drawn from (n, m, N) = (12, 12, 25), (36, 12, 50),
-(76, 32, 100) with probabilities 0.5, 0.35, 0.15 and every tick needs 1-8 SQP
-iterations.  Latency is measured on the host from the scheduled arrival time
-to the moment the tick's first control is back on the host.  The plant of
+Each arrival draws its shape from (n, m, N) = (12, 12, 25), (36, 12, 50),
+(76, 32, 100) with probabilities 0.5, 0.35, 0.15 and is one tick of an idle
+controller of that shape (a new controller opens when none is idle and the
+HBM budget allows; a controller closes after ``life`` ticks); every tick needs
+1-8 SQP iterations.  Latency is measured on the host from the scheduled arrival time
+to the moment the tick's first control is back on the host; the loop admits
+arrivals, hands finished ticks to their plants and launches pending ticks
+without waiting for the device.  The plant of
 every controller is its own model, advanced on the host with the returned
 control.
 """
@@ -23,13 +25,15 @@ def _plant_step(inst, x, u):
     return x + inst["h"] * (inst["A"] @ x + inst["B"] @ u + inst["kappa"] * np.tanh(inst["W"] @ x))
 
 
-def run(rate, duration, seed=0, life=200, budget_gb=64.0, max_group=256, n_streams=4, warm=True, log=None,
-        max_wall=None):
+def run(rate, duration, seed=0, life=200, budget_gb=64.0, max_group=256, n_streams=3, warm=True, log=None,
+        max_wall=None, tune="auto"):
     """Serve ``rate`` ticks/s for ``duration`` s; returns a summary dict.  With
     ``max_wall`` the run stops after that many seconds even if arrivals are
-    still queued; they are reported as ``backlog`` (an overloaded server)."""
+    still queued; they are reported as ``backlog`` (an overloaded server).
+    ``tune``: launch parameters of every bucket ("auto": the autotuner's, chosen
+    when the bucket opens, before the clock starts)."""
     rng = np.random.default_rng(seed)
-    server = MPCServer(budget_gb * 2**30, max_group=max_group, n_streams=n_streams)
+    server = MPCServer(budget_gb * 2**30, max_group=max_group, n_streams=n_streams, tune=tune)
     shapes, probs = syn.SERVING_SHAPES, syn.SERVING_PROBS
     if warm:
         # build every bucket (and its solvers) before the clock starts
@@ -41,20 +45,20 @@ def run(rate, duration, seed=0, life=200, budget_gb=64.0, max_group=256, n_strea
             server.close(cid)
     n_arr = rng.poisson(rate * duration)
     t_rel = np.sort(rng.uniform(0.0, duration, size=n_arr))
-    idle, state = [], {}
+    idle, state = {k: [] for k in range(len(shapes))}, {}
     latencies, by_shape, waits = [], {}, 0
     t0 = time.perf_counter()
     i = 0
     serial = 0
-    while i < n_arr or server.pending:
+    while i < n_arr or server.busy:
         now = time.perf_counter() - t0
         if max_wall is not None and now > max_wall:
             break
         while i < n_arr and t_rel[i] <= now:
-            if idle:
-                cid = idle.pop()
+            k = int(rng.choice(len(shapes), p=probs))
+            if idle[k]:
+                cid = idle[k].pop()
             else:
-                k = rng.choice(len(shapes), p=probs)
                 n, m, N = shapes[k]
                 serial += 1
                 inst = syn.make_instance(5, serial, seed, n=n, m=m, N=N)
@@ -63,14 +67,15 @@ def run(rate, duration, seed=0, life=200, budget_gb=64.0, max_group=256, n_strea
                     waits += 1
                     i += 1
                     continue
-                state[cid] = dict(inst=inst, x=inst["x0"].copy(), ticks=0, shape=(n, m, N))
+                state[cid] = dict(inst=inst, x=inst["x0"].copy(), ticks=0, shape=(n, m, N), k=k)
             server.request(cid, state[cid]["x"], int(rng.integers(1, 9)), t_arrive=t0 + t_rel[i])
             i += 1
-        if not server.pending:
-            if i < n_arr:
+        done = server.step()
+        if not done:
+            if not server.in_flight and i < n_arr:
                 time.sleep(max(0.0, min(1e-4, t_rel[i] - (time.perf_counter() - t0))))
             continue
-        for r in server.run_round():
+        for r in done:
             lat = r.t_done - r.t_arrive
             latencies.append(lat)
             s = state[r.cid]
@@ -81,10 +86,10 @@ def run(rate, duration, seed=0, life=200, budget_gb=64.0, max_group=256, n_strea
                 server.close(r.cid)
                 del state[r.cid]
             else:
-                idle.append(r.cid)
+                idle[s["k"]].append(r.cid)
     wall = time.perf_counter() - t0
     lat = np.array(latencies) * 1e3
-    backlog = (n_arr - i) + len(server.pending)
+    backlog = (n_arr - i) + len(server.pending) + sum(len(c[1]) for ln in server.in_flight for c in ln.cohorts)
     out = dict(rate=rate, duration=duration, served=len(lat), solves_per_s=len(lat) / wall, backlog=backlog,
                p50_ms=float(np.percentile(lat, 50)) if len(lat) else None,
                p99_ms=float(np.percentile(lat, 99)) if len(lat) else None,

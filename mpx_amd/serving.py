@@ -10,26 +10,33 @@ mpx/utils/mpc_wrapper.py:16-28, mpx/utils/mpc_wrapper_srbd.py:56-58,
 waits until others close.
 
 Requests (a controller's new measured state x0 and the number of SQP
-iterations for this tick) queue as they arrive.  Every round gathers the
-pending requests of each bucket, grouped by iteration count, into a staging
-prefix of the bucket, runs the batched SQP iterations of each group on its own
-HIP stream (groups overlap on the device), then scatters the iterate back into
-the slots with the warm-start shift and the per-controller NaN fallback of
-update_and_extract (mpx/utils/mpc_wrapper.py:79-96).  A controller's result
-does not depend on which other requests share its round: every launch shape is
-fixed per bucket, and the kernels never mix instances.
+iterations for this tick) queue as they arrive.  ``step()`` never blocks: it
+hands back the ticks whose work has finished on the device, and every bucket
+with a free staging area takes the requests pending for it (at most
+``max_group``, oldest first) as one launch on that area's own HIP stream, so
+launches of different buckets, and successive launches of one bucket, overlap
+on the device while new requests keep arriving.  A launch gathers its
+controllers' slots into the staging prefix sorted by iteration count
+(descending), runs SQP iteration j on the prefix of requests that need more
+than j iterations, and as soon as a cohort has had all its iterations it
+copies their first controls to the host, applies the warm-start shift with the
+per-controller NaN fallback of update_and_extract
+(mpx/utils/mpc_wrapper.py:79-96) on that cohort's slice, scatters it back into
+the slots and records an event.  A controller's result does not depend on
+which other requests share its launch: every launch shape is fixed per
+bucket, and the kernels never mix instances.
 """
 from __future__ import annotations
 
 import collections
 import time
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 import numpy as np
 import torch
 
 from . import _lib as L
-from .problem import DeviceBatch, SQPSolver
+from .problem import DeviceBatch, SQPSolver, shift_warmstart
 
 _PROBLEM_KEYS = ("A", "B", "W", "Q", "R", "QN", "xref", "uref", "D", "E", "e", "ulo", "uhi", "x0")
 _ITER_KEYS = ("X", "U", "V", "Yeq", "Ybox")
@@ -58,28 +65,49 @@ class _Controller:
     slot: int
     nbytes: int
     ticks: int = 0
+    busy: bool = False
+
+
+@dataclass
+class _Launch:
+    bucket: "_Bucket"
+    si: int
+    # cohorts in completion order: (event, requests, pinned u0, pinned reset flags)
+    cohorts: list = field(default_factory=list)
 
 
 class _Bucket:
-    """Resident slot arrays for one shape, plus per-stream staging batches."""
+    """Resident slot arrays for one shape, plus staging batches, each with its
+    own stream and workspace."""
 
-    def __init__(self, key, proto, capacity, max_group, dtype, device, n_streams, tune):
+    def __init__(self, key, proto, capacity, max_group, dtype, device, n_streams, params):
         self.key = key
         self.capacity = capacity
         self.free = list(range(capacity - 1, -1, -1))
         self.slots = DeviceBatch.from_instances([proto] * capacity, dtype=dtype, device=device)
         self.state = self.slots.iterate_from(*[None] * 5)
         self.max_group = max_group
-        self.params = L.default_params(**tune)
-        # staging batch, iterate and workspace per stream: groups of one round run
-        # concurrently, so nothing they write is shared
-        self.stage, self.stage_it, self.ws = [], [], []
-        for _ in range(n_streams):
+        self.params = params
+        self.stage, self.stage_it, self.ws, self.streams, self.io = [], [], [], [], []
+        self.idle = []
+        n, m = proto["n"], proto["m"]
+        for si in range(n_streams):
             st = DeviceBatch.from_instances([proto] * max_group, dtype=dtype, device=device)
             self.stage.append(st)
             self.stage_it.append(st.iterate_from(*[None] * 5))
             self.ws.append(torch.empty(SQPSolver.workspace_bytes(st, self.params), dtype=torch.uint8,
                                        device=device))
+            self.streams.append(torch.cuda.Stream(device=device))
+            # host <-> device transfer buffers of the area, allocated once: a
+            # pinned allocation inside the serving loop can wait for the device
+            self.io.append(dict(
+                slots_h=torch.empty(max_group, dtype=torch.long, pin_memory=True),
+                x0_h=torch.empty((max_group, n), dtype=dtype, pin_memory=True),
+                u0_h=torch.empty((max_group, m), dtype=dtype, pin_memory=True),
+                reset_h=torch.empty(max_group, dtype=torch.int32, pin_memory=True),
+                slots=torch.empty(max_group, dtype=torch.long, device=device),
+                reset=torch.empty(max_group, dtype=torch.int32, device=device)))
+            self.idle.append(si)
         self.solvers = {}
 
     def solver(self, k, si):
@@ -88,34 +116,58 @@ class _Bucket:
         return self.solvers[(k, si)]
 
     def load(self, slot, inst):
-        """Write an instance's problem data and a cold start into its slot."""
+        """Write an instance's problem data and a cold start into its slot: one
+        host-to-device copy of the packed arrays, then device-side copies into
+        the slot (on the current stream; launches wait for it)."""
         dt, dev = self.slots.dtype, self.slots.device
-        one = DeviceBatch.from_instances([inst], dtype=dt, device=dev)
+        dst, parts = [], []
         for k in _PROBLEM_KEYS:
-            if self.slots.t.get(k) is not None:
-                self.slots.t[k][slot].copy_(one.t[k][0])
+            d = self.slots.t.get(k)
+            if d is not None:
+                dst.append(d[slot])
+                parts.append(np.broadcast_to(np.asarray(inst[k]), d.shape[1:]).ravel())
+        flat = torch.from_numpy(np.concatenate(parts)).to(device=dev, dtype=dt)
+        off = 0
+        for d in dst:
+            d.view(-1).copy_(flat[off:off + d.numel()])
+            off += d.numel()
         N, n = inst["N"], inst["n"]
-        self.state["X"][slot] = torch.as_tensor(inst["x0"], dtype=dt, device=dev).expand(N + 1, n)
+        self.state["X"][slot].copy_(self.slots.t["x0"][slot].expand(N + 1, n))
         for k in ("U", "V", "Yeq", "Ybox"):
             self.state[k][slot].zero_()
+
+
+def _bucket_params(proto, dtype, device, tune):
+    """Launch parameters of a bucket: ``tune`` as given (a dict), or, with
+    ``tune='auto'``, the autotuner's choice for a group of 16 of its shape."""
+    if tune != "auto":
+        return L.default_params(**(tune or {}))
+    from . import autotune
+    insts = [proto] * 16
+    b = DeviceBatch.from_instances(insts, dtype=dtype, device=device)
+    it0 = b.iterate_from(*[None] * 5)
+    cands = [dict(segments=1, block_threads=t, trial_group=g) for t in (128, 256, 512) for g in (1, 0)]
+    best = autotune.tune(b, it0, iters=2, cands=cands)
+    return L.default_params(**{k: v for k, v in best.items() if k != "ms"})
 
 
 class MPCServer:
     """Admits controllers by HBM budget and serves their ticks with continuous batching."""
 
-    def __init__(self, hbm_budget_bytes, max_group=256, slots_per_bucket=512, n_streams=4,
+    def __init__(self, hbm_budget_bytes, max_group=256, slots_per_bucket=512, n_streams=2,
                  dtype=torch.float32, device="cuda", shift=1, tune=None):
         self.budget = int(hbm_budget_bytes)
         self.used = 0
         self.max_group = max_group
         self.slots_per_bucket = slots_per_bucket
+        self.n_streams = n_streams
         self.dtype, self.device = dtype, torch.device(device)
-        self.streams = [torch.cuda.Stream(device=self.device) for _ in range(n_streams)]
         self.shift = shift
-        self.tune = tune or {}
+        self.tune = tune
         self.buckets = {}
         self.controllers = {}
         self.pending = collections.deque()
+        self.in_flight = []
         self._cid = 0
         self._rid = 0
 
@@ -138,8 +190,9 @@ class MPCServer:
         key = shape_key(inst)
         b = self.buckets.get(key)
         if b is None:
+            params = _bucket_params(inst, self.dtype, self.device, self.tune)
             b = _Bucket(key, inst, self.slots_per_bucket, self.max_group, self.dtype, self.device,
-                        len(self.streams), self.tune)
+                        self.n_streams, params)
             self.buckets[key] = b
         if not b.free:
             return None
@@ -163,58 +216,97 @@ class MPCServer:
         self.pending.append(r)
         return r
 
-    def run_round(self):
-        """Serve every pending request once; returns the completed requests."""
-        if not self.pending:
-            return []
-        reqs = list(self.pending)
-        self.pending.clear()
-        groups = collections.defaultdict(list)
-        seen = set()
-        for r in reqs:
-            if r.cid in seen:            # one tick per controller per round
-                self.pending.append(r)
-                continue
-            seen.add(r.cid)
-            c = self.controllers[r.cid]
-            groups[(c.bucket.key, r.iters)].append(r)
-        main = torch.cuda.current_stream(self.device)
-        for s in self.streams:
-            s.wait_stream(main)          # slot loads of newly opened controllers
-        launched = []
-        si = 0
-        for (key, iters), rs in groups.items():
-            b = self.buckets[key]
-            for g0 in range(0, len(rs), b.max_group):
-                chunk = rs[g0:g0 + b.max_group]
-                stream = self.streams[si % len(self.streams)]
-                launched.append(self._launch(b, chunk, iters, stream, si % len(self.streams)))
-                si += 1
-        done = []
-        for ev, chunk, u0, reset in launched:
-            ev.synchronize()
-            t = time.perf_counter()
-            u0h = u0.numpy()
-            rh = reset.numpy()
-            for j, r in enumerate(chunk):
-                r.u0, r.reset, r.t_done = u0h[j].copy(), bool(rh[j]), t
-                self.controllers[r.cid].ticks += 1
-                done.append(r)
+    @property
+    def busy(self):
+        return bool(self.pending) or bool(self.in_flight)
+
+    def step(self):
+        """Non-blocking: returns the ticks completed since the last call and
+        launches pending requests onto idle staging areas."""
+        done = self._harvest(block=False)
+        self._dispatch()
         return done
 
-    def _launch(self, b, chunk, iters, stream, si):
-        """Gather -> iterate -> shift -> scatter for one group on ``stream``.
+    def run_round(self):
+        """Serve every request pending now (and nothing that arrives meanwhile);
+        blocks until they are done and returns them."""
+        want = {r.rid for r in self.pending}
+        done = []
+        while want:
+            self._dispatch()
+            got = self._harvest(block=True)
+            done.extend(got)
+            want -= {r.rid for r in got}
+        return done
 
-        Every device tensor of the launch is allocated while ``stream`` is
-        current, so the caching allocator orders its reuse after this stream's
+    # --- device work ---------------------------------------------------------
+    def _harvest(self, block):
+        done = []
+        keep = []
+        for ln in self.in_flight:
+            while ln.cohorts:
+                ev, reqs, u0_h, reset_h = ln.cohorts[0]
+                if block:
+                    ev.synchronize()
+                elif not ev.query():
+                    break
+                t = time.perf_counter()
+                u0 = u0_h.numpy()
+                rs = reset_h.numpy()
+                for j, r in enumerate(reqs):
+                    r.u0, r.reset, r.t_done = u0[j].copy(), bool(rs[j]), t
+                    c = self.controllers[r.cid]
+                    c.ticks += 1
+                    c.busy = False
+                    done.append(r)
+                ln.cohorts.pop(0)
+            if ln.cohorts:
+                keep.append(ln)
+            else:
+                ln.bucket.idle.append(ln.si)
+        self.in_flight = keep
+        return done
+
+    def _dispatch(self):
+        if not self.pending:
+            return
+        by_bucket = collections.OrderedDict()
+        rest = collections.deque()
+        for r in self.pending:
+            c = self.controllers[r.cid]
+            b = c.bucket
+            if c.busy or not b.idle or len(by_bucket.get(b.key, ())) >= b.max_group:
+                rest.append(r)           # one tick per controller in flight; wait for a free area
+                continue
+            c.busy = True
+            by_bucket.setdefault(b.key, []).append(r)
+        self.pending = rest
+        for key, reqs in by_bucket.items():
+            b = self.buckets[key]
+            self.in_flight.append(self._launch(b, reqs, b.idle.pop(0)))
+
+    def _launch(self, b, reqs, si):
+        """Gather -> iterate (by prefix) -> per cohort: first controls, shift,
+        scatter, event; all on the staging area's stream.
+
+        Every device tensor of the launch is allocated while that stream is
+        current, so the caching allocator orders its reuse after the stream's
         work; host inputs and outputs go through pinned buffers with
         asynchronous copies (the pinned allocator holds a buffer until its copy
         has completed)."""
-        k = len(chunk)
-        slots_h = torch.tensor([self.controllers[r.cid].slot for r in chunk], dtype=torch.long).pin_memory()
-        x0_h = torch.from_numpy(np.stack([r.x0 for r in chunk])).to(self.dtype).pin_memory()
+        reqs = sorted(reqs, key=lambda r: -r.iters)
+        k = len(reqs)
+        iters = [r.iters for r in reqs]
+        stream = b.streams[si]
+        stream.wait_stream(torch.cuda.current_stream(self.device))   # slot loads of new controllers
+        io = b.io[si]          # free: the area's previous launch has completed
+        io["slots_h"][:k].copy_(torch.tensor([self.controllers[r.cid].slot for r in reqs], dtype=torch.long))
+        io["x0_h"][:k].copy_(torch.from_numpy(np.stack([r.x0 for r in reqs])))
+        ln = _Launch(b, si)
         with torch.cuda.stream(stream):
-            slots = slots_h.to(self.device, non_blocking=True)
+            slots = io["slots"][:k]
+            slots.copy_(io["slots_h"][:k], non_blocking=True)
+            x0_h = io["x0_h"][:k]
             st, st_it = b.stage[si], b.stage_it[si]
             for key in _PROBLEM_KEYS:
                 src = b.slots.t.get(key)
@@ -223,17 +315,27 @@ class MPCServer:
             st.t["x0"][:k].copy_(x0_h, non_blocking=True)
             for key in _ITER_KEYS:
                 torch.index_select(b.state[key], 0, slots, out=st_it[key][:k])
-            it = {key: st_it[key][:k] for key in _ITER_KEYS}
-            solver = b.solver(k, si)
-            for _ in range(iters):
-                solver.iterate(it)
-            u0_h = torch.empty((k, st.m), dtype=self.dtype, pin_memory=True)
-            u0_h.copy_(it["U"][:, 0, :], non_blocking=True)
-            reset = solver.shift(it, self.shift)
-            reset_h = torch.empty(k, dtype=torch.int32, pin_memory=True)
-            reset_h.copy_(reset, non_blocking=True)
-            for key in _ITER_KEYS:
-                b.state[key].index_copy_(0, slots, it[key])
-            ev = torch.cuda.Event()
-            ev.record(stream)
-        return ev, chunk, u0_h, reset_h
+            for j in range(iters[0]):
+                kj = sum(1 for v in iters if v > j)
+                b.solver(kj, si).iterate({key: st_it[key][:kj] for key in _ITER_KEYS})
+                lo = sum(1 for v in iters if v > j + 1)
+                if lo < kj:
+                    ln.cohorts.append(self._finish(b, io, st, st_it, slots, reqs, lo, kj))
+        return ln
+
+    def _finish(self, b, io, st, st_it, slots, reqs, lo, hi):
+        """First controls, shift and scatter of requests lo..hi-1 of a launch
+        (on the current stream)."""
+        it = {key: st_it[key][lo:hi] for key in _ITER_KEYS}
+        u0_h = io["u0_h"][lo:hi]
+        u0_h.copy_(it["U"][:, 0, :], non_blocking=True)
+        reset = io["reset"][lo:hi]
+        reset.zero_()
+        shift_warmstart(st.slice(lo, hi), it, self.shift, reset=reset)
+        reset_h = io["reset_h"][lo:hi]
+        reset_h.copy_(reset, non_blocking=True)
+        for key in _ITER_KEYS:
+            b.state[key].index_copy_(0, slots[lo:hi], it[key])
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream(self.device))
+        return ev, reqs[lo:hi], u0_h, reset_h

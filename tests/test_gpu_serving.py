@@ -86,3 +86,27 @@ def test_nan_fallback_isolated():
         srv._last = reqs
     for i in (0, 2):
         assert np.array_equal(clean._last[i].u0, dirty._last[i].u0)
+
+
+def test_overlapped_launches_equal_alone():
+    """Requests launched by the non-blocking step() while earlier launches are
+    still on the device (a controller's next tick waits for its previous one)
+    get the same bits as the same ticks served one at a time."""
+    insts = _insts()
+    srv = _server()
+    cids = [srv.open(d) for d in insts]
+    reqs, done = {}, []
+    for k, iters in enumerate(ITERS):
+        for i, (c, d, it) in enumerate(zip(cids, insts, iters)):
+            reqs[(i, k)] = srv.request(c, _x(d, k), it)
+            done += srv.step()
+    while srv.busy:
+        done += srv.step()
+    assert len(done) == len(reqs)
+    alone = _server()
+    cids_a = [alone.open(d) for d in insts]
+    for k, iters in enumerate(ITERS):
+        for i, (c, d, it) in enumerate(zip(cids_a, insts, iters)):
+            r = alone.request(c, _x(d, k), it)
+            assert len(alone.run_round()) == 1
+            assert np.array_equal(r.u0, reqs[(i, k)].u0), (i, k)
