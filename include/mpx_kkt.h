@@ -103,7 +103,9 @@ typedef struct {
                              bit 1: no register-resident wave LQR kernel (FP32,
                              n + 1 <= 48, m <= 16, n_eq <= 16, one segment);
                              bit 2: single-kernel line search (one block per
-                             instance) instead of the stage-parallel one */
+                             instance) instead of the stage-parallel one;
+                             bit 3: linearisation parallel over stages (lower
+                             latency for small batches; n, m, n_eq <= 64) */
 } mpxkkt_params;
 
 /* Per-instance outputs of one SQP iteration (each may be NULL). */

@@ -272,6 +272,71 @@ __global__ void __launch_bounds__(256) k_linearize_inst(Prob<T> P, Prm<T> prm, I
   }
 }
 
+// K1 parallel over stages: the block's 8 waves take consecutive stages of one
+// instance and share its constant matrices, staged once in LDS as in
+// k_linearize_inst; each wave stages its x_t, u_t (projected into the box
+// interior and written back) and x_{t+1} and runs stage_terms on them.  The
+// per-stage arithmetic is that of k_linearize_inst, so both write the same
+// values; this one keeps more waves in flight per instance.
+constexpr int kLinStageWaves = 8;
+
+__host__ __device__ inline int lin_stage_wave_elems(int n, int m) {
+  return ((2 * n + m + 3) & ~3) + kInstScratch;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(512) k_linearize_stage(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws) {
+  const int b = blockIdx.y, tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & 63, wv = tid >> 6, wpb = nt >> 6;
+  const int n = P.n, m = P.m, N = P.N, N1 = N + 1;
+  const InstLds L = inst_lds(n, m, N, P.model == 1, P.Q.st == 0, P.R.st == 0, false);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sm = reinterpret_cast<T*>(smem_raw);
+  inst_stage_constants(P, b, sm, L);
+  if (blockIdx.x == 0 && P.model == 1)
+    for (int e = tid; e < n * m; e += nt) ws.Blin[(size_t)b * n * m + e] = P.h * P.Bm.at(b, 0)[e];
+  __syncthreads();
+  const int t = blockIdx.x * wpb + wv;
+  if (t > N) return;                               // whole wave; no barrier follows
+  T* x = sm + ((L.X + 3) & ~3) + (size_t)wv * lin_stage_wave_elems(n, m);
+  T* xn = x + n;
+  T* u = xn + n;
+  T* scr = x + ((2 * n + m + 3) & ~3);
+  const T* Xb = it.X + (size_t)b * N1 * n;
+  const T* Vb = it.V + (size_t)b * N1 * n;
+  // terminal wave: x = X_N, xn = X_0 (the initial-state row)
+  for (int i = lane; i < n; i += 64) {
+    x[i] = Xb[(size_t)t * n + i];
+    xn[i] = t < N ? Xb[(size_t)(t + 1) * n + i] : Xb[i];
+  }
+  if (t < N) {
+    T* Ut = it.U + ((size_t)b * N + t) * m;
+    for (int i = lane; i < m; i += 64) {
+      T v = Ut[i];
+      if (P.box) {   // box projection into the interior (in place)
+        const T lo = P.ulo.at(b, t)[i], hi = P.uhi.at(b, t)[i];
+        const T w = T(1e-3) * (hi - lo);
+        if (v >= hi) v = hi - w;
+        if (v <= lo) v = lo + w;
+        Ut[i] = v;
+      }
+      u[i] = v;
+    }
+  }
+  wave_lds_sync();
+  T o[3];
+  if (t < N)
+    stage_terms<T, true>(P, prm, it, ws, b, t, sm, L, x, u, xn, Vb + (size_t)(t + 1) * n, scr, o);
+  else
+    terminal_terms<T, true>(P, ws, b, x, xn, Vb, scr, o);
+  if (lane == 0) {
+    T* mp = ws.mpart + ((size_t)b * N1 + t) * NMP;
+    mp[MP_COST] = o[0];
+    mp[MP_VC] = o[1];
+    mp[MP_CC] = o[2];
+  }
+}
+
 // Sum of cnt values over the block's waves (every lane gets the result).
 template <typename T, int CNT>
 __device__ void inst_block_sum(T (&v)[CNT], T* red) {

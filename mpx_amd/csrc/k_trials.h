@@ -43,6 +43,14 @@ __host__ __device__ inline int trial_wave_elems(int n, int m) {
   return 3 * n4 + 3 * m4;
 }
 
+// With SH (tanh model, stage-invariant Q and R) the block's waves take
+// consecutive stages of one instance and share its A, W, B, Q, R, staged once
+// in LDS with rows padded to length + 1 (lane-per-row reads hit distinct
+// banks); elements of that area, rounded up to 4.
+__host__ __device__ inline int trial_shared_elems(int n, int m) {
+  return ((3 * n * (n + 1) + n * (m + 1) + m * (m + 1)) + 3) & ~3;
+}
+
 // y0 = row . v0, y1 = row . v1 (row in global memory, v0 and v1 in LDS).  V4:
 // the row is 16-byte aligned and L % 4 == 0; the sums run in the same order
 // either way.
@@ -156,17 +164,36 @@ __device__ int ls_first_accept(const Prm<T>& prm, const Ws<T>& ws, int b, int N1
 
 // Trials j0 <= j < j1 at stage t = blockIdx.x * waves + wave of instance
 // blockIdx.y.  RPL: state (and equality) rows per lane, ceil(max(n, p) / 64).
-template <typename T, int RPL, bool V4>
-__global__ void __launch_bounds__(256) k_trials(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, int j0, int j1) {
+template <typename T, int RPL, bool V4, bool SH>
+__global__ void __launch_bounds__(512) k_trials(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, int j0, int j1) {
   static_assert(!V4 || IsF32<T>::value, "float4 rows are FP32 only");
   const int wpb = blockDim.x >> 6, wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int b = blockIdx.y, t = blockIdx.x * wpb + wv;
   const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
-  if (t > N) return;                                          // whole wave: no block barrier follows
-  if (j0 > 0 && ls_first_accept(prm, ws, b, N1, j0) >= 0) return;   // an earlier pass accepted
+  // an earlier pass accepted: the same answer in every wave of the block (one instance)
+  if (j0 > 0 && ls_first_accept(prm, ws, b, N1, j0) >= 0) return;
   const int n4 = (n + 3) & ~3, m4 = (m + 3) & ~3;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* x0 = reinterpret_cast<T*>(smem_raw) + (size_t)wv * trial_wave_elems(n, m);
+  T* sh = reinterpret_cast<T*>(smem_raw);
+  const int lA = 0, lW = n * (n + 1), lB = 2 * n * (n + 1), lQ = lB + n * (m + 1), lR = lQ + n * (n + 1);
+  if constexpr (SH) {
+    const T* A = P.A.at(b, 0);
+    const T* W = P.W.at(b, 0);
+    const T* Q = P.Q.at(b, 0);
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+      const int i = e / n, j = e - i * n, o = i * (n + 1) + j;
+      sh[lA + o] = A[e];
+      sh[lW + o] = W[e];
+      sh[lQ + o] = Q[e];
+    }
+    const T* Bm = P.Bm.at(b, 0);
+    for (int e = threadIdx.x; e < n * m; e += blockDim.x) sh[lB + (e / m) * (m + 1) + e % m] = Bm[e];
+    const T* R = P.R.at(b, 0);
+    for (int e = threadIdx.x; e < m * m; e += blockDim.x) sh[lR + (e / m) * (m + 1) + e % m] = R[e];
+    __syncthreads();
+  }
+  if (t > N) return;                                          // whole wave: no block barrier follows
+  T* x0 = sh + (SH ? trial_shared_elems(n, m) : 0) + (size_t)wv * trial_wave_elems(n, m);
   T* x1 = x0 + n4;
   T* d0 = x1 + n4;
   T* u0 = d0 + n4;
@@ -198,6 +225,7 @@ __global__ void __launch_bounds__(256) k_trials(Prob<T> P, Prm<T> prm, Iter<T> i
   T F0[RPL], F1[RPL], w0[RPL], w1[RPL], q0[RPL], q1[RPL], dd0[RPL], dd1[RPL];
   T xn0[RPL], xn1[RPL], vn0[RPL], vn1[RPL];
   const bool tanh_model = P.model == 1;
+  (void)lA;
 #pragma unroll
   for (int r = 0; r < RPL; ++r) {
     const int i = lane + 64 * r;
@@ -208,7 +236,13 @@ __global__ void __launch_bounds__(256) k_trials(Prob<T> P, Prm<T> prm, Iter<T> i
       dd1[r] = x1[i];
       if (!term) {
         T za0, za1, zb0, zb1;
-        if (tanh_model) {
+        if (SH) {
+          row_dot2<T, false>(sh + lA + i * (n + 1), n, x0, x1, za0, za1);
+          row_dot2<T, false>(sh + lW + i * (n + 1), n, x0, x1, w0[r], w1[r]);
+          row_dot2<T, false>(sh + lB + i * (m + 1), m, u0, u1, zb0, zb1);
+          F0[r] = x0[i] + P.h * (za0 + zb0);
+          F1[r] = x1[i] + P.h * (za1 + zb1);
+        } else if (tanh_model) {
           row_dot2<T, V4>(P.A.at(b, 0) + (size_t)i * n, n, x0, x1, za0, za1);
           row_dot2<T, V4>(P.W.at(b, 0) + (size_t)i * n, n, x0, x1, w0[r], w1[r]);
           row_dot2<T, V4>(P.Bm.at(b, 0) + (size_t)i * m, m, u0, u1, zb0, zb1);
@@ -220,7 +254,8 @@ __global__ void __launch_bounds__(256) k_trials(Prob<T> P, Prm<T> prm, Iter<T> i
           F0[r] = za0 + zb0;
           F1[r] = za1 + zb1;
         }
-        row_dot2<T, V4>(P.Q.at(b, t) + (size_t)i * n, n, d0, x1, q0[r], q1[r]);
+        if (SH) row_dot2<T, false>(sh + lQ + i * (n + 1), n, d0, x1, q0[r], q1[r]);
+        else row_dot2<T, V4>(P.Q.at(b, t) + (size_t)i * n, n, d0, x1, q0[r], q1[r]);
         const size_t on = ox + n + i;
         xn0[r] = it.X[on];
         xn1[r] = ws.dx[on];
@@ -241,7 +276,8 @@ __global__ void __launch_bounds__(256) k_trials(Prob<T> P, Prm<T> prm, Iter<T> i
   // control row i = lane: R (u - ur), box bounds
   T ru0 = 0, ru1 = 0, ue0 = 0, ue1 = 0, uu0 = 0, lo = 0, hi = 0;
   if (!term && lane < m) {
-    row_dot2<T, V4>(P.R.at(b, t) + (size_t)lane * m, m, e0, u1, ru0, ru1);
+    if (SH) row_dot2<T, false>(sh + lR + lane * (m + 1), m, e0, u1, ru0, ru1);
+    else row_dot2<T, V4>(P.R.at(b, t) + (size_t)lane * m, m, e0, u1, ru0, ru1);
     ue0 = e0[lane];
     ue1 = u1[lane];
     uu0 = u0[lane];

@@ -312,9 +312,28 @@ mpxkkt_status run_fwd_stage(hipStream_t st, const mpxkkt_problem* pr, const Prob
   return launch_check("k_fwd_stage");
 }
 
+// K1: with flags bit 3 parallel over stages with shared constants
+// (k_linearize_stage: lower latency for small batches) when n, m, n_eq <= 64
+// and its LDS plan fits; else one block per instance (k_linearize_inst, higher
+// throughput for large batches) under the same limits; the general block
+// kernel otherwise.
 template <typename T>
 mpxkkt_status run_linearize(hipStream_t st, const mpxkkt_problem* pr, const Prob<T>& P, const Prm<T>& R,
                             const Iter<T>& I, const Ws<T>& W) {
+  if ((R.flags & 8) != 0 && pr->n <= 64 && pr->m <= 64 && pr->n_eq <= 64) {
+    const InstLds L = inst_lds(pr->n, pr->m, pr->horizon, pr->model == MPXKKT_MODEL_TANH, pr->Q.stage_stride == 0,
+                               pr->R.stage_stride == 0, false);
+    const size_t lds =
+        ((size_t)((L.X + 3) & ~3) + (size_t)kLinStageWaves * lin_stage_wave_elems(pr->n, pr->m)) * sizeof(T);
+    int lim = 0;
+    if (lds_limit(&lim) == MPXKKT_OK && lds <= (size_t)lim) {
+      mpxkkt_status s = prep_lds(k_linearize_stage<T>, lds);
+      if (s != MPXKKT_OK) return s;
+      const dim3 grid((pr->horizon + kLinStageWaves) / kLinStageWaves, pr->batch);
+      hipLaunchKernelGGL(k_linearize_stage<T>, grid, dim3(64 * kLinStageWaves), lds, st, P, R, I, W);
+      return launch_check("k_linearize_stage");
+    }
+  }
   if (use_inst<T>(pr, false)) {
     const InstLds L = inst_lds(pr->n, pr->m, pr->horizon, pr->model == MPXKKT_MODEL_TANH, pr->Q.stage_stride == 0,
                                pr->R.stage_stride == 0, false);
@@ -401,30 +420,58 @@ bool trial_rows16(const mpxkkt_problem* pr) {
   }
 }
 
-template <typename T, int RPL>
-mpxkkt_status launch_trials(hipStream_t st, bool v4, dim3 grid, size_t lds, const Prob<T>& P, const Prm<T>& R,
-                            const Iter<T>& I, const Ws<T>& W, int j0, int j1) {
+// Launch shape of k_trials: with `sh` (tanh model, stage-invariant Q and R,
+// the shared constants fit) 8 waves per block share the instance's matrices
+// in LDS, else 4 waves each read their rows from memory.
+struct TrialShape {
+  bool sh, v4;
+  int waves;
+  size_t lds;
+  dim3 grid;
+};
+
+template <typename T>
+TrialShape trial_shape(const mpxkkt_problem* pr) {
+  TrialShape ts;
+  const int N1 = pr->horizon + 1;
+  ts.v4 = trial_rows16<T>(pr);
+  int lim = 0;
+  const size_t shared = (size_t)trial_shared_elems(pr->n, pr->m) + 8 * (size_t)trial_wave_elems(pr->n, pr->m);
+  ts.sh = pr->model == MPXKKT_MODEL_TANH && pr->Q.stage_stride == 0 && pr->R.stage_stride == 0 &&
+          lds_limit(&lim) == MPXKKT_OK && shared * sizeof(T) <= (size_t)lim;
+  ts.waves = ts.sh ? 8 : kTrialWaves;
+  ts.lds = (ts.sh ? shared : (size_t)kTrialWaves * trial_wave_elems(pr->n, pr->m)) * sizeof(T);
+  ts.grid = dim3((N1 + ts.waves - 1) / ts.waves, pr->batch);
+  return ts;
+}
+
+template <typename T, int RPL, bool V4, bool SH>
+mpxkkt_status launch_trials_k(hipStream_t st, const TrialShape& ts, const Prob<T>& P, const Prm<T>& R,
+                              const Iter<T>& I, const Ws<T>& W, int j0, int j1) {
   mpxkkt_status s;
-  if constexpr (IsF32<T>::value) {
-    if (v4) {
-      if ((s = prep_lds(k_trials<T, RPL, true>, lds)) != MPXKKT_OK) return s;
-      hipLaunchKernelGGL((k_trials<T, RPL, true>), grid, dim3(64 * kTrialWaves), lds, st, P, R, I, W, j0, j1);
-      return launch_check("k_trials");
-    }
-  }
-  if ((s = prep_lds(k_trials<T, RPL, false>, lds)) != MPXKKT_OK) return s;
-  hipLaunchKernelGGL((k_trials<T, RPL, false>), grid, dim3(64 * kTrialWaves), lds, st, P, R, I, W, j0, j1);
+  if ((s = prep_lds(k_trials<T, RPL, V4, SH>, ts.lds)) != MPXKKT_OK) return s;
+  hipLaunchKernelGGL((k_trials<T, RPL, V4, SH>), ts.grid, dim3(64 * ts.waves), ts.lds, st, P, R, I, W, j0, j1);
   return launch_check("k_trials");
 }
 
+template <typename T, int RPL>
+mpxkkt_status launch_trials(hipStream_t st, const TrialShape& ts, const Prob<T>& P, const Prm<T>& R,
+                            const Iter<T>& I, const Ws<T>& W, int j0, int j1) {
+  if (ts.sh) return launch_trials_k<T, RPL, false, true>(st, ts, P, R, I, W, j0, j1);
+  if constexpr (IsF32<T>::value) {
+    if (ts.v4) return launch_trials_k<T, RPL, true, false>(st, ts, P, R, I, W, j0, j1);
+  }
+  return launch_trials_k<T, RPL, false, false>(st, ts, P, R, I, W, j0, j1);
+}
+
 template <typename T>
-mpxkkt_status run_trials(hipStream_t st, const mpxkkt_problem* pr, bool v4, dim3 grid, size_t lds,
-                         const Prob<T>& P, const Prm<T>& R, const Iter<T>& I, const Ws<T>& W, int j0, int j1) {
+mpxkkt_status run_trials(hipStream_t st, const mpxkkt_problem* pr, const TrialShape& ts, const Prob<T>& P,
+                         const Prm<T>& R, const Iter<T>& I, const Ws<T>& W, int j0, int j1) {
   const int rows = std::max(pr->n, pr->n_eq);
-  if (rows <= 64) return launch_trials<T, 1>(st, v4, grid, lds, P, R, I, W, j0, j1);
-  if (rows <= 128) return launch_trials<T, 2>(st, v4, grid, lds, P, R, I, W, j0, j1);
-  if (rows <= 256) return launch_trials<T, 4>(st, v4, grid, lds, P, R, I, W, j0, j1);
-  return launch_trials<T, 8>(st, v4, grid, lds, P, R, I, W, j0, j1);
+  if (rows <= 64) return launch_trials<T, 1>(st, ts, P, R, I, W, j0, j1);
+  if (rows <= 128) return launch_trials<T, 2>(st, ts, P, R, I, W, j0, j1);
+  if (rows <= 256) return launch_trials<T, 4>(st, ts, P, R, I, W, j0, j1);
+  return launch_trials<T, 8>(st, ts, P, R, I, W, j0, j1);
 }
 
 // K4: k_ls_setup, k_trials over [0, group) and then [group, max_trials) for the
@@ -439,13 +486,11 @@ mpxkkt_status run_linesearch(hipStream_t st, const mpxkkt_problem* pr, const Pro
     const size_t setup_lds = (size_t)6 * 256 * sizeof(T);
     hipLaunchKernelGGL(k_ls_setup<T>, dim3(B), dim3(256), setup_lds, st, P, R, I, W);
     if ((s = launch_check("k_ls_setup")) != MPXKKT_OK) return s;
-    const dim3 grid((N1 + kTrialWaves - 1) / kTrialWaves, B);
-    const size_t lds = (size_t)kTrialWaves * trial_wave_elems(pr->n, pr->m) * sizeof(T);
-    const bool v4 = trial_rows16<T>(pr);
-    if ((s = run_trials<T>(st, pr, v4, grid, lds, P, R, I, W, 0, R.group)) != MPXKKT_OK) return s;
-    if (R.group < R.max_trials &&
-        (s = run_trials<T>(st, pr, v4, grid, lds, P, R, I, W, R.group, R.max_trials)) != MPXKKT_OK)
+    const TrialShape ts = trial_shape<T>(pr);
+    if ((s = run_trials<T>(st, pr, ts, P, R, I, W, 0, R.group)) != MPXKKT_OK) return s;
+    if (R.group < R.max_trials && (s = run_trials<T>(st, pr, ts, P, R, I, W, R.group, R.max_trials)) != MPXKKT_OK)
       return s;
+    const dim3 grid((N1 + kTrialWaves - 1) / kTrialWaves, B);
     T* a = so ? static_cast<T*>(so->alpha) : nullptr;
     T* r = so ? static_cast<T*>(so->rho) : nullptr;
     T* f0 = so ? static_cast<T*>(so->merit0) : nullptr;

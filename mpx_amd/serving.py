@@ -146,7 +146,8 @@ def _bucket_params(proto, dtype, device, tune):
     insts = [proto] * 16
     b = DeviceBatch.from_instances(insts, dtype=dtype, device=device)
     it0 = b.iterate_from(*[None] * 5)
-    cands = [dict(segments=1, block_threads=t, trial_group=g) for t in (128, 256, 512) for g in (1, 0)]
+    cands = [dict(segments=1, block_threads=t, trial_group=g, flags=f)
+             for t in (128, 256, 512) for g in (1, 0) for f in (0, 8)]
     best = autotune.tune(b, it0, iters=2, cands=cands)
     return L.default_params(**{k: v for k, v in best.items() if k != "ms"})
 

@@ -188,3 +188,26 @@ def test_stage_parallel_line_search(wl, shape, iters, start, group, dtype):
                 assert _rel(g[key][i], go[key][i]) < 10 * tol, (k, i, key, _rel(g[key][i], go[key][i]))
                 if dtype == "f64" or key == "X":
                     assert _rel(g[key][i], r[key]) < 10 * tol, (k, i, key, _rel(g[key][i], r[key]))
+
+
+@pytest.mark.parametrize("wl,shape,iters,start", [
+    (2, dict(n=8, m=4, N=12), 4, "zero"),
+    (3, dict(n=36, m=12, N=10, p=8), 2, None),
+    (4, dict(n=60, m=9, N=6, p=64), 3, None),
+])
+def test_stage_parallel_linearization(wl, shape, iters, start):
+    """flags bit 3 (k_linearize_stage, one wave per stage) against the oracle
+    in FP64 and against the instance-per-block linearisation in FP32."""
+    insts, starts, refs, out, _, _ = _run(wl, 3, iters, torch.float64, shape=shape, start=start, flags=8)
+    for k in range(iters):
+        g, st = out[k]
+        for i in range(3):
+            r = refs[i][k]
+            assert abs(st["alpha"][i] - r["alpha"]) < 1e-12, (k, i)
+            for key in ("X", "U", "V"):
+                assert _rel(g[key][i], r[key]) < 1e-9, (k, i, key)
+    _, _, _, a, _, _ = _run(wl, 3, iters, torch.float32, shape=shape, start=start, flags=8)
+    _, _, _, b, _, _ = _run(wl, 3, iters, torch.float32, shape=shape, start=start)
+    for k in range(iters):
+        for key in ("X", "U", "V"):
+            assert _rel(a[k][0][key], b[k][0][key]) < TOL32_X, (k, key)
