@@ -23,10 +23,13 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-# The serving leg runs one HIP stream per (shape bucket, staging area); with
-# enough hardware queues for all of them a small shape's launch never queues
-# behind a large shape's kernels (read when HIP initialises, so set first).
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "12")
+# The serving leg runs one HIP stream per (shape bucket, staging area): one
+# each for the two fast shapes, three for the slow one.  With a hardware queue
+# for each of them a small shape's launch never queues behind a large shape's
+# kernels (HIP's default is 4; read when HIP initialises, so set before
+# anything touches the device).
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 PEAK_FP32 = 157.3e12     # MI355X dense FP32 (matrix) FLOP/s
 PEAK_HBM = 8.0e12        # B/s

@@ -365,52 +365,46 @@ __device__ __forceinline__ double rlane(double v, int src) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Wave 0 alone, no block barrier inside (other waves return at once): the
-// upper Cholesky factor U of the SPD m x m matrix H (H = U'U, m <= MM <= 64)
-// into the upper triangle of H, then Y = U^{-1} (upper triangular, zero below)
-// into Y.  Lane k keeps column k in registers: elimination step j takes row j
-// of U from the lanes by readlane and updates the trailing columns in place,
-// so the factorisation touches LDS only to publish U; the back substitution
-// solves U y = e_k with the U entries read back as LDS broadcasts.
+// Wave 0 alone, no block barrier inside (other waves return at once): for the
+// SPD m x m matrix H (LDS, row-major, m <= MM <= 64) the upper Cholesky factor
+// U (H = U'U) and Y = U^{-1} (upper triangular, zero below), written to Y.
+// Lane k keeps column k in registers, padded to MM with the identity so that
+// no step needs a bound: elimination step j takes row j of U from the lanes by
+// readlane and updates the trailing columns in place (lane k's h[j] becomes
+// U[j][k]), and the back substitution solves U y = e_k reading U[i][q] from
+// lane q the same way; LDS is touched only to load H and store Y.
 template <typename T, int MM>
-__device__ void chol_upper_inv_wave(T* H, T* Y, int m) {
+__device__ void chol_upper_inv_wave(const T* H, T* Y, int m) {
   if (threadIdx.x >= 64) return;
   const int k = threadIdx.x;
   const bool own = k < m;
+  const int kc = own ? k : 0;
   T h[MM];
 #pragma unroll
-  for (int i = 0; i < MM; ++i) h[i] = (own && i < m) ? H[i * m + k] : T(0);
-  wave_lds_sync();
+  for (int i = 0; i < MM; ++i) {
+    const T v = H[(i < m ? i : 0) * m + kc];
+    h[i] = (own && i < m) ? v : T(i == k ? 1 : 0);
+  }
 #pragma unroll
   for (int j = 0; j < MM; ++j) {
-    if (j < m) {
-      const T hj = h[j];                                          // H[j][k]
-      const T d = dev_sqrt(rlane(hj, j));
-      const T lk = (own && k > j) ? hj / d : T(0);                // U[j][k], k > j
-      if (own && k >= j) H[j * m + k] = k == j ? d : lk;
+    const T d = dev_sqrt(rlane(h[j], j));
+    const T lk = k > j ? h[j] / d : T(0);
+    h[j] = k == j ? d : lk;
 #pragma unroll
-      for (int i = j + 1; i < MM; ++i)
-        if (i < m) h[i] -= rlane(lk, i) * lk;
-    }
+    for (int i = j + 1; i < MM; ++i) h[i] -= rlane(lk, i) * lk;
   }
-  wave_lds_sync();
   T y[MM];
 #pragma unroll
   for (int i = MM - 1; i >= 0; --i) {
-    if (i < m) {
-      T v = k == i ? T(1) : T(0);
+    T v = k == i ? T(1) : T(0);
 #pragma unroll
-      for (int q = i + 1; q < MM; ++q)
-        if (q < m) v -= H[i * m + q] * y[q];
-      y[i] = v / H[i * m + i];
-    } else {
-      y[i] = T(0);
-    }
+    for (int q = i + 1; q < MM; ++q) v -= rlane(h[i], q) * y[q];
+    y[i] = v / rlane(h[i], i);
   }
   if (own) {
+    // rows past m land on row m - 1 first and are overwritten by it
 #pragma unroll
-    for (int i = 0; i < MM; ++i)
-      if (i < m) Y[i * m + k] = i <= k ? y[i] : T(0);
+    for (int i = MM - 1; i >= 0; --i) Y[(i < m ? i : m - 1) * m + k] = y[i];
   }
   wave_lds_sync();
 }

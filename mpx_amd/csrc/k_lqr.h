@@ -117,16 +117,16 @@ struct IsF32 { static constexpr bool value = false; };
 template <>
 struct IsF32<float> { static constexpr bool value = true; };
 
-// H <- H^{-1} for the SPD m x m matrix in LDS.  m > 16: wave 0 factors
+// H <- H^{-1} for the SPD m x m matrix in LDS.  16 < m <= 32: wave 0 factors
 // H = U'U and forms Y = U^{-1} in tmp with the columns in registers
 // (chol_upper_inv_wave), then the block forms H^{-1} = Y Y' on VALU (m^3
 // flops; one more matrix-core product site here pushes the FP32 kernels into
-// register spills).  m <= 16: Cholesky and the identity solved through both
-// triangles, block-wide in LDS (the register version spills at that template
-// size inside the Riccati kernels).
+// register spills).  Other m: Cholesky and the identity solved through both
+// triangles, block-wide in LDS (the register version spills at those template
+// sizes inside the Riccati kernels).
 template <typename T, int MMAX>
 __device__ void spd_inverse(T* H, T* tmp, int m) {
-  if constexpr (MMAX > 16) {
+  if constexpr (MMAX == 32) {
     chol_upper_inv_wave<T, MMAX>(H, tmp, m);
     lds_sync();
     gemm_lds<T, false, true, 0>(m, m, m, tmp, m, tmp, m, H, m);

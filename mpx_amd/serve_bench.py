@@ -25,7 +25,7 @@ def _plant_step(inst, x, u):
     return x + inst["h"] * (inst["A"] @ x + inst["B"] @ u + inst["kappa"] * np.tanh(inst["W"] @ x))
 
 
-def run(rate, duration, seed=0, life=200, budget_gb=64.0, max_group=256, n_streams=3, warm=True, log=None,
+def run(rate, duration, seed=0, life=200, budget_gb=64.0, max_group=256, n_streams=1, warm=True, log=None,
         max_wall=None, tune="auto"):
     """Serve ``rate`` ticks/s for ``duration`` s; returns a summary dict.  With
     ``max_wall`` the run stops after that many seconds even if arrivals are

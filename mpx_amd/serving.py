@@ -138,10 +138,11 @@ class _Bucket:
 
 
 def _bucket_params(proto, dtype, device, tune):
-    """Launch parameters of a bucket: ``tune`` as given (a dict), or, with
-    ``tune='auto'``, the autotuner's choice for a group of 16 of its shape."""
+    """Launch parameters of a bucket and its time per SQP iteration (ms, or
+    None): ``tune`` as given (a dict), or, with ``tune='auto'``, the
+    autotuner's choice for a group of 16 of its shape."""
     if tune != "auto":
-        return L.default_params(**(tune or {}))
+        return L.default_params(**(tune or {})), None
     from . import autotune
     insts = [proto] * 16
     b = DeviceBatch.from_instances(insts, dtype=dtype, device=device)
@@ -149,19 +150,26 @@ def _bucket_params(proto, dtype, device, tune):
     cands = [dict(segments=1, block_threads=t, trial_group=g, flags=f)
              for t in (128, 256, 512) for g in (1, 0) for f in (0, 8)]
     best = autotune.tune(b, it0, iters=2, cands=cands)
-    return L.default_params(**{k: v for k, v in best.items() if k != "ms"})
+    return L.default_params(**{k: v for k, v in best.items() if k != "ms"}), best["ms"]
 
 
 class MPCServer:
     """Admits controllers by HBM budget and serves their ticks with continuous batching."""
 
-    def __init__(self, hbm_budget_bytes, max_group=256, slots_per_bucket=512, n_streams=2,
-                 dtype=torch.float32, device="cuda", shift=1, tune=None):
+    def __init__(self, hbm_budget_bytes, max_group=256, slots_per_bucket=512, n_streams=1,
+                 dtype=torch.float32, device="cuda", shift=1, tune=None, n_streams_slow=3, slow_ms=2.0):
         self.budget = int(hbm_budget_bytes)
         self.used = 0
         self.max_group = max_group
         self.slots_per_bucket = slots_per_bucket
+        # staging areas (each with its stream) per bucket: n_streams, or
+        # n_streams_slow for a bucket whose SQP iteration takes slow_ms or more
+        # (known with tune='auto'): a fast bucket serves a launch in well under
+        # a millisecond and gains nothing from overlapping its own launches,
+        # while every extra stream competes with the others for the device
         self.n_streams = n_streams
+        self.n_streams_slow = n_streams_slow
+        self.slow_ms = slow_ms
         self.dtype, self.device = dtype, torch.device(device)
         self.shift = shift
         self.tune = tune
@@ -191,9 +199,9 @@ class MPCServer:
         key = shape_key(inst)
         b = self.buckets.get(key)
         if b is None:
-            params = _bucket_params(inst, self.dtype, self.device, self.tune)
-            b = _Bucket(key, inst, self.slots_per_bucket, self.max_group, self.dtype, self.device,
-                        self.n_streams, params)
+            params, ms = _bucket_params(inst, self.dtype, self.device, self.tune)
+            ns = self.n_streams_slow if ms is not None and ms >= self.slow_ms else self.n_streams
+            b = _Bucket(key, inst, self.slots_per_bucket, self.max_group, self.dtype, self.device, ns, params)
             self.buckets[key] = b
         if not b.free:
             return None
