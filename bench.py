@@ -24,7 +24,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 # The serving leg runs one HIP stream per (shape bucket, staging area): one
-# each for the two fast shapes, three for the slow one.  With a hardware queue
+# each for the two fast shapes, two for the slow one.  With a hardware queue
 # for each of them a small shape's launch never queues behind a large shape's
 # kernels (HIP's default is 4; read when HIP initialises, so set before
 # anything touches the device).

@@ -157,7 +157,7 @@ class MPCServer:
     """Admits controllers by HBM budget and serves their ticks with continuous batching."""
 
     def __init__(self, hbm_budget_bytes, max_group=256, slots_per_bucket=512, n_streams=1,
-                 dtype=torch.float32, device="cuda", shift=1, tune=None, n_streams_slow=3, slow_ms=2.0):
+                 dtype=torch.float32, device="cuda", shift=1, tune=None, n_streams_slow=2, slow_ms=2.0):
         self.budget = int(hbm_budget_bytes)
         self.used = 0
         self.max_group = max_group
