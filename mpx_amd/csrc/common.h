@@ -96,12 +96,41 @@ __device__ __forceinline__ void lds_sync() {
 // and keep the compiler from moving memory operations across.
 __device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
-// Sum over the 64 lanes of a wave (every lane gets the result).
+// value of v in lane `src` (uniform), for 32- and 64-bit types
+__device__ __forceinline__ float rlane(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ __forceinline__ double rlane(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// v from the lane CTRL rotates onto this one within its row of 16 (DPP
+// row_ror, a modifier of the add that consumes it: no LDS round trip).
+template <int CTRL>
+__device__ __forceinline__ float dpp_ror(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_ror(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 64 lanes of a wave (every lane gets the result; call with every
+// lane active): rotations by 8, 4, 2, 1 within each row of 16 lanes leave the
+// row's sum in every lane of the row, then the four row sums are read out.
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
+  v += dpp_ror<0x128>(v);
+  v += dpp_ror<0x124>(v);
+  v += dpp_ror<0x122>(v);
+  v += dpp_ror<0x121>(v);
+  return (rlane(v, 0) + rlane(v, 16)) + (rlane(v, 32) + rlane(v, 48));
 }
 
 // Block-wide sum of `cnt` values per thread (cnt <= NV); blockDim.x must be a
@@ -354,16 +383,6 @@ __device__ inline void wave_fence() {
 }
 
 
-// value of v in lane `src` (uniform), for 32- and 64-bit types
-__device__ __forceinline__ float rlane(float v, int src) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
-}
-__device__ __forceinline__ double rlane(double v, int src) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
 
 // Wave 0 alone, no block barrier inside (other waves return at once): for the
 // SPD m x m matrix H (LDS, row-major, m <= MM <= 64) the upper Cholesky factor

@@ -179,7 +179,9 @@ def test_stage_parallel_line_search(wl, shape, iters, start, group, dtype):
             r = refs[i][k]
             assert st["status"][i] == sto["status"][i], (k, i)
             assert st["trials"][i] == sto["trials"][i], (k, i, st["trials"][i], sto["trials"][i])
-            assert st["alpha"][i] == pytest.approx(sto["alpha"][i], rel=1e-6), (k, i)
+            # FP32: the two paths sum in different orders, so the fraction-to-boundary step
+            # can differ in its last bits after a few cold-start iterations
+            assert st["alpha"][i] == pytest.approx(sto["alpha"][i], rel=1e-6 if dtype == "f64" else 1e-4), (k, i)
             if dtype == "f64":
                 assert abs(st["alpha"][i] - r["alpha"]) < 1e-12, (k, i)
             for key in ("X", "U", "V", "Yeq", "Ybox"):
