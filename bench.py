@@ -98,6 +98,9 @@ def main():
     ap.add_argument("--serve", type=float, default=3.0,
                     help="seconds of the mixed serving stream after the batched run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="serving arrivals per second")
+    ap.add_argument("--serve-sweep", default="10000,40000",
+                    help="further arrival rates (comma list, '' for none), each served for --serve seconds "
+                         "with a wall-clock cap; reported in serving_sweep")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB: only copy a buffer of this size twice (reference for the HBM byte counters)")
     args = ap.parse_args()
@@ -213,6 +216,7 @@ def main():
                "sample": c["sample"]}
 
     serving = None
+    serving_sweep = []
     if args.serve > 0:
         from mpx_amd import serve_bench
         sv = serve_bench.run(args.serve_rate, args.serve, seed=args.seed, max_wall=args.serve + 20.0)
@@ -220,6 +224,12 @@ def main():
                                       "backlog", "by_shape")}
         log(f"serving at {args.serve_rate:.0f}/s: p50 {sv['p50_ms']:.2f} ms, p99 {sv['p99_ms']:.2f} ms, "
             f"{sv['solves_per_s']:.0f} solves/s")
+        for rate in [float(r) for r in args.serve_sweep.split(",") if r.strip()]:
+            sw = serve_bench.run(rate, args.serve, seed=args.seed, max_wall=2 * args.serve)
+            serving_sweep.append({k: sw[k] for k in ("rate", "duration", "served", "solves_per_s", "p50_ms",
+                                                     "p99_ms", "backlog", "rejected_for_budget")})
+            log(f"serving at {rate:.0f}/s: {sw['solves_per_s']:.0f} solves/s, p50 {sw['p50_ms']:.2f} ms, "
+                f"p99 {sw['p99_ms']:.2f} ms, backlog {sw['backlog']}, turned away {sw['rejected_for_budget']}")
 
     out = {
         "metric": "SQP iters/sec at B=4096 n=36 N=50 vs plain; p50/p99 MPC tick latency",
@@ -243,6 +253,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "serving": serving,
+        "serving_sweep": serving_sweep,
     }
     print(json.dumps(out))
 

@@ -16,6 +16,7 @@ from __future__ import annotations
 import time
 
 import numpy as np
+import torch
 
 from . import synthetic as syn
 from .serving import MPCServer
@@ -35,10 +36,10 @@ def run(rate, duration, seed=0, life=200, budget_gb=64.0, max_group=256, n_strea
     rng = np.random.default_rng(seed)
     server = MPCServer(budget_gb * 2**30, max_group=max_group, n_streams=n_streams, tune=tune)
     shapes, probs = syn.SERVING_SHAPES, syn.SERVING_PROBS
+    protos = [syn.make_instance(5, 10**6 + si, seed, n=n, m=m, N=N) for si, (n, m, N) in enumerate(shapes)]
     if warm:
         # build every bucket (and its solvers) before the clock starts
-        for si, (n, m, N) in enumerate(shapes):
-            inst = syn.make_instance(5, 10**6 + si, seed, n=n, m=m, N=N)
+        for inst in protos:
             cid = server.open(inst)
             server.request(cid, inst["x0"], 1)
             server.run_round()
@@ -59,14 +60,14 @@ def run(rate, duration, seed=0, life=200, budget_gb=64.0, max_group=256, n_strea
             if idle[k]:
                 cid = idle[k].pop()
             else:
+                if not server.room_for(protos[k]):     # turned away before any set-up work
+                    waits += 1
+                    i += 1
+                    continue
                 n, m, N = shapes[k]
                 serial += 1
                 inst = syn.make_instance(5, serial, seed, n=n, m=m, N=N)
                 cid = server.open(inst)
-                if cid is None:
-                    waits += 1
-                    i += 1
-                    continue
                 state[cid] = dict(inst=inst, x=inst["x0"].copy(), ticks=0, shape=(n, m, N), k=k)
             server.request(cid, state[cid]["x"], int(rng.integers(1, 9)), t_arrive=t0 + t_rel[i])
             i += 1
@@ -88,6 +89,7 @@ def run(rate, duration, seed=0, life=200, budget_gb=64.0, max_group=256, n_strea
             else:
                 idle[s["k"]].append(r.cid)
     wall = time.perf_counter() - t0
+    torch.cuda.synchronize()     # launches left in flight at the wall-clock cap finish before the buffers go
     lat = np.array(latencies) * 1e3
     backlog = (n_arr - i) + len(server.pending) + sum(len(c[1]) for ln in server.in_flight for c in ln.cohorts)
     out = dict(rate=rate, duration=duration, served=len(lat), solves_per_s=len(lat) / wall, backlog=backlog,

@@ -191,6 +191,14 @@ class MPCServer:
         work = N * (2 * n * n + 2 * m * n + 6 * n + 4 * m) + (N + 1) * (n * n + 8 * n)
         return es * (2 * (prob + state) + work)
 
+    def room_for(self, inst):
+        """Whether ``open`` would admit a controller of this instance's shape now
+        (only the shape fields of ``inst`` are read)."""
+        if self.used + self.controller_bytes(inst) > self.budget:
+            return False
+        b = self.buckets.get(shape_key(inst))
+        return b is None or bool(b.free)
+
     def open(self, inst):
         """Admit a controller if its bytes fit the budget; returns its id or None."""
         need = self.controller_bytes(inst)
