@@ -185,6 +185,10 @@ struct WaveStage {
   f4 E;           // sqrt(rho_c) E
   f4 H;           // R + diag(rd), identity in the padding
   f4 r;           // r: lane (g, c) holds r[4 g + v]
+  // with stage-invariant Q, B, R (held by the kernel) only the stage's vectors:
+  f4 qrow[NT];    // q[16 i + 4 g + v]: column n of tile row i
+  float qcol[NT]; // q[16 j + c]: row n of tile column j
+  f4 rdv;         // rd[4 g + v]
 };
 
 // Q~ tiles (i <= j, or all) of Q (ld n) and q
@@ -213,7 +217,9 @@ __device__ __forceinline__ void load_qbar(f4 (&S)[NT][NT], const float* Q, const
     }
 }
 
-template <int NT>
+// CQ: Q, B and R do not vary over the stages and the kernel holds them; the
+// stage brings only q and rd for them.
+template <int NT, bool CQ>
 __device__ __forceinline__ void load_wave_stage(WaveStage<NT>& d, const Prob<float>& P, const Ws<float>& ws,
                                                 const LqrIn<float>& in, int b, int t, float sr, int g, int c,
                                                 const LaneOff<NT>& L) {
@@ -237,11 +243,20 @@ __device__ __forceinline__ void load_wave_stage(WaveStage<NT>& d, const Prob<flo
       d.A[i][j] = x;
     }
   }
-  load_qbar<NT>(d.Q, P.Q.at(b, t), ws.qh + ((size_t)b * N1 + t) * n, n, g, c, true, L);
+  if constexpr (CQ) {
+    const rsrc_t rq = make_rsrc(ws.qh + ((size_t)b * N1 + t) * n, n);
 #pragma unroll
-  for (int i = 0; i < NT; ++i)
+    for (int i = 0; i < NT; ++i) {
+      d.qrow[i] = rows_vec<NT>(rq, i, L);
+      d.qcol[i] = rload(rq, L.c4 + 64 * i);
+    }
+  } else {
+    load_qbar<NT>(d.Q, P.Q.at(b, t), ws.qh + ((size_t)b * N1 + t) * n, n, g, c, true, L);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) d.B[i][v] = rload(rB, L.mm + (16 * i + v) * m * 4);
+    for (int i = 0; i < NT; ++i)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) d.B[i][v] = rload(rB, L.mm + (16 * i + v) * m * 4);
+  }
   if (p > 0) {
     const rsrc_t rD = make_rsrc(P.D.at(b, t), p * n), rE = make_rsrc(P.E.at(b, t), p * m);
 #pragma unroll
@@ -262,7 +277,8 @@ __device__ __forceinline__ void load_wave_stage(WaveStage<NT>& d, const Prob<flo
   for (int v = 0; v < 4; ++v) {
     const int row = 4 * g + v;
     const float dv = rload(rd, L.g16 + v * 4);
-    d.H[v] = rload(rR, L.mm + v * m * 4) + (row == c ? (row < m ? dv : 1.f) : 0.f);
+    if constexpr (CQ) d.rdv[v] = dv;
+    else d.H[v] = rload(rR, L.mm + v * m * 4) + (row == c ? (row < m ? dv : 1.f) : 0.f);
     d.r[v] = rload(rr, L.g16 + v * 4);
   }
 }
@@ -280,7 +296,9 @@ __device__ __forceinline__ void store_value_wave(const f4 (&S)[NT][NT], const Ws
 
 // segments == 1, FP32, n + 1 <= 16 NT, m <= 16, p <= 16: backward Riccati of
 // one instance per wave; stores S~_t and K~_t for k_fwd_wave.
-template <int NT>
+// CQ: Q, B, R stage-invariant (views of stage stride 0): held in registers
+// for the whole recursion instead of being re-read every stage.
+template <int NT, bool CQ>
 __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> prm, Iter<float> it_unused,
                                                     Ws<float> ws, LqrIn<float> in) {
   (void)it_unused;
@@ -295,8 +313,24 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
   load_qbar<NT>(S, P.QN.at(b, 0), ws.qh + ((size_t)b * N1 + N) * n, n, g, c, false, L);
   store_value_wave<NT>(S, ws, b, N, N1, n, L);
 
+  // stage-invariant Q (tiles i <= j, no q), B~ and R (CQ)
+  f4 Qc[NT][NT], Bc[NT], Rc;
+  if constexpr (CQ) {
+    const rsrc_t rQ = make_rsrc(P.Q.at(b, 0), n * n), rB = make_rsrc(in.Bm.at(b, 0), n * m);
+    const rsrc_t rR = make_rsrc(P.R.at(b, 0), m * m);
+#pragma unroll
+    for (int i = 0; i < NT; ++i)
+#pragma unroll
+      for (int j = i; j < NT; ++j) Qc[i][j] = tile_nn<NT>(rQ, n, i, j, L);
+#pragma unroll
+    for (int i = 0; i < NT; ++i)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Bc[i][v] = rload(rB, L.mm + (16 * i + v) * m * 4);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Rc[v] = rload(rR, L.mm + v * m * 4);
+  }
   WaveStage<NT> cur;
-  load_wave_stage<NT>(cur, P, ws, in, b, N - 1, sr, g, c, L);
+  load_wave_stage<NT, CQ>(cur, P, ws, in, b, N - 1, sr, g, c, L);
   for (int t = N - 1; t >= 0; --t) {
     const LaneOff<NT> Ls = lane_offsets<NT>(n, m, fresh(g), fresh(c));
     // SB = S~ B~ ; H = R + diag(rd) + rho_c E'E + B~' SB
@@ -307,14 +341,23 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
 #pragma unroll
       for (int k = 0; k < NT; ++k)
 #pragma unroll
-        for (int s = 0; s < 4; ++s) acc = mma(S[k][i][s], cur.B[k][s], acc);
+        for (int s = 0; s < 4; ++s) acc = mma(S[k][i][s], CQ ? Bc[k][s] : cur.B[k][s], acc);
       SB[i] = acc;
     }
-    f4 H = cur.H;
+    f4 H;
+    if constexpr (CQ) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = 4 * g + v;
+        H[v] = Rc[v] + (row == c ? (row < m ? cur.rdv[v] : 1.f) : 0.f);
+      }
+    } else {
+      H = cur.H;
+    }
 #pragma unroll
     for (int k = 0; k < NT; ++k)
 #pragma unroll
-      for (int s = 0; s < 4; ++s) H = mma(cur.B[k][s], SB[k][s], H);
+      for (int s = 0; s < 4; ++s) H = mma(CQ ? Bc[k][s] : cur.B[k][s], SB[k][s], H);
     // G~ = [0 | r] + rho_c E'[D | 0] + B~' S~ A~ ; Pm = Q~ + A~' S~ A~ (tiles
     // i <= j), one column tile of S~ A~ at a time
     f4 G[NT];
@@ -346,10 +389,21 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
 #pragma unroll
       for (int k = 0; k < NT; ++k)
 #pragma unroll
-        for (int s = 0; s < 4; ++s) G[j] = mma(cur.B[k][s], SAc[k][s], G[j]);
+        for (int s = 0; s < 4; ++s) G[j] = mma(CQ ? Bc[k][s] : cur.B[k][s], SAc[k][s], G[j]);
 #pragma unroll
       for (int i = 0; i <= j; ++i) {
-        f4 acc = cur.Q[i][j];
+        f4 acc;
+        if constexpr (CQ) {   // Q~ = [[Q, q], [q', 0]] from the held Q and this stage's q
+          const int tn = n >> 4;
+          acc = Qc[i][j];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            if (j == tn) acc[v] += (c == L.cn) ? cur.qrow[i][v] : 0.f;
+            if (i == tn) acc[v] += (4 * g + v == L.cn) ? cur.qcol[j] : 0.f;
+          }
+        } else {
+          acc = cur.Q[i][j];
+        }
 #pragma unroll
         for (int k = 0; k < NT; ++k)
 #pragma unroll
@@ -412,7 +466,7 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
     WaveStage<NT> nxt;
     // (the stage index is pinned to a scalar register: selected into a vector
     // register it makes every descriptor built from it look divergent)
-    load_wave_stage<NT>(nxt, P, ws, in, b, __builtin_amdgcn_readfirstlane(t > 0 ? t - 1 : 0), sr, g, c, Ls);
+    load_wave_stage<NT, CQ>(nxt, P, ws, in, b, __builtin_amdgcn_readfirstlane(t > 0 ? t - 1 : 0), sr, g, c, Ls);
 
     // stores: S_t, s_t ; K_t, k_t
     store_value_wave<NT>(S, ws, b, t, N1, n, Ls);

@@ -368,9 +368,17 @@ mpxkkt_status run_lqr_wave(hipStream_t st, const mpxkkt_problem* pr, const Prob<
                            const Iter<T>& I, const Ws<T>& W, const LqrIn<T>& in) {
   if constexpr (IsF32<T>::value) {
     const int NT = (pr->n + 1 + 15) / 16;
-    if (NT == 1) hipLaunchKernelGGL(k_lqr_wave<1>, dim3(pr->batch), dim3(64), 0, st, P, R, I, W, in);
-    else if (NT == 2) hipLaunchKernelGGL(k_lqr_wave<2>, dim3(pr->batch), dim3(64), 0, st, P, R, I, W, in);
-    else hipLaunchKernelGGL(k_lqr_wave<3>, dim3(pr->batch), dim3(64), 0, st, P, R, I, W, in);
+    const bool cq = P.Q.st == 0 && P.R.st == 0 && in.Bm.st == 0;   // Q, B, R held for the whole recursion
+    const dim3 gb(pr->batch);
+    if (cq) {
+      if (NT == 1) hipLaunchKernelGGL((k_lqr_wave<1, true>), gb, dim3(64), 0, st, P, R, I, W, in);
+      else if (NT == 2) hipLaunchKernelGGL((k_lqr_wave<2, true>), gb, dim3(64), 0, st, P, R, I, W, in);
+      else hipLaunchKernelGGL((k_lqr_wave<3, true>), gb, dim3(64), 0, st, P, R, I, W, in);
+    } else {
+      if (NT == 1) hipLaunchKernelGGL((k_lqr_wave<1, false>), gb, dim3(64), 0, st, P, R, I, W, in);
+      else if (NT == 2) hipLaunchKernelGGL((k_lqr_wave<2, false>), gb, dim3(64), 0, st, P, R, I, W, in);
+      else hipLaunchKernelGGL((k_lqr_wave<3, false>), gb, dim3(64), 0, st, P, R, I, W, in);
+    }
     return launch_check("k_lqr_wave");
   } else {
     return fail(MPXKKT_ERR_UNSUPPORTED, "wave LQR kernel is FP32 only");
