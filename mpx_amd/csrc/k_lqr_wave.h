@@ -506,8 +506,9 @@ __device__ __forceinline__ void load_aop(f4 (&out)[RT][NT], const float* M, cons
   }
 }
 
-// One forward step's A-operand tiles: A~_t, K~_t and B~_t.
-template <int NT>
+// One forward step's A-operand tiles: A~_t, K~_t and (unless the kernel holds
+// a stage-invariant B, CB) B~_t.
+template <int NT, bool CB>
 __device__ __forceinline__ void load_chain_stage(f4 (&A)[NT][NT], f4 (&K)[1][NT], f4 (&BT)[NT], const Ws<float>& ws,
                                                  const LqrIn<float>& in, int b, int t, int N, int n, int m, int g,
                                                  int c, const int (&off)[NT][4], const LaneOff<NT>& L) {
@@ -515,6 +516,7 @@ __device__ __forceinline__ void load_chain_stage(f4 (&A)[NT][NT], f4 (&K)[1][NT]
   load_aop<NT, NT>(A, in.A.at(b, t), ws.cdef + ((size_t)b * N1 + t + 1) * n, n, n, true, g, c, off, L);
   load_aop<NT, 1>(K, ws.K + ((size_t)b * N + t) * m * n, ws.k + ((size_t)b * N + t) * m, n, m, false, g, c, off,
                   L);
+  if constexpr (CB) return;
   const rsrc_t rB = make_rsrc(in.Bm.at(b, t), n * m);
 #pragma unroll
   for (int i = 0; i < NT; ++i)
@@ -540,7 +542,8 @@ __device__ __forceinline__ void aop_offsets(int (&off)[NT][4], int n, int g, int
 //   du_t = -K~_t x~_t,  x~_{t+1} = A~_t x~_t + B~_t du_t
 // on the matrix cores; writes dx and du.  The next stage's tiles are requested
 // before this stage's stores.
-template <int NT>
+// CB: B~ stage-invariant (stride 0), loaded once.
+template <int NT, bool CB>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 3))) k_fwd_wave(Prob<float> P, Ws<float> ws, LqrIn<float> in) {
   const int b = blockIdx.x;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
@@ -558,7 +561,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 3)))
         x[k][v] = rload(r0, L.col0 + (16 * k + v) * 4) + ((c == 0 && 16 * k + 4 * g + v == n) ? 1.f : 0.f);
   }
   f4 cA[NT][NT], cK[1][NT], cB[NT];
-  load_chain_stage<NT>(cA, cK, cB, ws, in, b, 0, N, n, m, g, c, off, L);
+  if constexpr (CB) {
+    const rsrc_t rB = make_rsrc(in.Bm.at(b, 0), n * m);
+#pragma unroll
+    for (int i = 0; i < NT; ++i)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) cB[i][s] = rload(rB, L.bt[s] + 16 * i * m * 4);
+  }
+  load_chain_stage<NT, CB>(cA, cK, cB, ws, in, b, 0, N, n, m, g, c, off, L);
   for (int t = 0; t < N; ++t) {
     // K~ x~ (tile 0, column 0), then x~_{t+1} = A~ x~ - B~ (K~ x~)
     f4 kx = zero4();
@@ -584,8 +594,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 3)))
       const LaneOff<NT> Ls = lane_offsets<NT>(n, m, gf, cf);
       int offs[NT][4];
       aop_offsets<NT>(offs, n, gf, cf);
-      load_chain_stage<NT>(nA, nK, nB, ws, in, b, __builtin_amdgcn_readfirstlane(t + 1 < N ? t + 1 : t), N, n, m, gf,
-                           cf, offs, Ls);
+      load_chain_stage<NT, CB>(nA, nK, nB, ws, in, b, __builtin_amdgcn_readfirstlane(t + 1 < N ? t + 1 : t), N, n, m,
+                               gf, cf, offs, Ls);
     }
     {
       const rsrc_t rdx = make_rsrc(ws.dx + ((size_t)b * N1 + t) * n, n);
@@ -601,7 +611,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 3)))
     for (int k = 0; k < NT; ++k) {
       x[k] = y[k];
       cK[0][k] = nK[0][k];
-      cB[k] = nB[k];
+      if constexpr (!CB) cB[k] = nB[k];
 #pragma unroll
       for (int i = 0; i < NT; ++i) cA[i][k] = nA[i][k];
     }

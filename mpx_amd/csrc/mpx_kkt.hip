@@ -291,14 +291,19 @@ mpxkkt_status run_fwd_stage(hipStream_t st, const mpxkkt_problem* pr, const Prob
     if (use_wave_lqr<T>(pr, R)) {
       const int NT = (pr->n + 1 + 15) / 16;
       const dim3 gc(pr->horizon + 1, pr->batch);
+      const dim3 gb(pr->batch);
+      const bool cb = in.Bm.st == 0;   // B~ held for the whole chain
       if (NT == 1) {
-        hipLaunchKernelGGL(k_fwd_wave<1>, dim3(pr->batch), dim3(64), 0, st, P, W, in);
+        if (cb) hipLaunchKernelGGL((k_fwd_wave<1, true>), gb, dim3(64), 0, st, P, W, in);
+        else hipLaunchKernelGGL((k_fwd_wave<1, false>), gb, dim3(64), 0, st, P, W, in);
         hipLaunchKernelGGL(k_costate_wave<1>, gc, dim3(64), 0, st, P, I, W);
       } else if (NT == 2) {
-        hipLaunchKernelGGL(k_fwd_wave<2>, dim3(pr->batch), dim3(64), 0, st, P, W, in);
+        if (cb) hipLaunchKernelGGL((k_fwd_wave<2, true>), gb, dim3(64), 0, st, P, W, in);
+        else hipLaunchKernelGGL((k_fwd_wave<2, false>), gb, dim3(64), 0, st, P, W, in);
         hipLaunchKernelGGL(k_costate_wave<2>, gc, dim3(64), 0, st, P, I, W);
       } else {
-        hipLaunchKernelGGL(k_fwd_wave<3>, dim3(pr->batch), dim3(64), 0, st, P, W, in);
+        if (cb) hipLaunchKernelGGL((k_fwd_wave<3, true>), gb, dim3(64), 0, st, P, W, in);
+        else hipLaunchKernelGGL((k_fwd_wave<3, false>), gb, dim3(64), 0, st, P, W, in);
         hipLaunchKernelGGL(k_costate_wave<3>, gc, dim3(64), 0, st, P, I, W);
       }
       return launch_check("k_fwd_wave / k_costate_wave");
