@@ -186,9 +186,18 @@ __device__ void stage_terms(const Prob<T>& P, const Prm<T>& prm, const Iter<T>& 
   if (LIN && L.A >= 0) {
     wave_lds_sync();
     T* Al = ws.Alin + ((size_t)b * N + t) * n * n;
+    // element e2 = i n + j sits at e2 + i in the padded LDS rows; (i, j) advance
+    // by (64 / n, 64 % n) with a carry instead of a division per element
+    const int di = 64 / n, dj = 64 - di * n;
+    int i = lane / n, j = lane - (lane / n) * n;
     for (int e2 = lane; e2 < n * n; e2 += 64) {
-      const int i = e2 / n, j = e2 - i * n;
-      Al[e2] = (i == j ? T(1) : T(0)) + P.h * (sm[L.A + i * (n + 1) + j] + P.kappa * sd[i] * sm[L.W + i * (n + 1) + j]);
+      Al[e2] = (i == j ? T(1) : T(0)) + P.h * (sm[L.A + e2 + i] + P.kappa * sd[i] * sm[L.W + e2 + i]);
+      i += di;
+      j += dj;
+      if (j >= n) {
+        j -= n;
+        ++i;
+      }
     }
   }
   out[0] = wave_sum(cost);
