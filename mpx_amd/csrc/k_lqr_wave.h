@@ -69,6 +69,10 @@ __device__ __forceinline__ int fresh(int x) {
 __device__ __forceinline__ float rload(rsrc_t r, int voff) {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0));
 }
+// four consecutive floats at a 16-byte aligned offset (all in range or all past it)
+__device__ __forceinline__ f4 rload4(rsrc_t r, int voff) {
+  return __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0));
+}
 __device__ __forceinline__ void rstore(rsrc_t r, int voff, float v) {
   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, 0, 0);
 }
@@ -489,12 +493,20 @@ template <int NT, int RT>
 __device__ __forceinline__ void load_aop(f4 (&out)[RT][NT], const float* M, const float* colv, int n, int nrows,
                                          bool corner, int g, int c, const int (&off)[NT][4], const LaneOff<NT>& L) {
   const rsrc_t rM = make_rsrc(M, nrows * n), rv = make_rsrc(colv, nrows);
+  // n % 4 == 0: a lane's four columns 16 k + 4 g .. + 3 of a row are contiguous,
+  // 16-byte aligned, and either all below n or all past it: one 128-bit load
+  const bool wide = (n & 3) == 0;
 #pragma unroll
   for (int i = 0; i < RT; ++i) {
 #pragma unroll
-    for (int k = 0; k < NT; ++k)
+    for (int k = 0; k < NT; ++k) {
+      if (wide) {
+        out[i][k] = rload4(rM, off[k][0] + 16 * i * n * 4);
+      } else {
 #pragma unroll
-      for (int s = 0; s < 4; ++s) out[i][k][s] = rload(rM, off[k][s] + 16 * i * n * 4);
+        for (int s = 0; s < 4; ++s) out[i][k][s] = rload(rM, off[k][s] + 16 * i * n * 4);
+      }
+    }
     const float cv = rload(rv, L.c4 + 64 * i);   // colv[16 i + c]
     const float cc = (corner && 16 * i + c == n) ? 1.f : 0.f;
     // column n by a per-lane test (a test on the tile index alone lets the
