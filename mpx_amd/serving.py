@@ -221,7 +221,13 @@ class MPCServer:
         return self._cid
 
     def close(self, cid):
-        c = self.controllers.pop(cid)
+        """Release a controller's slot and bytes; its pending ticks are dropped.
+        A controller whose tick is on the device cannot close until it returns."""
+        c = self.controllers[cid]
+        if c.busy:
+            raise RuntimeError(f"controller {cid} has a tick in flight")
+        self.pending = collections.deque(r for r in self.pending if r.cid != cid)
+        del self.controllers[cid]
         c.bucket.free.append(c.slot)
         self.used -= c.nbytes
 
@@ -245,12 +251,14 @@ class MPCServer:
         return done
 
     def run_round(self):
-        """Serve every request pending now (and nothing that arrives meanwhile);
-        blocks until they are done and returns them."""
+        """Serve every request pending now; blocks until they are done and
+        returns them (with anything else that finished meanwhile)."""
         want = {r.rid for r in self.pending}
         done = []
         while want:
             self._dispatch()
+            if not self.in_flight:
+                raise RuntimeError("pending ticks that cannot be launched")
             got = self._harvest(block=True)
             done.extend(got)
             want -= {r.rid for r in got}

@@ -110,3 +110,21 @@ def test_overlapped_launches_equal_alone():
             r = alone.request(c, _x(d, k), it)
             assert len(alone.run_round()) == 1
             assert np.array_equal(r.u0, reqs[(i, k)].u0), (i, k)
+
+
+def test_close_drops_pending_ticks():
+    insts = _insts()[:2]
+    srv = _server()
+    a, b = srv.open(insts[0]), srv.open(insts[1])
+    srv.request(a, _x(insts[0], 0), 2)
+    rb = srv.request(b, _x(insts[1], 0), 2)
+    srv.close(a)                      # its queued tick goes with it
+    done = srv.run_round()
+    assert [r.rid for r in done] == [rb.rid]
+    srv.request(b, _x(insts[1], 1), 3)
+    srv.step()                        # launched, not yet harvested
+    with pytest.raises(RuntimeError):
+        srv.close(b)
+    while srv.busy:
+        srv.step()
+    srv.close(b)
