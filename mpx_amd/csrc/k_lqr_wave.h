@@ -99,25 +99,45 @@ __device__ __forceinline__ f4 tile_transpose(const f4& x, float* buf, int g, int
   return y;
 }
 
-// In-place inverse of the SPD tile H (m x m in the top-left corner, identity
-// elsewhere): Gauss-Jordan without pivoting, pivot row j broadcast down the
-// columns, pivot column j broadcast along the rows.
-__device__ __forceinline__ void gj_inverse(f4& h, int m, int g, int c) {
-  for (int j = 0; j < m; ++j) {
-    const int jg = j >> 2, jv = j & 3;
-    const float rowj = __shfl(f4_at(h, jv), jg * 16 + c);   // H[j][c]
-    const float a = __shfl(rowj, j);                         // H[j][j] (lane j: g = 0, c = j)
-    f4 colj;
+// Lane j of each 16-lane row of the wave, broadcast to the row (DPP row_newbcast).
+template <int J>
+__device__ __forceinline__ float row_bcast(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x150 + J, 0xf, 0xf, false));
+}
+
+// Pivot J of gj_inverse.  Lane (g, c) holds H[4 g + v][c] in register v: the
+// pivot row H[J][c] comes from lane 16 (J / 4) + c, the pivot column
+// H[4 g + v][J] from lane J of the lane's own row of 16 (a DPP broadcast), and
+// H[J][J] from one lane (a scalar read).
+template <int J>
+__device__ __forceinline__ void gj_pivot(f4& h, int g, int c) {
+  constexpr int jg = J >> 2, jv = J & 3;
+  const float rowj = __shfl(h[jv], jg * 16 + c);                  // H[J][c]
+  const float a = rlane(h[jv], jg * 16 + J);                      // H[J][J]
+  f4 colj;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) colj[v] = __shfl(h[v], g * 16 + j);   // H[4g+v][j]
-    const float rnew = (c == j ? 1.f : rowj) / a;
+  for (int v = 0; v < 4; ++v) colj[v] = row_bcast<J>(h[v]);       // H[4g+v][J]
+  const float rnew = (c == J ? 1.f : rowj) / a;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const float hv = (c == j) ? 0.f : h[v];
-      h[v] = (4 * g + v == j) ? rnew : hv - colj[v] * rnew;
-    }
+  for (int v = 0; v < 4; ++v) {
+    const float hv = (c == J) ? 0.f : h[v];
+    h[v] = (4 * g + v == J) ? rnew : hv - colj[v] * rnew;
   }
 }
+
+template <int J>
+__device__ __forceinline__ void gj_pivots(f4& h, int g, int c) {
+  if constexpr (J < 16) {
+    gj_pivot<J>(h, g, c);
+    gj_pivots<J + 1>(h, g, c);
+  }
+}
+
+// In-place inverse of the SPD tile H (m x m in the top-left corner, identity
+// elsewhere): Gauss-Jordan without pivoting over all 16 pivots, straight-line
+// code that the scheduler can place among independent matrix products.  Past
+// the m x m block H is the identity, where a pivot changes nothing.
+__device__ __forceinline__ void gj_inverse(f4& h, int g, int c) { gj_pivots<0>(h, g, c); }
 
 // Loop-invariant lane offsets (bytes) of one instance's shapes.
 template <int NT>
@@ -377,6 +397,8 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
 #pragma unroll
         for (int s = 0; s < 4; ++s) G[j] = mma(cur.E[s], cur.D[j][s], G[j]);
     }
+    // H <- H^{-1} (symmetrised below, once the products of this stage are in)
+    gj_inverse(H, g, c);
     f4 Pm[NT][NT];
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
@@ -416,8 +438,7 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
       }
     }
 
-    // H <- H^{-1}, symmetrised so that it serves as its own transpose
-    gj_inverse(H, m, g, c);
+    // H^{-1} symmetrised so that it serves as its own transpose
     {
       const f4 Ht = tile_transpose(H, tbuf, g, c);
 #pragma unroll
