@@ -22,7 +22,10 @@
 namespace mpxk {
 
 constexpr int kInstWaves = 4;
-constexpr int kInstScratch = 4 * 64;   // per wave: x - xr, u - ur, mu + rho_c h, sech^2(W x)
+// per wave: x - xr, u - ur, mu + rho_c h, sech^2(W x), then the stage's
+// equality rows [D_t | E_t] when they fit in kEqStage elements
+constexpr int kEqStage = 512;
+constexpr int kInstScratch = 4 * 64 + kEqStage;
 
 // Offsets (elements) of the instance LDS plan.
 struct InstLds {
@@ -91,17 +94,61 @@ __device__ void stage_terms(const Prob<T>& P, const Prm<T>& prm, const Iter<T>& 
   T* w = scr + 128;    // mu + rho_c h
   T* sd = scr + 192;   // sech^2(W x)
   T cost = 0, vc = 0, cc = 0;
-  const T* xr = P.xref.at(b, t);
-  const T* ur = P.uref.at(b, t);
-  if (lane < n) d[lane] = x[lane] - xr[lane];
-  if (lane < m) e[lane] = u[lane] - ur[lane];
-  if (lane < p) {
+  // The stage's per-lane entries in global memory are requested together up
+  // front: the LDS syncs below keep the compiler from moving a load across
+  // them, and each load left behind one costs the wave a memory round trip.
+  const T xr_l = lane < n ? P.xref.at(b, t)[lane] : T(0);
+  const T ur_l = lane < m ? P.uref.at(b, t)[lane] : T(0);
+  const T e_l = lane < p ? P.e.at(b, t)[lane] : T(0);
+  const T mu_l = lane < p ? it.Yeq[((size_t)b * N + t) * p + lane] : T(0);
+  const T vn_l = lane < n ? vnext[lane] : T(0);
+  T lo_l = 0, hi_l = 0, nhi_l = 0, nlo_l = 0;
+  if (P.box && lane < m) {
+    lo_l = P.ulo.at(b, t)[lane];
+    hi_l = P.uhi.at(b, t)[lane];
+    if (LIN) {
+      const T* nu = it.Ybox + ((size_t)b * N + t) * 2 * m;
+      nhi_l = nu[lane];
+      nlo_l = nu[m + lane];
+    }
+  }
+  if (lane < n) d[lane] = x[lane] - xr_l;
+  if (lane < m) e[lane] = u[lane] - ur_l;
+  // The stage's equality rows [D_t | E_t] go to the wave's scratch with every
+  // load in flight at once: the dot-product loops below would otherwise wait
+  // on global memory once per group of unrolled iterations.
+  const bool eq_lds = p > 0 && p * (n + m) <= kEqStage;
+  T* Ds = scr + 256;
+  T* Es = Ds + p * n;
+  if (eq_lds) {
     const T* D = P.D.at(b, t);
     const T* E = P.E.at(b, t);
-    T h = P.e.at(b, t)[lane];
-    for (int j = 0; j < n; ++j) h += D[lane * n + j] * x[j];
-    for (int j = 0; j < m; ++j) h += E[lane * m + j] * u[j];
-    const T mu = it.Yeq[((size_t)b * N + t) * p + lane];
+    const int pn = p * n, pt = p * (n + m);
+    T r[kEqStage / 64];
+#pragma unroll
+    for (int k = 0; k < kEqStage / 64; ++k) {
+      const int idx = lane + 64 * k;
+      r[k] = idx < pn ? D[idx] : (idx < pt ? E[idx - pn] : T(0));
+    }
+#pragma unroll
+    for (int k = 0; k < kEqStage / 64; ++k) {
+      const int idx = lane + 64 * k;
+      if (idx < pt) Ds[idx] = r[k];
+    }
+    wave_lds_sync();
+  }
+  if (lane < p) {
+    T h = e_l;
+    if (eq_lds) {
+      for (int j = 0; j < n; ++j) h += Ds[lane * n + j] * x[j];
+      for (int j = 0; j < m; ++j) h += Es[lane * m + j] * u[j];
+    } else {
+      const T* D = P.D.at(b, t);
+      const T* E = P.E.at(b, t);
+      for (int j = 0; j < n; ++j) h += D[lane * n + j] * x[j];
+      for (int j = 0; j < m; ++j) h += E[lane * m + j] * u[j];
+    }
+    const T mu = mu_l;
     w[lane] = mu + prm.rho_c * h;
     cost += mu * h + T(0.5) * prm.rho_c * h * h;
   }
@@ -118,17 +165,19 @@ __device__ void stage_terms(const Prob<T>& P, const Prm<T>& prm, const Iter<T>& 
     }
     cost += T(0.5) * e[i] * Ru;
     T r = Ru, dg = 0;
-    if (p > 0) {
+    if (eq_lds) {
+      for (int k = 0; k < p; ++k) r += Es[k * m + i] * w[k];
+    } else if (p > 0) {
       const T* E = P.E.at(b, t);
       for (int k = 0; k < p; ++k) r += E[k * m + i] * w[k];
     }
     if (P.box) {
-      const T lo = P.ulo.at(b, t)[i], hi = P.uhi.at(b, t)[i];
+      const T lo = lo_l, hi = hi_l;
       const T shi = hi - u[i], slo = u[i] - lo;
       cost -= prm.mu_b * (dev_log(shi) + dev_log(slo));
       if (LIN) {
         T* nu = it.Ybox + ((size_t)b * N + t) * 2 * m;
-        T nhi = nu[i], nlo = nu[m + i];
+        T nhi = nhi_l, nlo = nlo_l;
         if (!(nhi > T(0))) { nhi = prm.mu_b / shi; nu[i] = nhi; }
         if (!(nlo > T(0))) { nlo = prm.mu_b / slo; nu[m + i] = nlo; }
         r += prm.mu_b / shi - prm.mu_b / slo;
@@ -171,11 +220,13 @@ __device__ void stage_terms(const Prob<T>& P, const Prm<T>& prm, const Iter<T>& 
       f = ax + bu;
     }
     const T c = f - xn[i];
-    vc += vnext[i] * c;
+    vc += vn_l * c;
     cc += c * c;
     if (LIN) {
       T q = Qx;
-      if (p > 0) {
+      if (eq_lds) {
+        for (int k = 0; k < p; ++k) q += Ds[k * n + i] * w[k];
+      } else if (p > 0) {
         const T* D = P.D.at(b, t);
         for (int k = 0; k < p; ++k) q += D[k * n + i] * w[k];
       }
@@ -236,8 +287,10 @@ __device__ void terminal_terms(const Prob<T>& P, const Ws<T>& ws, int b, const T
   wave_lds_sync();
 }
 
+// (four waves per SIMD: the stage loop is latency bound, and the register
+// budget of four waves keeps it from spilling)
 template <typename T>
-__global__ void __launch_bounds__(256) k_linearize_inst(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_linearize_inst(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws) {
   const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int n = P.n, m = P.m, N = P.N, N1 = N + 1;
