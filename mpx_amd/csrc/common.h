@@ -96,6 +96,45 @@ __device__ __forceinline__ void lds_sync() {
 // and keep the compiler from moving memory operations across.
 __device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
+// Block copies from global memory into LDS with K loads of each thread in
+// flight before its first store (a plain strided loop waits on global memory
+// once per iteration).  lds_copy: dst[e] = src[e]; lds_copy_padded: the
+// row-major rows of length ld land in rows of length ld + 1, dst[e + e / ld].
+template <typename T, int K>
+__device__ __forceinline__ void lds_copy(T* dst, const T* src, int cnt) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int base = 0; base < cnt; base += K * nt) {
+    T v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int e = base + tid + k * nt;
+      v[k] = e < cnt ? src[e] : T(0);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int e = base + tid + k * nt;
+      if (e < cnt) dst[e] = v[k];
+    }
+  }
+}
+template <typename T, int K>
+__device__ __forceinline__ void lds_copy_padded(T* dst, const T* src, int rows, int ld) {
+  const int tid = threadIdx.x, nt = blockDim.x, cnt = rows * ld;
+  for (int base = 0; base < cnt; base += K * nt) {
+    T v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int e = base + tid + k * nt;
+      v[k] = e < cnt ? src[e] : T(0);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int e = base + tid + k * nt;
+      if (e < cnt) dst[e + e / ld] = v[k];
+    }
+  }
+}
+
 // value of v in lane `src` (uniform), for 32- and 64-bit types
 __device__ __forceinline__ float rlane(float v, int src) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));

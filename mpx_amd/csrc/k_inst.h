@@ -54,29 +54,14 @@ __host__ __device__ inline InstLds inst_lds(int n, int m, int N, bool tanh_model
 
 template <typename T>
 __device__ void inst_stage_constants(const Prob<T>& P, int b, T* sm, const InstLds& L) {
-  const int n = P.n, m = P.m, tid = threadIdx.x, nt = blockDim.x;
+  const int n = P.n, m = P.m;
   if (L.A >= 0) {
-    const T* A = P.A.at(b, 0);
-    const T* W = P.W.at(b, 0);
-    for (int e = tid; e < n * n; e += nt) {
-      const int i = e / n, j = e - i * n;
-      sm[L.A + i * (n + 1) + j] = A[e];
-      sm[L.W + i * (n + 1) + j] = W[e];
-    }
-    const T* Bm = P.Bm.at(b, 0);
-    for (int e = tid; e < n * m; e += nt) {
-      const int i = e / m, j = e - i * m;
-      sm[L.B + i * (m + 1) + j] = Bm[e];
-    }
+    lds_copy_padded<T, 8>(sm + L.A, P.A.at(b, 0), n, n);
+    lds_copy_padded<T, 8>(sm + L.W, P.W.at(b, 0), n, n);
+    lds_copy_padded<T, 4>(sm + L.B, P.Bm.at(b, 0), n, m);
   }
-  if (L.Q >= 0) {
-    const T* Q = P.Q.at(b, 0);
-    for (int e = tid; e < n * n; e += nt) sm[L.Q + (e / n) * (n + 1) + e % n] = Q[e];
-  }
-  if (L.R >= 0) {
-    const T* R = P.R.at(b, 0);
-    for (int e = tid; e < m * m; e += nt) sm[L.R + (e / m) * (m + 1) + e % m] = R[e];
-  }
+  if (L.Q >= 0) lds_copy_padded<T, 8>(sm + L.Q, P.Q.at(b, 0), n, n);
+  if (L.R >= 0) lds_copy_padded<T, 4>(sm + L.R, P.R.at(b, 0), m, m);
 }
 
 // Stage t < N of the trajectory (x = X[t], u = U[t], x' = X[t+1] in LDS; v' =
@@ -300,23 +285,44 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   inst_stage_constants(P, b, sm, L);
   const T* Xb = it.X + (size_t)b * N1 * n;
   const T* Vb = it.V + (size_t)b * N1 * n;
-  for (int e = tid; e < N1 * n; e += nt) sm[L.X + e] = Xb[e];
+  lds_copy<T, 8>(sm + L.X, Xb, N1 * n);
   T* Ub = it.U + (size_t)b * N * m;
-  for (int e = tid; e < N * m; e += nt) {
-    T u = Ub[e];
-    if (P.box) {   // box projection into the interior (in place)
-      const int t = e / m, i = e - t * m;
-      const T lo = P.ulo.at(b, t)[i], hi = P.uhi.at(b, t)[i];
-      const T w = T(1e-3) * (hi - lo);
-      if (u >= hi) u = hi - w;
-      if (u <= lo) u = lo + w;
-      Ub[e] = u;
+  // U with the box projection into the interior (in place), four loads of a
+  // thread in flight at a time
+  for (int base = 0; base < N * m; base += 4 * nt) {
+    T uv[4], lov[4], hiv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = base + tid + k * nt;
+      uv[k] = lov[k] = hiv[k] = T(0);
+      if (e < N * m) {
+        uv[k] = Ub[e];
+        if (P.box) {
+          const int t = e / m, i = e - t * m;
+          lov[k] = P.ulo.at(b, t)[i];
+          hiv[k] = P.uhi.at(b, t)[i];
+        }
+      }
     }
-    sm[L.U + e] = u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = base + tid + k * nt;
+      if (e < N * m) {
+        T u = uv[k];
+        if (P.box) {
+          const T lo = lov[k], hi = hiv[k];
+          const T w = T(1e-3) * (hi - lo);
+          if (u >= hi) u = hi - w;
+          if (u <= lo) u = lo + w;
+          Ub[e] = u;
+        }
+        sm[L.U + e] = u;
+      }
+    }
   }
-  if (P.model == 1)
-    for (int e = tid; e < n * m; e += nt) ws.Blin[(size_t)b * n * m + e] = P.h * P.Bm.at(b, 0)[e];
   __syncthreads();
+  if (P.model == 1)   // B of the tanh model's f_u = h B, from its LDS copy
+    for (int e = tid; e < n * m; e += nt) ws.Blin[(size_t)b * n * m + e] = P.h * sm[L.B + e + e / m];
   T* scr = sm + L.scratch + wv * kInstScratch;
   for (int t = wv; t < N1; t += kInstWaves) {
     T o[3];

@@ -180,16 +180,53 @@ __global__ void __launch_bounds__(512) k_trials(Prob<T> P, Prm<T> prm, Iter<T> i
     const T* A = P.A.at(b, 0);
     const T* W = P.W.at(b, 0);
     const T* Q = P.Q.at(b, 0);
-    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-      const int i = e / n, j = e - i * n, o = i * (n + 1) + j;
-      sh[lA + o] = A[e];
-      sh[lW + o] = W[e];
-      sh[lQ + o] = Q[e];
-    }
     const T* Bm = P.Bm.at(b, 0);
-    for (int e = threadIdx.x; e < n * m; e += blockDim.x) sh[lB + (e / m) * (m + 1) + e % m] = Bm[e];
     const T* R = P.R.at(b, 0);
-    for (int e = threadIdx.x; e < m * m; e += blockDim.x) sh[lR + (e / m) * (m + 1) + e % m] = R[e];
+    const int tid = threadIdx.x, nt = blockDim.x, nn = n * n, nm = n * m, mm = m * m;
+    // element e = i ld + j of a row-major matrix goes to e + i in the padded rows
+    if (nn <= 4 * nt && nm <= 2 * nt && mm <= nt) {
+      // one pass with every load of the thread in flight before its first store
+      // (a loop would wait on global memory once per iteration)
+      T va[4], vw[4], vq[4], vb[2], vr;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int e = tid + k * nt;
+        va[k] = e < nn ? A[e] : T(0);
+        vw[k] = e < nn ? W[e] : T(0);
+        vq[k] = e < nn ? Q[e] : T(0);
+      }
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const int e = tid + k * nt;
+        vb[k] = e < nm ? Bm[e] : T(0);
+      }
+      vr = tid < mm ? R[tid] : T(0);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int e = tid + k * nt;
+        if (e < nn) {
+          const int o = e + e / n;
+          sh[lA + o] = va[k];
+          sh[lW + o] = vw[k];
+          sh[lQ + o] = vq[k];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const int e = tid + k * nt;
+        if (e < nm) sh[lB + e + e / m] = vb[k];
+      }
+      if (tid < mm) sh[lR + tid + tid / m] = vr;
+    } else {
+      for (int e = tid; e < nn; e += nt) {
+        const int o = e + e / n;
+        sh[lA + o] = A[e];
+        sh[lW + o] = W[e];
+        sh[lQ + o] = Q[e];
+      }
+      for (int e = tid; e < nm; e += nt) sh[lB + e + e / m] = Bm[e];
+      for (int e = tid; e < mm; e += nt) sh[lR + e + e / m] = R[e];
+    }
     __syncthreads();
   }
   if (t > N) return;                                          // whole wave: no block barrier follows
