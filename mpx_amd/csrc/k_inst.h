@@ -103,6 +103,9 @@ __device__ void stage_terms(const Prob<T>& P, const Prm<T>& prm, const Iter<T>& 
   // load in flight at once: the dot-product loops below would otherwise wait
   // on global memory once per group of unrolled iterations.
   const bool eq_lds = p > 0 && p * (n + m) <= kEqStage;
+  // FP32 with n % 4 == 0 and x, d 16-byte aligned: the LDS vectors x and d are
+  // read four elements at a time
+  const bool vec4 = IsF32<T>::value && (n & 3) == 0 && ((reinterpret_cast<size_t>(x) | reinterpret_cast<size_t>(d)) & 15) == 0;
   T* Ds = scr + 256;
   T* Es = Ds + p * n;
   if (eq_lds) {
@@ -179,7 +182,19 @@ __device__ void stage_terms(const Prob<T>& P, const Prm<T>& prm, const Iter<T>& 
     T Qx = 0;
     if (L.Q >= 0) {
       const T* Qr = sm + L.Q + i * (n + 1);
-      for (int j = 0; j < n; ++j) Qx += Qr[j] * d[j];
+      if (vec4) {   // d four at a time, the same order of sums
+        const float4* d4 = reinterpret_cast<const float4*>(d);
+        for (int q = 0; q < (n >> 2); ++q) {
+          const float4 v = d4[q];
+          const T* r = Qr + 4 * q;
+          Qx += r[0] * v.x;
+          Qx += r[1] * v.y;
+          Qx += r[2] * v.z;
+          Qx += r[3] * v.w;
+        }
+      } else {
+        for (int j = 0; j < n; ++j) Qx += Qr[j] * d[j];
+      }
     } else {
       const T* Q = P.Q.at(b, t);
       for (int j = 0; j < n; ++j) Qx += Q[i * n + j] * d[j];
@@ -191,7 +206,20 @@ __device__ void stage_terms(const Prob<T>& P, const Prm<T>& prm, const Iter<T>& 
       const T* Wr = sm + L.W + i * (n + 1);
       const T* Br = sm + L.B + i * (m + 1);
       T ax = 0, wx = 0, bu = 0;
-      for (int j = 0; j < n; ++j) { ax += Ar[j] * x[j]; wx += Wr[j] * x[j]; }
+      if (vec4) {   // x four at a time, the same order of sums
+        const float4* x4 = reinterpret_cast<const float4*>(x);
+        for (int q = 0; q < (n >> 2); ++q) {
+          const float4 v = x4[q];
+          const T* ra = Ar + 4 * q;
+          const T* rw = Wr + 4 * q;
+          ax += ra[0] * v.x; wx += rw[0] * v.x;
+          ax += ra[1] * v.y; wx += rw[1] * v.y;
+          ax += ra[2] * v.z; wx += rw[2] * v.z;
+          ax += ra[3] * v.w; wx += rw[3] * v.w;
+        }
+      } else {
+        for (int j = 0; j < n; ++j) { ax += Ar[j] * x[j]; wx += Wr[j] * x[j]; }
+      }
       for (int j = 0; j < m; ++j) bu += Br[j] * u[j];
       const T th = dev_tanh(wx);
       if (LIN) sd[i] = T(1) - th * th;
@@ -272,10 +300,10 @@ __device__ void terminal_terms(const Prob<T>& P, const Ws<T>& ws, int b, const T
   wave_lds_sync();
 }
 
-// (four waves per SIMD: the stage loop is latency bound, and the register
-// budget of four waves keeps it from spilling)
+// (four waves per SIMD in FP32, three in FP64: the stage loop is latency
+// bound, and these are the largest register budgets that do not spill)
 template <typename T>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_linearize_inst(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3))) k_linearize_inst(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws) {
   const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int n = P.n, m = P.m, N = P.N, N1 = N + 1;

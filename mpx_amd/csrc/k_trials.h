@@ -79,6 +79,34 @@ __device__ __forceinline__ void row_dot2(const T* __restrict__ row, int L, const
   y1 = a1;
 }
 
+// row_dot2 for a padded LDS row and 16-byte aligned LDS vectors (FP32): the
+// vectors are read four elements at a time, the sums run in the same order.
+template <typename T>
+__device__ __forceinline__ void row_dot2_lds(const T* row, int L, const T* v0, const T* v1, T& y0, T& y1) {
+  if constexpr (IsF32<T>::value) {
+    T a0 = 0, a1 = 0;
+    const int L4 = L & ~3;
+    const float4* p0 = reinterpret_cast<const float4*>(v0);
+    const float4* p1 = reinterpret_cast<const float4*>(v1);
+    for (int q = 0; q < (L4 >> 2); ++q) {
+      const float4 x = p0[q], y = p1[q];
+      const T* w = row + 4 * q;
+      a0 += w[0] * x.x; a1 += w[0] * y.x;
+      a0 += w[1] * x.y; a1 += w[1] * y.y;
+      a0 += w[2] * x.z; a1 += w[2] * y.z;
+      a0 += w[3] * x.w; a1 += w[3] * y.w;
+    }
+    for (int j = L4; j < L; ++j) {
+      a0 += row[j] * v0[j];
+      a1 += row[j] * v1[j];
+    }
+    y0 = a0;
+    y1 = a1;
+  } else {
+    row_dot2<T, false>(row, L, v0, v1, y0, y1);
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_ls_setup(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws) {
   const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
@@ -274,9 +302,9 @@ __global__ void __launch_bounds__(512) k_trials(Prob<T> P, Prm<T> prm, Iter<T> i
       if (!term) {
         T za0, za1, zb0, zb1;
         if (SH) {
-          row_dot2<T, false>(sh + lA + i * (n + 1), n, x0, x1, za0, za1);
-          row_dot2<T, false>(sh + lW + i * (n + 1), n, x0, x1, w0[r], w1[r]);
-          row_dot2<T, false>(sh + lB + i * (m + 1), m, u0, u1, zb0, zb1);
+          row_dot2_lds<T>(sh + lA + i * (n + 1), n, x0, x1, za0, za1);
+          row_dot2_lds<T>(sh + lW + i * (n + 1), n, x0, x1, w0[r], w1[r]);
+          row_dot2_lds<T>(sh + lB + i * (m + 1), m, u0, u1, zb0, zb1);
           F0[r] = x0[i] + P.h * (za0 + zb0);
           F1[r] = x1[i] + P.h * (za1 + zb1);
         } else if (tanh_model) {
@@ -291,7 +319,7 @@ __global__ void __launch_bounds__(512) k_trials(Prob<T> P, Prm<T> prm, Iter<T> i
           F0[r] = za0 + zb0;
           F1[r] = za1 + zb1;
         }
-        if (SH) row_dot2<T, false>(sh + lQ + i * (n + 1), n, d0, x1, q0[r], q1[r]);
+        if (SH) row_dot2_lds<T>(sh + lQ + i * (n + 1), n, d0, x1, q0[r], q1[r]);
         else row_dot2<T, V4>(P.Q.at(b, t) + (size_t)i * n, n, d0, x1, q0[r], q1[r]);
         const size_t on = ox + n + i;
         xn0[r] = it.X[on];
@@ -313,7 +341,7 @@ __global__ void __launch_bounds__(512) k_trials(Prob<T> P, Prm<T> prm, Iter<T> i
   // control row i = lane: R (u - ur), box bounds
   T ru0 = 0, ru1 = 0, ue0 = 0, ue1 = 0, uu0 = 0, lo = 0, hi = 0;
   if (!term && lane < m) {
-    if (SH) row_dot2<T, false>(sh + lR + lane * (m + 1), m, e0, u1, ru0, ru1);
+    if (SH) row_dot2_lds<T>(sh + lR + lane * (m + 1), m, e0, u1, ru0, ru1);
     else row_dot2<T, V4>(P.R.at(b, t) + (size_t)lane * m, m, e0, u1, ru0, ru1);
     ue0 = e0[lane];
     ue1 = u1[lane];
