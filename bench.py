@@ -220,16 +220,18 @@ def main():
     if args.serve > 0:
         from mpx_amd import serve_bench
         sv = serve_bench.run(args.serve_rate, args.serve, seed=args.seed, max_wall=args.serve + 20.0)
-        serving = {k: sv[k] for k in ("rate", "duration", "served", "solves_per_s", "p50_ms", "p99_ms",
-                                      "backlog", "by_shape")}
+        serving = {k: sv[k] for k in ("rate", "duration", "arrivals", "served", "solves_per_s", "p50_ms",
+                                      "p99_ms", "p99_ms_all_arrivals", "backlog", "turned_away",
+                                      "device_bytes_held", "by_shape")}
         log(f"serving at {args.serve_rate:.0f}/s: p50 {sv['p50_ms']:.2f} ms, p99 {sv['p99_ms']:.2f} ms, "
             f"{sv['solves_per_s']:.0f} solves/s")
         for rate in [float(r) for r in args.serve_sweep.split(",") if r.strip()]:
             sw = serve_bench.run(rate, args.serve, seed=args.seed, max_wall=2 * args.serve)
-            serving_sweep.append({k: sw[k] for k in ("rate", "duration", "served", "solves_per_s", "p50_ms",
-                                                     "p99_ms", "backlog", "rejected_for_budget")})
+            serving_sweep.append({k: sw[k] for k in ("rate", "duration", "arrivals", "served", "solves_per_s",
+                                                     "p50_ms", "p99_ms", "p99_ms_all_arrivals", "backlog",
+                                                     "turned_away", "device_bytes_held")})
             log(f"serving at {rate:.0f}/s: {sw['solves_per_s']:.0f} solves/s, p50 {sw['p50_ms']:.2f} ms, "
-                f"p99 {sw['p99_ms']:.2f} ms, backlog {sw['backlog']}, turned away {sw['rejected_for_budget']}")
+                f"p99 {sw['p99_ms']:.2f} ms, backlog {sw['backlog']}, turned away {sw['turned_away']}")
 
     out = {
         "metric": "SQP iters/sec at B=4096 n=36 N=50 vs plain; p50/p99 MPC tick latency",

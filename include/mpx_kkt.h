@@ -106,6 +106,10 @@ typedef struct {
                              instance) instead of the stage-parallel one;
                              bit 3: linearisation parallel over stages (lower
                              latency for small batches; n, m, n_eq <= 64) */
+  double hess_floor;      /* pivot floor of the m x m Cholesky factorisations
+                             (H_t = R + B'S B, ...): pivot j is raised to
+                             hess_floor * |H_jj| when it falls below; keeps
+                             the step defined for indefinite stage Hessians */
 } mpxkkt_params;
 
 /* Per-instance outputs of one SQP iteration (each may be NULL). */

@@ -60,6 +60,7 @@ class Params(C.Structure):
         ("armijo_slack", C.c_double),
         ("max_trials", C.c_int32), ("limited_memory", C.c_int32), ("segments", C.c_int32),
         ("block_threads", C.c_int32), ("trial_group", C.c_int32), ("flags", C.c_int32),
+        ("hess_floor", C.c_double),
     ]
 
 

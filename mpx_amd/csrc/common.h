@@ -25,6 +25,7 @@ struct Prob {
 template <typename T>
 struct Prm {
   T rho0, rho_max, rho_c, mu_b, tau, beta, c1, slack;
+  T hfloor;  // Cholesky pivot floor, relative to the pivot's diagonal entry
   int max_trials, group, P;
   int flags;  // bit 0: form FP32 products on VALU instead of the matrix cores
 };
@@ -250,13 +251,30 @@ __device__ void gemm_lds(int M, int N, int K, const T* A, int lda, const T* B, i
   }
 }
 
-// In-place lower Cholesky of the m x m SPD matrix H (row-major, LDS).
-// A non-positive pivot leaves NaN, which then propagates (reported per instance).
+// Pivot j of a Cholesky factorisation: a (the pivot before its square root)
+// becomes max(|a|, floor |h0|) (h0 = the matrix's original diagonal entry), so
+// the factor is that of the matrix plus a non-negative diagonal; NaN stays NaN.
 template <typename T>
-__device__ void chol_lds(T* H, int m) {
+__device__ __forceinline__ T floored_pivot(T a, T h0, T floor) {
+  const T fl = floor * (h0 < T(0) ? -h0 : h0);
+  const T aa = a < T(0) ? -a : a;
+  return (aa >= fl || aa != aa) ? aa : fl;
+}
+
+// In-place lower Cholesky of the m x m symmetric matrix H (row-major, LDS),
+// pivots modified by floored_pivot with the original diagonal entry, which
+// thread 0 recovers as H_jj + sum_k L_jk^2.  A NaN pivot
+// propagates (reported per instance).
+template <typename T>
+__device__ void chol_lds(T* H, int m, T floor) {
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int j = 0; j < m; ++j) {
-    if (tid == 0) H[j * m + j] = dev_sqrt(H[j * m + j]);
+    if (tid == 0) {
+      const T a = H[j * m + j];
+      T h0 = a;
+      for (int k = 0; k < j; ++k) h0 += H[j * m + k] * H[j * m + k];
+      H[j * m + j] = dev_sqrt(floored_pivot(a, h0, floor));
+    }
     lds_sync();
     const T d = H[j * m + j];
     for (int i = j + 1 + tid; i < m; i += nt) H[i * m + j] /= d;
@@ -424,28 +442,31 @@ __device__ inline void wave_fence() {
 
 
 // Wave 0 alone, no block barrier inside (other waves return at once): for the
-// SPD m x m matrix H (LDS, row-major, m <= MM <= 64) the upper Cholesky factor
-// U (H = U'U) and Y = U^{-1} (upper triangular, zero below), written to Y.
+// symmetric m x m matrix H (LDS, row-major, m <= MM <= 64) the upper Cholesky
+// factor U (H = U'U, pivots floored as in chol_lds) and Y = U^{-1} (upper
+// triangular, zero below), written to Y.
 // Lane k keeps column k in registers, padded to MM with the identity so that
 // no step needs a bound: elimination step j takes row j of U from the lanes by
 // readlane and updates the trailing columns in place (lane k's h[j] becomes
 // U[j][k]), and the back substitution solves U y = e_k reading U[i][q] from
 // lane q the same way; LDS is touched only to load H and store Y.
 template <typename T, int MM>
-__device__ void chol_upper_inv_wave(const T* H, T* Y, int m) {
+__device__ void chol_upper_inv_wave(const T* H, T* Y, int m, T floor) {
   if (threadIdx.x >= 64) return;
   const int k = threadIdx.x;
   const bool own = k < m;
   const int kc = own ? k : 0;
   T h[MM];
+  T hkk = 0;   // the lane's original diagonal entry (pivot floors)
 #pragma unroll
   for (int i = 0; i < MM; ++i) {
     const T v = H[(i < m ? i : 0) * m + kc];
     h[i] = (own && i < m) ? v : T(i == k ? 1 : 0);
+    if (i == k) hkk = h[i];
   }
 #pragma unroll
   for (int j = 0; j < MM; ++j) {
-    const T d = dev_sqrt(rlane(h[j], j));
+    const T d = dev_sqrt(floored_pivot(rlane(h[j], j), rlane(hkk, j), floor));
     const T lk = k > j ? h[j] / d : T(0);
     h[j] = k == j ? d : lk;
 #pragma unroll

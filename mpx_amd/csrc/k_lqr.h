@@ -125,13 +125,13 @@ struct IsF32<float> { static constexpr bool value = true; };
 // triangles, block-wide in LDS (the register version spills at those template
 // sizes inside the Riccati kernels).
 template <typename T, int MMAX>
-__device__ void spd_inverse(T* H, T* tmp, int m) {
+__device__ void spd_inverse(T* H, T* tmp, int m, T floor) {
   if constexpr (MMAX == 32) {
-    chol_upper_inv_wave<T, MMAX>(H, tmp, m);
+    chol_upper_inv_wave<T, MMAX>(H, tmp, m, floor);
     lds_sync();
     gemm_lds<T, false, true, 0>(m, m, m, tmp, m, tmp, m, H, m);
   } else {
-    chol_lds(H, m);
+    chol_lds(H, m, floor);
     for (int e = threadIdx.x; e < m * m; e += blockDim.x) tmp[e] = (e / m == e % m) ? T(1) : T(0);
     lds_sync();
     trsm_lower(H, m, tmp, m, m);
@@ -208,7 +208,7 @@ __device__ void riccati_step(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& w
   }
   lds_sync();
   // C: H <- H^{-1}
-  spd_inverse<T, MMAX>(L.H, L.Kb, m);
+  spd_inverse<T, MMAX>(L.H, L.Kb, m, prm.hfloor);
   lds_sync();
   // D: [K | k] = H^{-1} [G | g]
   if constexpr (IsF32<T>::value) {

@@ -24,9 +24,11 @@
 // s_t = q + A'(S c + s) - G'k, F_t = A - BK, f_t = c - Bk) with the vectors
 // riding in column n.  The corner of S~ is reset to 0 every step.
 //
-// The m x m inverse is an in-place Gauss-Jordan sweep in one tile (H is SPD:
-// no pivoting), with ds_bpermute broadcasts of the pivot row and column; the
-// symmetric tiles are mirrored through a 1 KB LDS transpose.  The forward pass
+// The m x m inverse is a Cholesky factorisation with the forward substitution
+// of L^{-1} in the same sweep over one tile (broadcasts of the pivot column by
+// DPP and of the pivot row by ds_bpermute), then H^{-1} = L^{-T} L^{-1} on the
+// matrix cores; the symmetric value tiles are mirrored through a 1 KB LDS
+// transpose.  The forward pass
 // (k_fwd_wave) runs on the matrix cores too, with x~ as column 0 of a
 // B-operand tile: every product's result tile is again in that layout, so
 // x~_{t+1} = A~ x~ + B~ du and du = -K~ x~ chain from registers and
@@ -105,39 +107,64 @@ __device__ __forceinline__ float row_bcast(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x150 + J, 0xf, 0xf, false));
 }
 
-// Pivot J of gj_inverse.  Lane (g, c) holds H[4 g + v][c] in register v: the
-// pivot row H[J][c] comes from lane 16 (J / 4) + c, the pivot column
-// H[4 g + v][J] from lane J of the lane's own row of 16 (a DPP broadcast), and
-// H[J][J] from one lane (a scalar read).
+// Step J of chol_inverse.  Lane (g, c) holds H[4 g + v][c] and Z[4 g + v][c]
+// in register v.  The pivot a = H[J][J] comes from one lane (a scalar read)
+// and is floored at floor |H0[J][J]| (H0: the tile before the factorisation);
+// column J of L = H[:, J] / sqrt(a) below the diagonal comes from lane J of
+// the lane's own row of 16 (a DPP broadcast) and L[c][J] from row J (lane
+// 16 (J / 4) + c; equal to H[c][J] by symmetry).  The trailing matrix takes
+// H[r][c] -= L[r][J] L[c][J] (r, c > J), column J becomes L's column J (zero
+// above the diagonal), and the forward substitution of L Z = I advances by
+// the same column: Z[J][:] /= L[J][J], Z[r][:] -= L[r][J] Z[J][:] (r > J).
 template <int J>
-__device__ __forceinline__ void gj_pivot(f4& h, int g, int c) {
+__device__ __forceinline__ void chol_step(f4& h, f4& z, const f4& h0, float floor, int g, int c) {
   constexpr int jg = J >> 2, jv = J & 3;
-  const float rowj = __shfl(h[jv], jg * 16 + c);                  // H[J][c]
-  const float a = rlane(h[jv], jg * 16 + J);                      // H[J][J]
-  f4 colj;
-#pragma unroll
-  for (int v = 0; v < 4; ++v) colj[v] = row_bcast<J>(h[v]);       // H[4g+v][J]
-  const float rnew = (c == J ? 1.f : rowj) / a;
+  const float a = floored_pivot(rlane(h[jv], jg * 16 + J), rlane(h0[jv], jg * 16 + J), floor);
+  const float d = sqrtf(a);
+  const float id = 1.f / d;
+  const float lc = __shfl(h[jv], jg * 16 + c) * id;              // L[c][J] (c > J)
+  const float zj = __shfl(z[jv], jg * 16 + c) * id;              // Z[J][c] after the step
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    const float hv = (c == J) ? 0.f : h[v];
-    h[v] = (4 * g + v == J) ? rnew : hv - colj[v] * rnew;
+    const int r = 4 * g + v;
+    const float lr = row_bcast<J>(h[v]) * id;                     // L[r][J] (r > J)
+    float hv = h[v];
+    if (r > J && c > J) hv -= lr * lc;
+    if (c == J) hv = r > J ? lr : (r == J ? d : 0.f);
+    h[v] = hv;
+    float zv = z[v];
+    if (r == J) zv = zj;
+    else if (r > J) zv -= lr * zj;
+    z[v] = zv;
   }
 }
 
 template <int J>
-__device__ __forceinline__ void gj_pivots(f4& h, int g, int c) {
+__device__ __forceinline__ void chol_steps(f4& h, f4& z, const f4& h0, float floor, int g, int c) {
   if constexpr (J < 16) {
-    gj_pivot<J>(h, g, c);
-    gj_pivots<J + 1>(h, g, c);
+    chol_step<J>(h, z, h0, floor, g, c);
+    chol_steps<J + 1>(h, z, h0, floor, g, c);
   }
 }
 
-// In-place inverse of the SPD tile H (m x m in the top-left corner, identity
-// elsewhere): Gauss-Jordan without pivoting over all 16 pivots, straight-line
-// code that the scheduler can place among independent matrix products.  Past
-// the m x m block H is the identity, where a pivot changes nothing.
-__device__ __forceinline__ void gj_inverse(f4& h, int g, int c) { gj_pivots<0>(h, g, c); }
+// H <- H^{-1} for the symmetric tile H (m x m in the top-left corner, the
+// identity elsewhere): Cholesky H = L L' with floored pivots and Z = L^{-1}
+// in one sweep of 16 steps (straight-line code that the scheduler can place
+// among independent matrix products; past the m x m block every step is the
+// identity), then H^{-1} = Z' Z on the matrix cores.  Z' Z forms entries
+// (i, j) and (j, i) from the same products, so the inverse is symmetric and
+// serves as its own transpose (no mirror through LDS).
+__device__ __forceinline__ void chol_inverse(f4& h, float floor, int g, int c) {
+  const f4 h0 = h;
+  f4 z;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) z[v] = (4 * g + v == c) ? 1.f : 0.f;
+  chol_steps<0>(h, z, h0, floor, g, c);
+  f4 acc = zero4();
+#pragma unroll
+  for (int s = 0; s < 4; ++s) acc = mma(z[s], z[s], acc);
+  h = acc;
+}
 
 // Loop-invariant lane offsets (bytes) of one instance's shapes.
 template <int NT>
@@ -397,8 +424,8 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
 #pragma unroll
         for (int s = 0; s < 4; ++s) G[j] = mma(cur.E[s], cur.D[j][s], G[j]);
     }
-    // H <- H^{-1} (symmetrised below, once the products of this stage are in)
-    gj_inverse(H, g, c);
+    // H <- H^{-1} (Cholesky; exactly symmetric)
+    chol_inverse(H, prm.hfloor, g, c);
     f4 Pm[NT][NT];
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
@@ -438,12 +465,6 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
       }
     }
 
-    // H^{-1} symmetrised so that it serves as its own transpose
-    {
-      const f4 Ht = tile_transpose(H, tbuf, g, c);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) H[v] = 0.5f * (H[v] + Ht[v]);
-    }
     // K~ = H^{-1} G~
     f4 K[NT];
 #pragma unroll

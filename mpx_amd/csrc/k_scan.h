@@ -104,7 +104,7 @@ __device__ void leaf_combine_up(const Prob<T>& P, const Prm<T>& prm, const Ws<T>
     }
     for (int i = tid; i < m; i += nt) Z[i * ld + n] = sr[i];
     __syncthreads();
-    chol_lds(LR, m);
+    chol_lds(LR, m, prm.hfloor);
     trsm_lower(LR, m, Z, n + 1, ld);
     __syncthreads();
     trsm_upper_t(LR, m, Z, n + 1, ld);
@@ -185,7 +185,7 @@ __device__ void leaf_combine_up(const Prob<T>& P, const Prm<T>& prm, const Ws<T>
       G[i * ld + n] = v;
     }
     __syncthreads();
-    chol_lds(H, m);
+    chol_lds(H, m, prm.hfloor);
     // P2 = A2 B ; Y2 = L_H^{-1} P2' (into Z, m x n) ; Y3 = L_H^{-1} JB' (into LR area? use T2 temporarily)
     gemm_lds<T, false, false, 0>(n, m, n, gA, n, sB, m, P2, m);
     __syncthreads();

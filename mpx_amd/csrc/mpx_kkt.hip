@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -70,8 +71,16 @@ int effective_segments(const mpxkkt_problem* pr, const mpxkkt_params* prm) {
   return P;
 }
 
+// The register-resident wave LQR kernels cover FP32 on the matrix cores with
+// n + 1 <= 48, m <= 16, n_eq <= 16 and one segment (flags bits 0 and 1 turn
+// them off); they never form the closed-loop maps F_t = A_t - B_t K_t.
+bool wave_path(const mpxkkt_problem* pr, int flags, int P) {
+  return pr->dtype == MPXKKT_F32 && (flags & 3) == 0 && P <= 1 && pr->n + 1 <= 48 && pr->m <= 16 &&
+         pr->n_eq <= 16;
+}
+
 template <typename T>
-size_t carve(const mpxkkt_problem* pr, int P, Ws<T>* ws, char* base) {
+size_t carve(const mpxkkt_problem* pr, int P, bool wave, Ws<T>* ws, char* base) {
   const size_t B = pr->batch, n = pr->n, m = pr->m, N = pr->horizon, N1 = N + 1;
   size_t off = 0;
   auto take = [&](size_t cnt) -> T* {
@@ -91,8 +100,8 @@ size_t carve(const mpxkkt_problem* pr, int P, Ws<T>* ws, char* base) {
   ws->s = take(B * N1 * n);
   ws->K = take(B * N * m * n);
   ws->k = take(B * N * m);
-  ws->F = take(B * N * n * n);
-  ws->f = take(B * N * n);
+  ws->F = take(wave ? 0 : B * N * n * n);
+  ws->f = take(wave ? 0 : B * N * n);
   ws->dx = take(B * N1 * n);
   ws->du = take(B * N * m);
   ws->dv = take(B * N1 * n);
@@ -111,12 +120,13 @@ size_t carve(const mpxkkt_problem* pr, int P, Ws<T>* ws, char* base) {
 
 size_t workspace_bytes(const mpxkkt_problem* pr, const mpxkkt_params* prm) {
   const int P = effective_segments(pr, prm);
+  const bool wave = wave_path(pr, prm->flags, P);
   if (pr->dtype == MPXKKT_F64) {
     Ws<double> w;
-    return carve<double>(pr, P, &w, nullptr);
+    return carve<double>(pr, P, wave, &w, nullptr);
   }
   Ws<float> w;
-  return carve<float>(pr, P, &w, nullptr);
+  return carve<float>(pr, P, wave, &w, nullptr);
 }
 
 mpxkkt_status check_view(const mpxkkt_view& v, const char* name, bool needed) {
@@ -162,17 +172,23 @@ mpxkkt_status check_iterate(const mpxkkt_problem* pr, const mpxkkt_iterate* it) 
   return MPXKKT_OK;
 }
 
-int g_max_lds = -1;
+// LDS bytes per block of each device, queried once per device (any thread).
+constexpr int kMaxDevices = 64;
+std::mutex g_lds_mu;
+int g_max_lds[kMaxDevices] = {0};
 
 mpxkkt_status lds_limit(int* out) {
-  if (g_max_lds < 0) {
-    int dev = 0, v = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return fail(MPXKKT_ERR_DEVICE, "hipGetDevice failed");
-    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return fail(MPXKKT_ERR_DEVICE, "hipGetDevice failed");
+  std::lock_guard<std::mutex> lk(g_lds_mu);
+  if (g_max_lds[dev] <= 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
       return fail(MPXKKT_ERR_DEVICE, "cannot query LDS size");
-    g_max_lds = v;
+    g_max_lds[dev] = v;
   }
-  *out = g_max_lds;
+  *out = g_max_lds[dev];
   return MPXKKT_OK;
 }
 
@@ -204,6 +220,7 @@ Prm<T> make_prm(const mpxkkt_params* prm, int P) {
   r.rho0 = (T)prm->rho0; r.rho_max = (T)prm->rho_max; r.rho_c = (T)prm->rho_c;
   r.mu_b = (T)prm->mu_b; r.tau = (T)prm->tau; r.beta = (T)prm->beta; r.c1 = (T)prm->c1;
   r.slack = (T)prm->armijo_slack;
+  r.hfloor = (T)prm->hess_floor;
   r.max_trials = std::max(1, std::min(prm->max_trials, MAX_TRIALS));
   int g = prm->trial_group <= 0 ? r.max_trials : prm->trial_group;
   if (prm->limited_memory) g = 1;
@@ -357,15 +374,9 @@ mpxkkt_status run_linearize(hipStream_t st, const mpxkkt_problem* pr, const Prob
   return launch_check("k_linearize");
 }
 
-// The register-resident wave kernel covers FP32 on the matrix cores with
-// n + 1 <= 48, m <= 16, p <= 16 and one segment (flags bit 1 turns it off).
 template <typename T>
 bool use_wave_lqr(const mpxkkt_problem* pr, const Prm<T>& R) {
-  if constexpr (!IsF32<T>::value) {
-    return false;
-  } else {
-    return (R.flags & 3) == 0 && R.P <= 1 && pr->n + 1 <= 48 && pr->m <= 16 && pr->n_eq <= 16;
-  }
+  return IsF32<T>::value && wave_path(pr, R.flags, R.P);
 }
 
 template <typename T>
@@ -543,35 +554,41 @@ mpxkkt_status run_linesearch(hipStream_t st, const mpxkkt_problem* pr, const Pro
   return launch_check("k_linesearch");
 }
 
-// phase-timing record (host-side; events on the caller's stream)
+// phase-timing record (host-side; events on the caller's stream), shared by
+// the threads of the process under g_prof_mu
 struct PhaseEvents {
   hipEvent_t ev[MPXKKT_PHASES + 1];
 };
+std::mutex g_prof_mu;
 bool g_prof_on = false;
 std::vector<PhaseEvents> g_prof;
 
-void prof_mark(hipStream_t st, PhaseEvents* pe, int k) {
+void prof_mark(hipStream_t st, const PhaseEvents* pe, int k) {
   if (pe) (void)hipEventRecord(pe->ev[k], st);
 }
 
 template <typename T>
 mpxkkt_status sqp_iterate_t(hipStream_t st, const mpxkkt_problem* pr, const mpxkkt_params* prm,
-                            mpxkkt_iterate* it, mpxkkt_stats* so, void* wsp) {
+                            mpxkkt_iterate* it, mpxkkt_stats* so, void* wsp, bool profile_ok = true) {
   const int Pseg = effective_segments(pr, prm);
   Ws<T> W;
-  carve<T>(pr, Pseg, &W, static_cast<char*>(wsp));
+  carve<T>(pr, Pseg, wave_path(pr, prm->flags, Pseg), &W, static_cast<char*>(wsp));
   const Prob<T> P = make_prob<T>(pr);
   const Prm<T> R = make_prm<T>(prm, Pseg);
   const Iter<T> I = make_iter<T>(it);
   const LqrIn<T> in = make_lqr_in<T>(pr, W);
   mpxkkt_status s;
-  PhaseEvents* pe = nullptr;
-  if (g_prof_on) {
-    PhaseEvents e;
+  PhaseEvents ev_store;
+  const PhaseEvents* pe = nullptr;
+  bool prof = false;
+  {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    prof = g_prof_on && profile_ok;
+  }
+  if (prof) {
     for (int k = 0; k <= MPXKKT_PHASES; ++k)
-      if (hipEventCreate(&e.ev[k]) != hipSuccess) return fail(MPXKKT_ERR_DEVICE, "hipEventCreate failed");
-    g_prof.push_back(e);
-    pe = &g_prof.back();
+      if (hipEventCreate(&ev_store.ev[k]) != hipSuccess) return fail(MPXKKT_ERR_DEVICE, "hipEventCreate failed");
+    pe = &ev_store;
   }
   prof_mark(st, pe, 0);
   if ((s = run_linearize<T>(st, pr, P, R, I, W)) != MPXKKT_OK) return s;
@@ -582,6 +599,10 @@ mpxkkt_status sqp_iterate_t(hipStream_t st, const mpxkkt_problem* pr, const mpxk
   prof_mark(st, pe, 3);
   s = run_linesearch<T>(st, pr, P, R, I, W, in, so);
   prof_mark(st, pe, 4);
+  if (prof) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof.push_back(ev_store);
+  }
   return s;
 }
 
@@ -589,7 +610,8 @@ template <typename T>
 mpxkkt_status kkt_t(hipStream_t st, const mpxkkt_problem* pr, const mpxkkt_params* prm, const mpxkkt_iterate* it,
                     void* out, void* wsp) {
   Ws<T> W;
-  carve<T>(pr, effective_segments(pr, prm), &W, static_cast<char*>(wsp));
+  const int Pseg = effective_segments(pr, prm);
+  carve<T>(pr, Pseg, wave_path(pr, prm->flags, Pseg), &W, static_cast<char*>(wsp));
   const Prob<T> P = make_prob<T>(pr);
   const Prm<T> R = make_prm<T>(prm, 1);
   const Iter<T> I = make_iter<T>(it);
@@ -624,6 +646,7 @@ mpxkkt_status mpxkkt_default_params(mpxkkt_params* out) {
   out->segments = 1;
   out->block_threads = 0;
   out->trial_group = 0;
+  out->hess_floor = 1e-8;
   return MPXKKT_OK;
 }
 
@@ -645,7 +668,7 @@ mpxkkt_status mpxkkt_sqp_iterate(void* stream, const mpxkkt_problem* prob, const
   if (prm->max_trials < 1 || prm->max_trials > MAX_TRIALS)
     return fail(MPXKKT_ERR_ARG, "max_trials must be in [1, 16]");
   if (!(prm->beta > 0.0 && prm->beta < 1.0) || !(prm->tau > 0.0 && prm->tau < 1.0) || !(prm->mu_b > 0.0) ||
-      !(prm->rho_c >= 0.0))
+      !(prm->rho_c >= 0.0) || !(prm->hess_floor >= 0.0 && prm->hess_floor < 1.0))
     return fail(MPXKKT_ERR_ARG, "line-search / barrier parameters out of range");
   const size_t need = workspace_bytes(prob, prm);
   if (!workspace || workspace_bytes_ < need)
@@ -706,6 +729,7 @@ mpxkkt_status mpxkkt_shift_warmstart(void* stream, const mpxkkt_problem* prob, i
 }
 
 mpxkkt_status mpxkkt_profile_enable(int32_t on) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
   g_prof_on = on != 0;
   return MPXKKT_OK;
 }
@@ -714,6 +738,7 @@ mpxkkt_status mpxkkt_profile_collect(double ms_per_phase[MPXKKT_PHASES], int32_t
   if (!ms_per_phase || !calls) return fail(MPXKKT_ERR_ARG, "null output");
   for (int k = 0; k < MPXKKT_PHASES; ++k) ms_per_phase[k] = 0.0;
   mpxkkt_status s = MPXKKT_OK;
+  std::lock_guard<std::mutex> lk(g_prof_mu);
   for (auto& e : g_prof) {
     if (hipEventSynchronize(e.ev[MPXKKT_PHASES]) != hipSuccess)
       s = fail(MPXKKT_ERR_DEVICE, "hipEventSynchronize failed");

@@ -47,7 +47,7 @@ def run(rate, duration, seed=0, life=200, budget_gb=64.0, max_group=256, n_strea
     n_arr = rng.poisson(rate * duration)
     t_rel = np.sort(rng.uniform(0.0, duration, size=n_arr))
     idle, state = {k: [] for k in range(len(shapes))}, {}
-    latencies, by_shape, waits = [], {}, 0
+    latencies, by_shape, waits, turned_by_shape = [], {}, 0, {}
     t0 = time.perf_counter()
     i = 0
     serial = 0
@@ -62,6 +62,7 @@ def run(rate, duration, seed=0, life=200, budget_gb=64.0, max_group=256, n_strea
             else:
                 if not server.room_for(protos[k]):     # turned away before any set-up work
                     waits += 1
+                    turned_by_shape[shapes[k]] = turned_by_shape.get(shapes[k], 0) + 1
                     i += 1
                     continue
                 n, m, N = shapes[k]
@@ -81,7 +82,7 @@ def run(rate, duration, seed=0, life=200, budget_gb=64.0, max_group=256, n_strea
             latencies.append(lat)
             s = state[r.cid]
             by_shape.setdefault(s["shape"], []).append(lat)
-            s["x"] = _plant_step(s["inst"], s["x"], np.clip(np.nan_to_num(r.u0), -1.0, 1.0))
+            s["x"] = _plant_step(s["inst"], s["x"], np.clip(r.u0, -1.0, 1.0))
             s["ticks"] += 1
             if s["ticks"] >= life:
                 server.close(r.cid)
@@ -92,12 +93,21 @@ def run(rate, duration, seed=0, life=200, budget_gb=64.0, max_group=256, n_strea
     torch.cuda.synchronize()     # launches left in flight at the wall-clock cap finish before the buffers go
     lat = np.array(latencies) * 1e3
     backlog = (n_arr - i) + len(server.pending) + sum(len(c[1]) for ln in server.in_flight for c in ln.cohorts)
-    out = dict(rate=rate, duration=duration, served=len(lat), solves_per_s=len(lat) / wall, backlog=backlog,
+    # p99 over every arrival: an arrival turned away (or left in the backlog)
+    # counts as a miss of infinite latency
+    every = np.concatenate([lat, np.full(waits + backlog, np.inf)])
+    p99_all = float(np.percentile(every, 99)) if len(every) else None
+    out = dict(rate=rate, duration=duration, arrivals=int(n_arr), served=len(lat),
+               solves_per_s=len(lat) / wall, backlog=backlog,
                p50_ms=float(np.percentile(lat, 50)) if len(lat) else None,
                p99_ms=float(np.percentile(lat, 99)) if len(lat) else None,
-               rejected_for_budget=waits, controllers_opened=serial,
+               p99_ms_all_arrivals=p99_all if p99_all is None or np.isfinite(p99_all) else "inf",
+               turned_away=waits, turned_away_budget=server.turned_away["budget"],
+               turned_away_slots=server.turned_away["slots"], controllers_opened=serial,
+               device_bytes_held=server.used,
                by_shape={f"{k[0]}x{k[1]}x{k[2]}": dict(n=len(v), p50_ms=float(np.percentile(v, 50) * 1e3),
-                                                      p99_ms=float(np.percentile(v, 99) * 1e3))
+                                                      p99_ms=float(np.percentile(v, 99) * 1e3),
+                                                      turned_away=turned_by_shape.get(k, 0))
                          for k, v in by_shape.items()})
     if log:
         log(out)

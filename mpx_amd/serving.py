@@ -19,10 +19,18 @@ on the device while new requests keep arriving.  A launch gathers its
 controllers' slots into the staging prefix sorted by iteration count
 (descending), runs SQP iteration j on the prefix of requests that need more
 than j iterations, and as soon as a cohort has had all its iterations it
-copies their first controls to the host, applies the warm-start shift with the
-per-controller NaN fallback of update_and_extract
-(mpx/utils/mpc_wrapper.py:79-96) on that cohort's slice, scatters it back into
-the slots and records an event.  A controller's result does not depend on
+applies the warm-start shift with the per-controller NaN fallback of
+update_and_extract (mpx/utils/mpc_wrapper.py:79-96) on that cohort's slice
+(a failed controller restarts from x0, u_ref and zero multipliers and its
+tick returns the next control of the plan it started from, U0[1]), copies the
+first controls to the host, scatters the slice back into the slots and
+records an event.
+
+Memory: a bucket's slot arrays start small and double when full, each growth
+admitted against the HBM budget; the budget counts every byte the engine
+holds on the device -- slot arrays at their capacity, and per staging area
+its batch arrays and the library's workspace (``mpxkkt_workspace_size``).
+An arrival that would exceed the budget is turned away.  A controller's result does not depend on
 which other requests share its launch: every launch shape is fixed per
 bucket, and the kernels never mix instances.
 """
@@ -63,7 +71,6 @@ class _Controller:
     cid: int
     bucket: "_Bucket"
     slot: int
-    nbytes: int
     ticks: int = 0
     busy: bool = False
 
@@ -76,20 +83,27 @@ class _Launch:
     cohorts: list = field(default_factory=list)
 
 
+def _tensor_bytes(tensors):
+    return sum(t.numel() * t.element_size() for t in tensors if t is not None)
+
+
 class _Bucket:
-    """Resident slot arrays for one shape, plus staging batches, each with its
-    own stream and workspace."""
+    """Resident slot arrays for one shape (growable), plus staging batches,
+    each with its own stream and workspace."""
 
     def __init__(self, key, proto, capacity, max_group, dtype, device, n_streams, params):
         self.key = key
+        self.proto = proto
         self.capacity = capacity
         self.free = list(range(capacity - 1, -1, -1))
         self.slots = DeviceBatch.from_instances([proto] * capacity, dtype=dtype, device=device)
         self.state = self.slots.iterate_from(*[None] * 5)
+        self.slot_bytes = (_tensor_bytes(self.slots.t.values()) + _tensor_bytes(self.state.values())) // capacity
         self.max_group = max_group
         self.params = params
         self.stage, self.stage_it, self.ws, self.streams, self.io = [], [], [], [], []
         self.idle = []
+        self.fixed_bytes = 0
         n, m = proto["n"], proto["m"]
         for si in range(n_streams):
             st = DeviceBatch.from_instances([proto] * max_group, dtype=dtype, device=device)
@@ -97,6 +111,8 @@ class _Bucket:
             self.stage_it.append(st.iterate_from(*[None] * 5))
             self.ws.append(torch.empty(SQPSolver.workspace_bytes(st, self.params), dtype=torch.uint8,
                                        device=device))
+            self.fixed_bytes += (_tensor_bytes(st.t.values()) + _tensor_bytes(self.stage_it[-1].values())
+                                 + self.ws[-1].numel())
             self.streams.append(torch.cuda.Stream(device=device))
             # host <-> device transfer buffers of the area, allocated once: a
             # pinned allocation inside the serving loop can wait for the device
@@ -106,9 +122,33 @@ class _Bucket:
                 u0_h=torch.empty((max_group, m), dtype=dtype, pin_memory=True),
                 reset_h=torch.empty(max_group, dtype=torch.int32, pin_memory=True),
                 slots=torch.empty(max_group, dtype=torch.long, device=device),
-                reset=torch.empty(max_group, dtype=torch.int32, device=device)))
+                reset=torch.empty(max_group, dtype=torch.int32, device=device),
+                u0=torch.empty((max_group, m), dtype=dtype, device=device),
+                u_next=torch.empty((max_group, m), dtype=dtype, device=device),
+                u_reset=torch.empty((max_group, m), dtype=dtype, device=device)))
             self.idle.append(si)
         self.solvers = {}
+
+    def grow(self, capacity):
+        """Move the slot arrays to a larger capacity.  The copy runs on the
+        current stream after the work already queued on the bucket's staging
+        streams (whose launches gather from and scatter to the old arrays);
+        later launches wait for the current stream before they gather."""
+        cur = torch.cuda.current_stream(self.slots.device)
+        for s in self.streams:
+            cur.wait_stream(s)
+        new = DeviceBatch.from_instances([self.proto] * capacity, dtype=self.slots.dtype,
+                                         device=self.slots.device)
+        st = new.iterate_from(*[None] * 5)
+        old = self.capacity
+        for k, v in self.slots.t.items():
+            if v is not None:
+                new.t[k][:old].copy_(v)
+        for k, v in self.state.items():
+            st[k][:old].copy_(v)
+        self.slots, self.state = new, st
+        self.free = list(range(capacity - 1, old - 1, -1)) + self.free
+        self.capacity = capacity
 
     def solver(self, k, si):
         if (k, si) not in self.solvers:
@@ -137,6 +177,24 @@ class _Bucket:
             self.state[k][slot].zero_()
 
 
+def _workspace_bytes_for(proto, k, dtype, params):
+    """mpxkkt_workspace_size for a batch of k instances of proto's shape (a
+    host-side query on a problem struct with placeholder pointers)."""
+    import ctypes as C
+    n, m, N, p = proto["n"], proto["m"], proto["N"], proto["p"]
+    pr = L.Problem()
+    pr.dtype = L.F64 if dtype == torch.float64 else L.F32
+    pr.model = L.MODEL_TANH if proto["model"] == "tanh" else L.MODEL_LTV
+    pr.batch, pr.n, pr.m, pr.horizon, pr.n_eq = k, n, m, N, p
+    pr.has_box = 1 if proto.get("ulo") is not None else 0
+    for name, typ in L.Problem._fields_:
+        if typ is L.View:
+            getattr(pr, name).ptr = 256     # never dereferenced by the size query
+    nbytes = C.c_size_t(0)
+    L.check(L.load().mpxkkt_workspace_size(C.byref(pr), C.byref(params), C.byref(nbytes)))
+    return int(nbytes.value)
+
+
 def _bucket_params(proto, dtype, device, tune):
     """Launch parameters of a bucket and its time per SQP iteration (ms, or
     None): ``tune`` as given (a dict), or, with ``tune='auto'``, the
@@ -154,14 +212,19 @@ def _bucket_params(proto, dtype, device, tune):
 
 
 class MPCServer:
-    """Admits controllers by HBM budget and serves their ticks with continuous batching."""
+    """Admits controllers by HBM budget and serves their ticks with continuous
+    batching.  ``slots_per_bucket``: a new bucket's initial slot capacity
+    (doubled whenever it is full, up to ``max_slots_per_bucket``, each growth
+    admitted against the budget)."""
 
-    def __init__(self, hbm_budget_bytes, max_group=256, slots_per_bucket=512, n_streams=1,
-                 dtype=torch.float32, device="cuda", shift=1, tune=None, n_streams_slow=2, slow_ms=2.0):
+    def __init__(self, hbm_budget_bytes, max_group=256, slots_per_bucket=64, n_streams=1,
+                 dtype=torch.float32, device="cuda", shift=1, tune=None, n_streams_slow=2, slow_ms=2.0,
+                 max_slots_per_bucket=1 << 16):
         self.budget = int(hbm_budget_bytes)
-        self.used = 0
         self.max_group = max_group
         self.slots_per_bucket = slots_per_bucket
+        self.max_slots = max_slots_per_bucket
+        self.turned_away = dict(budget=0, slots=0)
         # staging areas (each with its stream) per bucket: n_streams, or
         # n_streams_slow for a bucket whose SQP iteration takes slow_ms or more
         # (known with tune='auto'): a fast bucket serves a launch in well under
@@ -174,6 +237,7 @@ class MPCServer:
         self.shift = shift
         self.tune = tune
         self.buckets = {}
+        self._probe, self._params, self._bucket_ms = {}, {}, {}
         self.controllers = {}
         self.pending = collections.deque()
         self.in_flight = []
@@ -181,61 +245,111 @@ class MPCServer:
         self._rid = 0
 
     # --- admission -------------------------------------------------------
-    def controller_bytes(self, inst):
-        """Resident HBM bytes of one controller: problem data + warm start + its
-        share of the bucket's staging and workspace."""
-        n, m, N, p = inst["n"], inst["m"], inst["N"], inst["p"]
-        es = 4 if self.dtype == torch.float32 else 8
-        prob = 3 * n * n + n * m + m * m + (N + 1) * n + N * m + N * p * (n + m + 1) + 3 * n + 2 * m
-        state = 2 * (N + 1) * n + N * m + N * p + 2 * N * m
-        work = N * (2 * n * n + 2 * m * n + 6 * n + 4 * m) + (N + 1) * (n * n + 8 * n)
-        return es * (2 * (prob + state) + work)
+    @property
+    def used(self):
+        """Device bytes the engine holds: slot arrays at their capacity plus
+        every staging area's arrays and workspace."""
+        return sum(b.fixed_bytes + b.capacity * b.slot_bytes for b in self.buckets.values())
+
+    def _growth(self, b):
+        """Capacity a full bucket would grow to, or None at the slot limit."""
+        cap = min(2 * b.capacity, self.max_slots)
+        return cap if cap > b.capacity else None
+
+    def bytes_to_admit(self, inst):
+        """Device bytes that admitting a controller of this shape now would add
+        (0 when its bucket has a free slot; None when the bucket is at its slot
+        limit).  A bucket not yet built is measured by building its staging
+        batch's workspace query on the host (no device memory)."""
+        b = self.buckets.get(shape_key(inst))
+        if b is None:
+            return self._new_bucket_bytes(inst)
+        if b.free:
+            return 0
+        cap = self._growth(b)
+        return None if cap is None else (cap - b.capacity) * b.slot_bytes
+
+    def _new_bucket_bytes(self, inst):
+        key = shape_key(inst)
+        if key not in self._probe:
+            n, m, N, p = inst["n"], inst["m"], inst["N"], inst["p"]
+            box = inst.get("ulo") is not None
+            es = 4 if self.dtype == torch.float32 else 8
+            nw = n * n if inst["model"] == "tanh" else 0
+            nab = (n * n + n * m) if inst["model"] == "tanh" else N * (n * n + n * m)
+            prob = (nab + nw + n * n * (1 if inst["Q"].shape[0] == 1 else N)
+                    + m * m * (1 if inst["R"].shape[0] == 1 else N) + n * n + (N + 1) * n + N * m
+                    + N * p * (n + m + 1) + (2 * m if box else 0) + n)
+            state = 2 * (N + 1) * n + N * m + N * p + 2 * N * m
+            ws = _workspace_bytes_for(inst, self.max_group, self.dtype, self.params_for(inst))
+            ns = self.n_streams_slow if self._bucket_ms.get(key, 0.0) >= self.slow_ms else self.n_streams
+            self._probe[key] = ((prob + state) * es, ns * ((prob + state) * es * self.max_group + ws))
+        slot, fixed = self._probe[key]
+        return fixed + self.slots_per_bucket * slot
+
+    def params_for(self, inst):
+        key = shape_key(inst)
+        if key not in self._params:
+            self._params[key], self._bucket_ms[key] = _bucket_params(inst, self.dtype, self.device, self.tune)
+        return self._params[key]
 
     def room_for(self, inst):
         """Whether ``open`` would admit a controller of this instance's shape now
         (only the shape fields of ``inst`` are read)."""
-        if self.used + self.controller_bytes(inst) > self.budget:
-            return False
-        b = self.buckets.get(shape_key(inst))
-        return b is None or bool(b.free)
+        need = self.bytes_to_admit(inst)
+        return need is not None and self.used + need <= self.budget
 
     def open(self, inst):
-        """Admit a controller if its bytes fit the budget; returns its id or None."""
-        need = self.controller_bytes(inst)
+        """Admit a controller if the device bytes it needs fit the budget (and
+        its bucket's slot limit); returns its id or None."""
+        need = self.bytes_to_admit(inst)
+        if need is None:
+            self.turned_away["slots"] += 1
+            return None
         if self.used + need > self.budget:
+            self.turned_away["budget"] += 1
             return None
         key = shape_key(inst)
         b = self.buckets.get(key)
         if b is None:
-            params, ms = _bucket_params(inst, self.dtype, self.device, self.tune)
+            params = self.params_for(inst)
+            ms = self._bucket_ms.get(key)
             ns = self.n_streams_slow if ms is not None and ms >= self.slow_ms else self.n_streams
             b = _Bucket(key, inst, self.slots_per_bucket, self.max_group, self.dtype, self.device, ns, params)
             self.buckets[key] = b
-        if not b.free:
-            return None
+        elif not b.free:
+            b.grow(self._growth(b))
         slot = b.free.pop()
         b.load(slot, inst)
         self._cid += 1
-        self.controllers[self._cid] = _Controller(self._cid, b, slot, need)
-        self.used += need
+        self.controllers[self._cid] = _Controller(self._cid, b, slot)
         return self._cid
 
     def close(self, cid):
-        """Release a controller's slot and bytes; its pending ticks are dropped.
-        A controller whose tick is on the device cannot close until it returns."""
+        """Release a controller's slot; its pending ticks are dropped.  A
+        controller whose tick is on the device cannot close until it returns."""
         c = self.controllers[cid]
         if c.busy:
             raise RuntimeError(f"controller {cid} has a tick in flight")
         self.pending = collections.deque(r for r in self.pending if r.cid != cid)
         del self.controllers[cid]
         c.bucket.free.append(c.slot)
-        self.used -= c.nbytes
 
     # --- requests ----------------------------------------------------------
     def request(self, cid, x0, iters, t_arrive=None):
+        """Queue one tick of controller ``cid`` (open) with measured state x0 and
+        ``iters`` >= 1 SQP iterations."""
+        if cid not in self.controllers:
+            raise KeyError(f"controller {cid} is not open")
+        iters = int(iters)
+        if iters < 1:
+            raise ValueError(f"a tick needs at least one SQP iteration (got {iters})")
+        x0 = np.asarray(x0, dtype=np.float64)
+        if x0.shape != (self.controllers[cid].bucket.slots.n,):
+            raise ValueError(f"x0 has shape {x0.shape}, controller {cid} expects "
+                             f"({self.controllers[cid].bucket.slots.n},)")
         self._rid += 1
-        r = Request(self._rid, cid, int(iters), time.perf_counter() if t_arrive is None else t_arrive,
-                    np.asarray(x0, dtype=np.float64))
+        r = Request(self._rid, cid, iters, time.perf_counter() if t_arrive is None else t_arrive, x0)
         self.pending.append(r)
         return r
 
@@ -297,14 +411,18 @@ class MPCServer:
             return
         by_bucket = collections.OrderedDict()
         rest = collections.deque()
+        taken = set()
         for r in self.pending:
             c = self.controllers[r.cid]
             b = c.bucket
-            if c.busy or not b.idle or len(by_bucket.get(b.key, ())) >= b.max_group:
+            if c.busy or r.cid in taken or not b.idle or len(by_bucket.get(b.key, ())) >= b.max_group:
                 rest.append(r)           # one tick per controller in flight; wait for a free area
                 continue
-            c.busy = True
+            taken.add(r.cid)
             by_bucket.setdefault(b.key, []).append(r)
+        # the pass over the queue is complete: now mark the launched controllers
+        for cid in taken:
+            self.controllers[cid].busy = True
         self.pending = rest
         for key, reqs in by_bucket.items():
             b = self.buckets[key]
@@ -340,6 +458,11 @@ class MPCServer:
             st.t["x0"][:k].copy_(x0_h, non_blocking=True)
             for key in _ITER_KEYS:
                 torch.index_select(b.state[key], 0, slots, out=st_it[key][:k])
+            # NaN fallback inputs: the next control of the plan each tick starts
+            # from (returned if the tick fails) and the reset control u_ref
+            N = st.N
+            io["u_next"][:k].copy_(st_it["U"][:k, min(1, N - 1), :])
+            io["u_reset"][:k].copy_(st.t["uref"][:k, 0, :])
             for j in range(iters[0]):
                 kj = sum(1 for v in iters if v > j)
                 b.solver(kj, si).iterate({key: st_it[key][:kj] for key in _ITER_KEYS})
@@ -350,13 +473,19 @@ class MPCServer:
 
     def _finish(self, b, io, st, st_it, slots, reqs, lo, hi):
         """First controls, shift and scatter of requests lo..hi-1 of a launch
-        (on the current stream)."""
+        (on the current stream).  A tick whose first control is NaN returns
+        the next control of the plan it started from (U0[1]) and its
+        controller restarts from (x0, u_ref, 0), as update_and_extract does
+        (mpx/utils/mpc_wrapper.py:87-95)."""
         it = {key: st_it[key][lo:hi] for key in _ITER_KEYS}
-        u0_h = io["u0_h"][lo:hi]
-        u0_h.copy_(it["U"][:, 0, :], non_blocking=True)
+        u0 = io["u0"][lo:hi]
+        u0.copy_(it["U"][:, 0, :])
         reset = io["reset"][lo:hi]
         reset.zero_()
-        shift_warmstart(st.slice(lo, hi), it, self.shift, reset=reset)
+        shift_warmstart(st.slice(lo, hi), it, self.shift, u_reset=io["u_reset"][lo:hi], reset=reset)
+        torch.where(reset[:, None] != 0, io["u_next"][lo:hi], u0, out=u0)
+        u0_h = io["u0_h"][lo:hi]
+        u0_h.copy_(u0, non_blocking=True)
         reset_h = io["reset_h"][lo:hi]
         reset_h.copy_(reset, non_blocking=True)
         for key in _ITER_KEYS:

@@ -97,7 +97,7 @@ def solve_batch(insts, starts, iters, threads=0, params=None):
     Yeq = _arr(np.stack([s[3] for s in starts]))
     Ybox = _arr(np.stack([s[4] for s in starts]))
     pv = _arr([prm["rho0"], prm["rho_max"], prm["rho_c"], prm["mu_b"], prm["tau"], prm["beta"],
-               prm["c1"], prm["armijo_slack"], prm["max_trials"]])
+               prm["c1"], prm["armijo_slack"], prm["max_trials"], prm["hess_floor"]])
     alphas = np.zeros((B, iters))
     threads = threads or os.cpu_count() or 1
     _load().pdilqr_solve_batch(

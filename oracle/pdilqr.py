@@ -20,8 +20,16 @@ scan -- with these fixed decisions:
 D1  V_t is the multiplier of the constraint c_t in L = f + sum V^T c, with
     c_init = x0 - X_0 paired with V_0 and c_t = f(x_t,u_t) - x_{t+1} paired with
     V_{t+1}; V is then the gradient of the cost-to-go (V_t = q_t + A_t^T V_{t+1}).
-D2  No Hessian regularisation; Gauss-Newton Hessians of the quadratic tracking
-    cost are exact.
+D2  The stage Hessians are used as given (Gauss-Newton Hessians of the tracking
+    cost are exact); every m x m Cholesky factorisation (H_t = R + B'SB of the
+    Riccati step, R^ and H of the scan's leaf elements) modifies its pivots:
+    pivot j (the diagonal entry before its square root) becomes
+    max(|pivot|, hess_floor * |H_jj|), so the factor is that of H + E with E
+    diagonal and non-negative.  For a positive definite H with
+    cond(H) < 1 / hess_floor nothing changes (pivot_j >= lambda_min >=
+    H_jj / cond); for an indefinite Q (the reference clips penalty Hessians to
+    [-1e6, 1e6], mpx/utils/objectives.py:194-196) it keeps the LQR step
+    defined and its gains bounded.
 D3  Merit penalty rho = clip(max(rho0, 2 (d_f + d_c + kappa/2) / |c|^2), rho_max)
     with d_f = grad^T dz, d_c = (dV - V)^T c, kappa = dz^T H dz; this gives
     phi'(0) <= -rho |c|^2 / 2 - kappa / 2 < 0.
@@ -67,6 +75,7 @@ DEFAULT_PARAMS = dict(
     c1=1e-4,           # Armijo constant
     max_trials=10,     # line-search trials
     armijo_slack=1e-6, # relative noise floor of the Armijo test
+    hess_floor=1e-8,   # Cholesky pivot floor relative to the diagonal entry (D2)
 )
 
 
@@ -270,7 +279,34 @@ def lqr_dense(L):
                 lam=lam.reshape(N + 1, n))
 
 
-def riccati_step(A, B, Q, R, M, q, r, c, S, s):
+def chol_floor(H, floor=DEFAULT_PARAMS["hess_floor"]):
+    """Lower Cholesky factor of the symmetric H with modified pivots (D2):
+    pivot j, a = H_jj - sum_k L_jk^2, is replaced by max(|a|, floor |H_jj|)
+    (a NaN pivot stays NaN).  L L' = H + E with E diagonal, non-negative, and
+    zero when every pivot of H is at least floor |H_jj|."""
+    m = H.shape[0]
+    L = np.zeros_like(H)
+    for j in range(m):
+        a = abs(H[j, j] - L[j, :j] @ L[j, :j])
+        fl = floor * abs(H[j, j])
+        if not (a >= fl or np.isnan(a)):
+            a = fl
+        L[j, j] = np.sqrt(a)
+        L[j + 1:, j] = (H[j + 1:, j] - L[j + 1:, :j] @ L[j, :j]) / L[j, j]
+    return L
+
+
+def _floor(L):
+    return L.get("hess_floor", DEFAULT_PARAMS["hess_floor"])
+
+
+def spd_solve(H, G, floor=DEFAULT_PARAMS["hess_floor"]):
+    """H^{-1} G through the floored Cholesky factor (D2)."""
+    Lc = chol_floor(H, floor)
+    return np.linalg.solve(Lc.T, np.linalg.solve(Lc, G))
+
+
+def riccati_step(A, B, Q, R, M, q, r, c, S, s, floor=DEFAULT_PARAMS["hess_floor"]):
     """One backward Riccati step with cross term M (value V(x) = x'Sx/2 + s'x).
 
     Returns (S_t, s_t, K_t, k_t) with du = -K dx - k.
@@ -279,7 +315,7 @@ def riccati_step(A, B, Q, R, M, q, r, c, S, s):
     H = R + B.T @ S @ B
     G = M.T + B.T @ S @ A
     gu = r + B.T @ Sc
-    Hc = np.linalg.cholesky(H)
+    Hc = chol_floor(H, floor)
     K = np.linalg.solve(Hc.T, np.linalg.solve(Hc, G))
     k = np.linalg.solve(Hc.T, np.linalg.solve(Hc, gu))
     S_t = Q + A.T @ S @ A - G.T @ K
@@ -312,7 +348,8 @@ def lqr_riccati(L):
     S[N], s[N] = L["QN"], L["qN"]
     for t in range(N - 1, -1, -1):
         S[t], s[t], K[t], k[t] = riccati_step(A[t], B[t], L["Q"][t], L["R"][t], L["M"][t],
-                                              L["q"][t], L["r"][t], L["c"][t], S[t + 1], s[t + 1])
+                                              L["q"][t], L["r"][t], L["c"][t], S[t + 1], s[t + 1],
+                                              _floor(L))
     dx, du, lam = _forward(L, K, k, S, s)
     return dict(dx=dx, du=du, lam=lam, S=S, s=s, K=K, k=k)
 
@@ -326,7 +363,7 @@ def lqr_riccati(L):
 def leaf_element(L, t):
     """Element of stage t after eliminating u (u = -R^{-1}(M'x + r) + v)."""
     A, B, Q, R, M = L["A"][t], L["B"][t], L["Q"][t], L["R"][t], L["M"][t]
-    Ri = np.linalg.inv(R)
+    Ri = spd_solve(R, np.eye(R.shape[0]), _floor(L))
     At = A - B @ Ri @ M.T
     b = L["c"][t] - B @ Ri @ L["r"][t]
     C = B @ Ri @ B.T
@@ -367,7 +404,7 @@ def combine_leaf(L, t, e2):
     At, b1, _, eta1, J1 = leaf_element(L, t)
     B, R = L["B"][t], L["R"][t]
     H = R + B.T @ J2 @ B
-    Hi = np.linalg.inv(H)
+    Hi = spd_solve(H, np.eye(H.shape[0]), _floor(L))
     JB = J2 @ B
     n = At.shape[0]
     XT = np.eye(n) - JB @ Hi @ B.T          # X^T = I - J2 B H^{-1} B^T
@@ -413,8 +450,8 @@ def _policy(L, S, s):
         H = L["R"][t] + B[t].T @ S[t + 1] @ B[t]
         G = L["M"][t].T + B[t].T @ S[t + 1] @ A[t]
         gu = L["r"][t] + B[t].T @ (S[t + 1] @ L["c"][t] + s[t + 1])
-        K[t] = np.linalg.solve(H, G)
-        k[t] = np.linalg.solve(H, gu)
+        K[t] = spd_solve(H, G, _floor(L))
+        k[t] = spd_solve(H, gu, _floor(L))
     return K, k
 
 
@@ -454,7 +491,8 @@ def lqr_scan(L, P=4, tree=False):
         S[N], s[N] = L["QN"], L["qN"]
         for t in range(N - 1, a - 1, -1):
             S[t], s[t], _, _ = riccati_step(A[t], B[t], L["Q"][t], L["R"][t], L["M"][t],
-                                            L["q"][t], L["r"][t], L["c"][t], S[t + 1], s[t + 1])
+                                            L["q"][t], L["r"][t], L["c"][t], S[t + 1], s[t + 1],
+                                            _floor(L))
         # up-sweep of the other segments: leaf combines from the right
         aggs = []
         for (a, b) in segs[:-1]:
@@ -472,7 +510,8 @@ def lqr_scan(L, P=4, tree=False):
             S[b], s[b] = starts[i + 1]
             for t in range(b - 1, a - 1, -1):
                 S[t], s[t], _, _ = riccati_step(A[t], B[t], L["Q"][t], L["R"][t], L["M"][t],
-                                                L["q"][t], L["r"][t], L["c"][t], S[t + 1], s[t + 1])
+                                                L["q"][t], L["r"][t], L["c"][t], S[t + 1], s[t + 1],
+                                                _floor(L))
     K, k = _policy(L, S, s)
     # forward pass as an affine-map scan over P segments
     F = np.stack([A[t] - B[t] @ K[t] for t in range(N)])
@@ -538,6 +577,7 @@ def sqp_step(inst, X, U, V, Yeq, Ybox, params=None, lqr="riccati", P=4):
     if box:
         Sl, Ybox = box_duals(inst, U, Ybox, mu_b)
     L = linearize(inst, X, U, Yeq, Ybox, rho_c, mu_b)
+    L["hess_floor"] = prm["hess_floor"]
     if lqr == "riccati":
         sol = lqr_riccati(L)
     elif lqr == "dense":

@@ -17,7 +17,7 @@ typedef struct {
   const double *A, *B, *W, *Q, *R, *QN, *xref, *uref, *D, *E, *e, *ulo, *uhi, *x0;
 } inst_t;
 
-enum { P_RHO0, P_RHOMAX, P_RHOC, P_MUB, P_TAU, P_BETA, P_C1, P_SLACK, P_TRIALS };
+enum { P_RHO0, P_RHOMAX, P_RHOC, P_MUB, P_TAU, P_BETA, P_C1, P_SLACK, P_TRIALS, P_HFLOOR };
 
 /* C = A (r x k) * B (k x c) */
 static void mm(const double* A, const double* B, double* C, int r, int k, int c) {
@@ -39,11 +39,15 @@ static void mtm(const double* A, const double* B, double* C, int k, int r, int c
     }
 }
 
-/* in-place Cholesky (lower); returns 0 on success */
-static int chol(double* H, int m) {
+/* in-place Cholesky (lower) with pivots max(|pivot|, floor |H_jj|) (decision
+ * D2 of oracle/pdilqr.py); returns 0 on success */
+static int chol(double* H, int m, double floor) {
   for (int j = 0; j < m; ++j) {
     double d = H[j * m + j];
+    const double fl = floor * fabs(H[j * m + j]);
     for (int k = 0; k < j; ++k) d -= H[j * m + k] * H[j * m + k];
+    d = fabs(d);
+    if (!(d >= fl) && !isnan(d)) d = fl;
     if (!(d > 0)) return -1;
     d = sqrt(d);
     H[j * m + j] = d;
@@ -285,7 +289,7 @@ static double sqp_one(const inst_t* I, double* X, double* U, double* V, double* 
       for (int k = 0; k < n; ++k) v += B[k * m + i] * Sc[k];
       Gg[i * (n + 1) + n] = v;
     }
-    if (chol(H, m)) { free(mem); return NAN; }
+    if (chol(H, m, prm[P_HFLOOR])) { free(mem); return NAN; }
     double* Y = G;                   /* keep G' = [G | g] for S update */
     memcpy(Y, Gg, sizeof(double) * m * (n + 1));
     chol_solve(H, Gg, m, n + 1);     /* [K | k] */

@@ -10,14 +10,17 @@ Relative KKT eps = kkt / max(1, kkt at the start).
 * Quadruped-sized workload (n=36, m=12, N=50, 8 equality rows, B=4096, three
   iterations from a rollout start): |eps_fp32 - eps_fp64| <= 1e-5 for every
   instance at every iteration.
+* Config 2 at full shape (n=24, m=12, N=100, |u| <= 1, B=1024, five
+  iterations from a rollout): |eps_fp32 - eps_fp64| <= 1e-5.
 * Humanoid-sized workload (n=76, m=32, N=50, 12 equality rows and |u| <= 1,
   B=256, ten iterations from a zero trajectory): the per-iteration relative
   errors of solution, multipliers and eps are printed and, when a
   ``gpurun_out`` folder exists, written there as JSON.  Those ten iterations
   are the pre-asymptotic phase (the box's fraction-to-boundary rule holds the
   step near 0.1-0.4 and eps is O(1-10); most instances converge between
-  iterations 20 and 30), where FP32 rounding moves the iterate by about 1e-4
-  relative; the bounds asserted are about three times the measured errors.
+  iterations 20 and 30).  The bound is an error model, not a measured value:
+  the FP32 error must stay within 200x the spread of the FP64 iterates under
+  FP32-sized input perturbations (tests/error_model.py).
 """
 import json
 import os
@@ -94,9 +97,50 @@ def test_workload3_fp32_kkt_within_1e5():
         assert r["d_eps"] <= 1e-5, r
 
 
-def test_workload4_fp32_vs_fp64_per_iteration():
-    rows = _compare(4, 256, syn.WORKLOADS[4]["iters"], "zero")
+def test_workload2_fp32_kkt_within_1e5():
+    """Config 2 at its full shape: B=1024, n=24, m=12, N=100, |u| <= 1, five
+    iterations from the rollout warm start."""
+    rows = _compare(2, 1024, syn.WORKLOADS[2]["iters"], "rollout")
     for r in rows:
-        assert max(r["x"], r["u"], r["v"]) <= 2e-3, r
-        assert max(r["y_eq"], r["y_box"]) <= 5e-3, r
-        assert r["d_eps_rel"] <= 1e-3, r
+        assert r["d_eps"] <= 1e-5, r
+
+
+def _spread(wl, B, iters, start, reps=2):
+    """Per-iteration relative spread of the FP64 iterates under FP32-sized
+    relative perturbations of every input (tests/error_model.py)."""
+    from error_model import perturb_instance, perturb_start
+    insts = [syn.make_instance(wl, i) for i in range(B)]
+    starts = [syn.initial_iterate(d, start) for d in insts]
+    threads = min(16, os.cpu_count() or 1)
+
+    def run(ins, sts):
+        rows, cur = [], [tuple(s) for s in sts]
+        for _ in range(iters):
+            out = cport.solve_batch(ins, cur, 1, threads=threads)
+            rows.append({k: out[j] for j, k in enumerate(KEYS)})
+            cur = [tuple(out[j][i] for j in range(5)) for i in range(B)]
+        return rows
+
+    base = run(insts, starts)
+    rng = np.random.default_rng(1)
+    spread = [dict.fromkeys(("x", "u", "v", "y_eq", "y_box"), 0.0) for _ in range(iters)]
+    for _ in range(reps):
+        pert = run([perturb_instance(d, rng) for d in insts], [perturb_start(s, rng) for s in starts])
+        for k in range(iters):
+            for key, name in zip(KEYS, ("x", "u", "v", "y_eq", "y_box")):
+                spread[k][name] = max(spread[k][name], _rel(pert[k][key], base[k][key]))
+    return spread
+
+
+def test_workload4_fp32_vs_fp64_per_iteration():
+    """Config 4 (B=256, n=76, m=32, N=50, 12 rows, |u| <= 1, ten iterations
+    from zero): the FP32 error of solution and multipliers stays within 200x
+    the FP64 iterates' spread under FP32-sized input perturbations, iteration
+    by iteration (the problem's own sensitivity; these ten iterations are the
+    pre-asymptotic phase, where the box's fraction-to-boundary rule holds the
+    step at 0.1-0.4 and the relative KKT residual is O(1-10))."""
+    rows = _compare(4, 256, syn.WORKLOADS[4]["iters"], "zero")
+    spread = _spread(4, 256, syn.WORKLOADS[4]["iters"], "zero")
+    for r, sp in zip(rows, spread):
+        for name in ("x", "u", "v", "y_eq", "y_box"):
+            assert r[name] <= 200 * sp[name] + 1e-6, (r["iter"], name, r[name], sp[name])

@@ -73,13 +73,19 @@ def test_fp32_workload3_shape(segments, flags):
     """n=36, m=12, N=50, p=8 (the benchmark shape), 3 iterations, FP32; flags=1
     forms the products on VALU instead of the matrix cores."""
     insts, starts, refs, out, solver, it = _run(3, 4, 3, torch.float32, segments=segments, flags=flags)
+    k0 = [max(1.0, orc.kkt_residual(d, *s)) for d, s in zip(insts, starts)]
     for k in range(3):
         g, st = out[k]
         for i in range(4):
             r = refs[i][k]
             assert st["status"][i] == 0
             assert _rel(g["X"][i], r["X"]) < TOL32_X, (k, i, _rel(g["X"][i], r["X"]))
-            assert _rel(g["U"][i], r["U"]) < TOL32_X * 10, (k, i, _rel(g["U"][i], r["U"]))
+            for key in ("U", "V", "Yeq"):
+                assert _rel(g[key][i], r[key]) < TOL32_X * 10, (k, i, key, _rel(g[key][i], r[key]))
+            # relative KKT residual of the FP32 iterate against the oracle's (both in FP64)
+            e32 = orc.kkt_residual(insts[i], *[g[key][i] for key in ("X", "U", "V", "Yeq", "Ybox")]) / k0[i]
+            e64 = orc.kkt_residual(insts[i], *[r[key] for key in ("X", "U", "V", "Yeq", "Ybox")]) / k0[i]
+            assert abs(e32 - e64) <= TOL32_KKT, (k, i, e32, e64)
 
 
 def test_lq_exact_one_step_fp32():
@@ -213,3 +219,94 @@ def test_stage_parallel_linearization(wl, shape, iters, start):
     for k in range(iters):
         for key in ("X", "U", "V"):
             assert _rel(a[k][0][key], b[k][0][key]) < TOL32_X, (k, key)
+
+
+def _cond_instances(B, cond, n=12, m=12, N=20):
+    """LTV instances (workload 1's distributions) whose control cost R_t has
+    eigenvalues log-spaced over [1, cond] in a random basis, so that
+    H_t = R_t + B_t' S_{t+1} B_t (B_t entries ~ 0.02) has about that condition."""
+    insts = []
+    for i in range(B):
+        d = syn.make_instance(1, 60 + i, n=n, m=m, N=N)
+        rng = np.random.default_rng([17, i])
+        Uq, _ = np.linalg.qr(rng.standard_normal((m, m)))
+        R = (Uq * np.logspace(0.0, np.log10(cond), m)) @ Uq.T
+        d["R"] = np.broadcast_to(0.5 * (R + R.T), (N, m, m)).copy()
+        insts.append(d)
+    return insts
+
+
+def test_fp32_wave_kernel_ill_conditioned_H():
+    """The register Cholesky of the wave LQR kernel at cond(H_t) >= 5e5 against
+    the FP64 oracle: the FP32 error of the step (X, U after one SQP iteration
+    of the LQ problem) stays within 200x the FP64 solution's spread under
+    FP32-sized input perturbations (tests/error_model.py)."""
+    from error_model import perturb_instance, perturb_start, rel_err
+    insts = _cond_instances(4, 1e6)
+    starts = [syn.initial_iterate(d, "tile") for d in insts]
+    # the condition of H_t the kernel meets
+    conds = []
+    for d, s in zip(insts, starts):
+        st = orc.sqp_step(d, *s)
+        L, sol = st["lqr"], orc.lqr_riccati(st["lqr"])
+        for t in range(d["N"]):
+            H = L["R"][t] + L["B"][t].T @ sol["S"][t + 1] @ L["B"][t]
+            conds.append(np.linalg.cond(H))
+    assert max(conds) >= 5e5, max(conds)
+    batch, solver = _solver(insts, torch.float32)
+    it = batch.iterate_from(*[np.stack([s[k] for s in starts]) for k in range(5)])
+    solver.iterate(it)
+    torch.cuda.synchronize()
+    refs = [orc.sqp_step(d, *s) for d, s in zip(insts, starts)]
+    rng = np.random.default_rng(0)
+    spread = {"X": 0.0, "U": 0.0}
+    for _ in range(3):
+        pert = [orc.sqp_step(perturb_instance(d, rng), *perturb_start(s, rng)) for d, s in zip(insts, starts)]
+        for k in spread:
+            spread[k] = max(spread[k], rel_err(np.stack([p[k] for p in pert]), np.stack([r[k] for r in refs])))
+    for k in ("X", "U"):
+        got = it[k].double().cpu().numpy()
+        err = rel_err(got, np.stack([r[k] for r in refs]))
+        print(f"cond(H) max {max(conds):.2e}: {k} FP32 error {err:.2e}, FP64 spread under u32 {spread[k]:.2e}")
+        assert err <= 200 * spread[k] + 1e-6, (k, err, spread[k])
+
+
+def _indefinite_instances(B, n=10, m=6, N=12):
+    """Tanh-model instances whose control Hessian R_t has a strongly negative
+    eigenvalue, so that H_t = R_t + B'S B is indefinite at some stage and the
+    modified pivots of the Cholesky factorisation (D2) decide the step."""
+    insts = []
+    for i in range(B):
+        d = syn.make_instance(2, 80 + i, n=n, m=m, N=N, box=False)
+        R = d["R"][0].copy()
+        R[1, 1] = -5.0
+        d["R"] = R[None]
+        insts.append(d)
+    return insts
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_indefinite_stage_hessian(dtype):
+    """Indefinite H_t: the device follows the oracle's modified Cholesky
+    (hess_floor = 1e-2) -- FP64 to 1e-9, FP32 (register wave kernel) to the
+    FP32 tolerance -- and the iterate stays finite."""
+    insts = _indefinite_instances(3)
+    starts = [syn.initial_iterate(d, "rollout") for d in insts]
+    L0 = orc.sqp_step(insts[0], *starts[0], params=dict(hess_floor=1e-2))["lqr"]
+    sol = orc.lqr_riccati(L0)
+    mins = [np.linalg.eigvalsh(L0["R"][t] + L0["B"][t].T @ sol["S"][t + 1] @ L0["B"][t]).min()
+            for t in range(insts[0]["N"])]
+    assert min(mins) < 0, "the instance must make H_t indefinite"
+    dt = torch.float64 if dtype == "f64" else torch.float32
+    batch, solver = _solver(insts, dt, hess_floor=1e-2)
+    it = batch.iterate_from(*[np.stack([s[k] for s in starts]) for k in range(5)])
+    refs = [orc.solve(d, *s, 2, params=dict(hess_floor=1e-2)) for d, s in zip(insts, starts)]
+    tol = 1e-9 if dtype == "f64" else TOL32_X
+    for k in range(2):
+        solver.iterate(it)
+        torch.cuda.synchronize()
+        for i in range(3):
+            for key in ("X", "U", "V"):
+                got = it[key][i].double().cpu().numpy()
+                assert np.all(np.isfinite(got))
+                assert _rel(got, refs[i][k][key]) < (1 if key == "X" else 10) * tol, (k, i, key)

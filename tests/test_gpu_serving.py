@@ -1,6 +1,9 @@
 """Serving: continuous batching gives every controller the same result alone
-and in the mix; admission follows the HBM budget; a NaN in one controller
-triggers its own fallback and leaves its batch neighbours untouched."""
+and in the mix, and the result of the oracle on the same data (FP64); the
+humanoid-sized class (n=76, m=32, N=100) is served in the mix; admission
+follows the HBM budget; a NaN in one controller triggers its own fallback
+(update_and_extract, mpx/utils/mpc_wrapper.py:87-95) and leaves its batch
+neighbours untouched."""
 import numpy as np
 import pytest
 
@@ -10,8 +13,8 @@ from mpx_amd import synthetic as syn  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(12, 12, 25), (36, 12, 50), (12, 12, 25), (24, 12, 20), (36, 12, 50)]
-ITERS = [[1, 3, 2, 8, 1], [2, 2, 5, 1, 4], [3, 1, 1, 2, 6]]
+SHAPES = [(12, 12, 25), (36, 12, 50), (12, 12, 25), (24, 12, 20), (36, 12, 50), (76, 32, 100)]
+ITERS = [[1, 3, 2, 8, 1, 2], [2, 2, 5, 1, 4, 3], [3, 1, 1, 2, 6, 1]]
 
 
 def _server(**kw):
@@ -54,23 +57,131 @@ def test_alone_equals_mix():
 
 
 def test_admission_by_budget():
+    """The budget counts the bytes the engine holds (slot arrays at capacity,
+    staging arrays and workspaces); a full bucket grows only within it."""
     from mpx_amd.serving import MPCServer
     d = _insts()[1]
-    probe = MPCServer(1 << 40, max_group=2, slots_per_bucket=4, n_streams=1)
-    need = probe.controller_bytes(d)
-    srv = MPCServer(2 * need + need // 2, max_group=2, slots_per_bucket=4, n_streams=1)
-    a, b = srv.open(d), srv.open(d)
-    assert a is not None and b is not None
-    assert srv.open(d) is None              # over budget
-    srv.close(a)
-    assert srv.open(d) is not None          # freed bytes are reusable
+    probe = MPCServer(1 << 40, max_group=2, slots_per_bucket=2, n_streams=1)
+    first = probe.bytes_to_admit(d)         # a new bucket: staging + 2 slots
+    a = probe.open(d)
+    assert a is not None and probe.used == first
+    b_ = probe.buckets[next(iter(probe.buckets))]
+    assert probe.bytes_to_admit(d) == 0     # a free slot
+    probe.open(d)
+    grow = probe.bytes_to_admit(d)          # full: doubling adds 2 slots
+    assert grow == 2 * b_.slot_bytes
+    srv = MPCServer(first + grow // 2, max_group=2, slots_per_bucket=2, n_streams=1)
+    x, y = srv.open(d), srv.open(d)
+    assert x is not None and y is not None
+    assert srv.open(d) is None              # growth would exceed the budget
+    assert srv.turned_away["budget"] == 1
+    srv.close(x)
+    z = srv.open(d)                         # the freed slot is reused
+    assert z is not None and srv.used == first
+    big = MPCServer(first + grow, max_group=2, slots_per_bucket=2, n_streams=1)
+    cids = [big.open(d) for _ in range(4)]  # grows to 4 slots within the budget
+    assert all(c is not None for c in cids) and big.used == first + grow
+    for k, c in enumerate(cids):            # data of the slots survives growth
+        big.request(c, _x(d, k), 1)
+    assert len(big.run_round()) == 4
+
+
+def test_request_validation():
+    from mpx_amd.serving import MPCServer
+    d = _insts()[0]
+    srv = _server()
+    c = srv.open(d)
+    with pytest.raises(ValueError):
+        srv.request(c, d["x0"], 0)
+    with pytest.raises(KeyError):
+        srv.request(c + 99, d["x0"], 1)
+    with pytest.raises(ValueError):
+        srv.request(c, d["x0"][:-1], 1)
+    srv.close(c)
+    with pytest.raises(KeyError):
+        srv.request(c, d["x0"], 1)
+    assert not srv.busy
+
+
+def _oracle_tick(inst, state, x0, iters, shift=1):
+    """One served tick on the oracle: iters SQP iterations from the
+    controller's warm start with the measured state x0, first control, shift."""
+    from oracle import pdilqr as orc
+    d = dict(inst)
+    d["x0"] = np.asarray(x0, dtype=np.float64)
+    out = orc.solve(d, *state, iters)[-1]
+    new = [out[k] for k in ("X", "U", "V", "Yeq", "Ybox")]
+    u0 = new[1][0].copy()
+    return u0, [orc.shift_warmstart(a, shift) for a in new]
+
+
+def test_fp64_served_ticks_equal_oracle():
+    """FP64 serving: every tick's first control and the controller's slot
+    after the tick (X, U, V, multipliers) equal the oracle's, to 1e-9."""
+    insts = [syn.make_instance(5, 300 + i, n=n, m=m, N=N) for i, (n, m, N) in
+             enumerate([(12, 12, 25), (24, 12, 20), (12, 12, 25)])]
+    srv = _server(dtype=torch.float64)
+    cids = [srv.open(d) for d in insts]
+    states = []
+    for d in insts:
+        N, n, m = d["N"], d["n"], d["m"]
+        states.append([np.tile(d["x0"], (N + 1, 1)), np.zeros((N, m)), np.zeros((N + 1, n)),
+                       np.zeros((N, d["p"])), np.zeros((N, 2 * m))])
+    for k, iters in enumerate([[1, 3, 2], [4, 1, 2], [2, 2, 5]]):
+        reqs = [srv.request(c, _x(d, k), it) for c, d, it in zip(cids, insts, iters)]
+        srv.run_round()
+        for i, (r, d) in enumerate(zip(reqs, insts)):
+            u0, states[i] = _oracle_tick(d, states[i], _x(d, k), iters[i])
+            scale = max(1.0, np.max(np.abs(u0)))
+            assert np.max(np.abs(r.u0 - u0)) / scale < 1e-9, (k, i)
+            c = srv.controllers[cids[i]]
+            for key, ref in zip(("X", "U", "V", "Yeq", "Ybox"), states[i]):
+                got = c.bucket.state[key][c.slot].cpu().numpy()
+                if ref.size:
+                    assert np.max(np.abs(got - ref)) / max(1.0, np.max(np.abs(ref))) < 1e-9, (k, i, key)
+
+
+def test_fp32_humanoid_class_against_fp64():
+    """The humanoid-sized class (n=76, m=32, N=100, |u| <= 1) served in FP32
+    against the FP64 C restatement of the same ticks: first controls within
+    2e-3 of max(1, |u0|) after 1-3 iterations from the controller's cold start."""
+    from oracle import cport
+    if not cport.available():
+        pytest.skip("C restatement of the oracle unavailable")
+    d = syn.make_instance(5, 400, n=76, m=32, N=100)
+    srv = _server()
+    c = srv.open(d)
+    N, n, m = d["N"], d["n"], d["m"]
+    state = [np.tile(d["x0"], (N + 1, 1)), np.zeros((N, m)), np.zeros((N + 1, n)), np.zeros((N, 0)),
+             np.zeros((N, 2 * m))]
+    from oracle import pdilqr as orc
+    for k, iters in enumerate([1, 3, 2]):
+        r = srv.request(c, _x(d, k), iters)
+        srv.run_round()
+        dk = dict(d)
+        dk["x0"] = _x(d, k)
+        out = cport.solve_batch([dk], [tuple(state)], iters)
+        new = [a[0] for a in out[:5]]
+        u0 = new[1][0].copy()
+        state = [orc.shift_warmstart(a, 1) for a in new]
+        err = np.max(np.abs(r.u0 - u0)) / max(1.0, np.max(np.abs(u0)))
+        assert err < 2e-3, (k, err)
 
 
 def test_nan_fallback_isolated():
+    """A tick whose solve turns NaN returns the next control of the plan it
+    started from (U0[1]), restarts its controller from (x0, u_ref, 0), and
+    leaves its neighbours' results untouched."""
     insts = _insts()[:3]
     clean, dirty = _server(), _server()
     cc = [clean.open(d) for d in insts]
     cd = [dirty.open(d) for d in insts]
+    for srv, cids in ((clean, cc), (dirty, cd)):       # one good tick first: a warm plan
+        for c, d in zip(cids, insts):
+            srv.request(c, _x(d, 0), 2)
+        srv.run_round()
+    ctl = dirty.controllers[cd[1]]
+    plan_u1 = ctl.bucket.state["U"][ctl.slot][1].cpu().numpy().copy()
     for srv, cids, bad in ((clean, cc, False), (dirty, cd, True)):
         reqs = []
         for i, (c, d) in enumerate(zip(cids, insts)):
@@ -86,6 +197,16 @@ def test_nan_fallback_isolated():
         srv._last = reqs
     for i in (0, 2):
         assert np.array_equal(clean._last[i].u0, dirty._last[i].u0)
+    r = dirty._last[1]
+    assert np.all(np.isfinite(r.u0)) and np.array_equal(r.u0, plan_u1.astype(r.u0.dtype))
+    d = insts[1]
+    U = ctl.bucket.state["U"][ctl.slot].cpu().numpy()
+    assert np.array_equal(U, np.broadcast_to(d["uref"][0], U.shape).astype(U.dtype))
+    X = ctl.bucket.state["X"][ctl.slot].cpu().numpy()
+    x_bad = _x(d, 1).copy()
+    x_bad[0] = float("nan")          # reset to the measured state x0 (NaN included)
+    np.testing.assert_array_equal(X, np.tile(x_bad, (d["N"] + 1, 1)).astype(X.dtype))
+    assert not np.any(ctl.bucket.state["V"][ctl.slot].cpu().numpy())
 
 
 def test_overlapped_launches_equal_alone():

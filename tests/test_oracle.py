@@ -268,3 +268,57 @@ def test_kkt_residual_matches_lagrangian_fd():
     r = np.concatenate([gX.ravel(), gU.ravel(), c.ravel(), h.ravel(), comp.ravel()])
     kkt = orc.kkt_residual(inst, X, U, V, Yeq, Ybox)
     assert abs(kkt - np.sqrt(r @ r)) < 1e-6 * max(1.0, kkt)
+
+
+def test_floored_cholesky():
+    """D2: with no pivot below the floor the factor is the textbook Cholesky
+    factor; with an indefinite matrix the factor is exact for H + E with E
+    diagonal, non-negative, nonzero only at the modified pivots, and bounded
+    (twice the pivot's magnitude, or the floor)."""
+    rng = np.random.default_rng(11)
+    G = rng.standard_normal((6, 6))
+    H = G @ G.T + 0.1 * np.eye(6)
+    assert _rel(orc.chol_floor(H, 1e-8), np.linalg.cholesky(H)) < 1e-12
+    # indefinite: eigenvalues of both signs
+    Uq, _ = np.linalg.qr(rng.standard_normal((6, 6)))
+    Hi = (Uq * np.array([5.0, 3.0, -2.0, 1.0, -0.5, 4.0])) @ Uq.T
+    Hi = 0.5 * (Hi + Hi.T)
+    fl = 1e-3
+    Lc = orc.chol_floor(Hi, fl)
+    E = Lc @ Lc.T - Hi
+    assert np.max(np.abs(E - np.diag(np.diag(E)))) < 1e-12
+    e = np.diag(E)
+    assert np.all(e >= -1e-12) and np.any(e > 1e-6)
+    assert np.max(e) < 2 * np.max(np.abs(np.linalg.eigvalsh(Hi))) * 6
+    # the modification is positive definite
+    assert np.linalg.eigvalsh(Hi + E).min() > 0
+    # a pivot that vanishes (singular H) is raised to the floor exactly
+    Hs = np.array([[1.0, 1.0], [1.0, 1.0]])
+    Ls = orc.chol_floor(Hs, fl)
+    assert abs(Ls[1, 1] ** 2 - fl) < 1e-15
+
+
+def test_riccati_with_floor_matches_dense_kkt_of_modified_problem():
+    """An LQR whose H_t = R + B'SB is indefinite at some stage: the floored
+    Riccati pass with floor f equals the dense KKT solve of the problem whose
+    R_t is raised by the diagonal the floor added (same factorisation), so
+    the floor changes only the m x m blocks it names."""
+    rng = np.random.default_rng(5)
+    N, n, m = 5, 4, 3
+    L = _random_lqr(rng, N, n, m, cross=False)
+    L["R"][2] = np.diag([1.0, -1e3, 0.5])           # indefinite H_2 = R_2 + B'SB
+    L["hess_floor"] = 1e-2
+    sol = orc.lqr_riccati(L)
+    # rebuild each stage's H and the diagonal the floor added
+    L2 = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in L.items()}
+    added = 0
+    for t in range(N):
+        H = L["R"][t] + L["B"][t].T @ sol["S"][t + 1] @ L["B"][t]
+        Lc = orc.chol_floor(H, L["hess_floor"])
+        E = np.diag(np.diag(Lc @ Lc.T - H))
+        added += np.count_nonzero(np.diag(E) > 1e-12)
+        L2["R"][t] = L["R"][t] + E
+    assert added >= 1
+    ref = orc.lqr_dense(L2)
+    for k in ("dx", "du", "lam"):
+        assert _rel(sol[k], ref[k]) < 1e-9, k
