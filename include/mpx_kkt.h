@@ -54,7 +54,11 @@ enum { MPXKKT_F32 = 0, MPXKKT_F64 = 1 };
 /* dynamics families evaluated on the device */
 enum {
   MPXKKT_MODEL_LTV = 0,   /* x' = A_t x + B_t u                                   */
-  MPXKKT_MODEL_TANH = 1   /* x' = x + h (A x + B u + kappa tanh(W x))            */
+  MPXKKT_MODEL_TANH = 1,  /* x' = x + h (A x + B u + kappa tanh(W x))            */
+  MPXKKT_MODEL_CALLER = 2 /* the caller's own dynamics and cost, linearised by the
+                             caller at the iterate (see mpxkkt_problem) and
+                             evaluated by the caller at the line-search trials:
+                             mpxkkt_sqp_direction + mpxkkt_sqp_accept             */
 };
 
 typedef struct {
@@ -63,10 +67,28 @@ typedef struct {
   int64_t stage_stride;
 } mpxkkt_view;
 
-/* One batch of same-shaped instances.  Cost: sum_t 1/2 |x_t - xref_t|_Q_t^2 +
- * 1/2 |u_t - uref_t|_R_t^2 + 1/2 |x_N - xref_N|_QN^2 (the Gauss-Newton
- * (Q, R, M = 0) triple of hessian_approx, mpx/utils/objectives.py:75), with
- * optional rows D_t x + E_t u + e_t = 0 and boxes ulo <= u <= uhi. */
+/* One batch of same-shaped instances.  Cost of the device model families
+ * (LTV, TANH): sum_t 1/2 |x_t - xref_t|_Q_t^2 + 1/2 |u_t - uref_t|_R_t^2 +
+ * 1/2 |x_N - xref_N|_QN^2 (the Gauss-Newton (Q, R, M = 0) triple of
+ * hessian_approx, mpx/utils/objectives.py:75), with optional rows
+ * D_t x + E_t u + e_t = 0 and boxes ulo <= u <= uhi.
+ *
+ * MPXKKT_MODEL_CALLER: the caller's dynamics(x, u, t, parameter)
+ * (mpx/utils/models.py:6) and cost(W, reference, x, u, t)
+ * (mpx/utils/objectives.py:15) stay with the caller, who passes what the
+ * reference solver takes from jax.jacobian / jax.grad and from
+ * hessian_approx(W, reference, x, u, t) -> (Q, R, M) (objectives.py:75, :202)
+ * at the current iterate (X, U), per stage t < N unless noted:
+ *   A = f_x (n x n), B = f_u (n x m), fbar = f(x_t, u_t) (n),
+ *   Q (n x n), R (m x m), M (n x m, NULL = 0): the stage Hessian
+ *   [[Q, M], [M', R]] (used as given; D2 floors the m x m pivots),
+ *   q = grad_x l_t (n, rows 0..N), r = grad_u l_t (m),
+ *   g = l_t (1, rows 0..N), QN = terminal Hessian (n x n).
+ * xref, uref, W, h, kappa are unused.  The equality rows and boxes stay the
+ * library's (their augmented-Lagrangian and barrier terms are added on the
+ * device).  With boxes, linearise at U moved inside the box the way the
+ * library does on entry: a coordinate u >= hi becomes hi - 1e-3 (hi - lo),
+ * u <= lo becomes lo + 1e-3 (hi - lo). */
 typedef struct {
   int32_t dtype;        /* MPXKKT_F32 / MPXKKT_F64 for every array */
   int32_t model;        /* MPXKKT_MODEL_* */
@@ -78,6 +100,7 @@ typedef struct {
   mpxkkt_view D, E, e;  /* p x n, p x m, p (NULL when n_eq == 0) */
   mpxkkt_view ulo, uhi; /* m (NULL when has_box == 0) */
   mpxkkt_view x0;       /* n */
+  mpxkkt_view fbar, M, q, r, g;  /* MPXKKT_MODEL_CALLER only (else NULL) */
 } mpxkkt_problem;
 
 typedef struct {
@@ -136,6 +159,67 @@ mpxkkt_status mpxkkt_workspace_size(const mpxkkt_problem* prob, const mpxkkt_par
 mpxkkt_status mpxkkt_sqp_iterate(void* stream, const mpxkkt_problem* prob,
                                  const mpxkkt_params* prm, mpxkkt_iterate* it,
                                  mpxkkt_stats* stats, void* workspace, size_t workspace_bytes);
+
+/* The caller-dynamics SQP iteration (MPXKKT_MODEL_CALLER) in two calls, the
+ * merit line search split where the reference solver evaluates the caller's
+ * functions at the trial points:
+ *  1. mpxkkt_sqp_direction: augmentation of the caller's linearisation (K1),
+ *     LQR (K2/K3), multiplier steps, merit penalty rho, phi(0), phi'(0) and the
+ *     fraction-to-boundary cap a_max.  Writes the step dX [batch][N+1][n],
+ *     dU [batch][N][m] and a_max [batch] (dtype) into `dir`.
+ *  2. the caller evaluates, for trials j < n_trials at a_j = a_max beta^j,
+ *     x = X + a_j dX, u = U + a_j dU:
+ *        trial_g [batch][n_trials][N+1] = l_t(x_t, u_t) (t = N: terminal),
+ *        trial_f [batch][n_trials][N][n] = f(x_t, u_t, t)
+ *     (all candidates at once: the vmap over candidates of the reference).
+ *  3. mpxkkt_sqp_accept: the trials' merits, the first a_j that passes the
+ *     Armijo test (or the smallest, status 2), the multiplier and iterate
+ *     update in place on `it`, stats.
+ * Between the calls the workspace holds the step: no other call may use it. */
+typedef struct {
+  void* dX;         /* [batch][N+1][n] dtype */
+  void* dU;         /* [batch][N][m]   dtype */
+  void* alpha_max;  /* [batch]         dtype */
+} mpxkkt_direction;
+
+mpxkkt_status mpxkkt_sqp_direction(void* stream, const mpxkkt_problem* prob, const mpxkkt_params* prm,
+                                   mpxkkt_iterate* it, mpxkkt_direction* dir, void* workspace,
+                                   size_t workspace_bytes);
+
+mpxkkt_status mpxkkt_sqp_accept(void* stream, const mpxkkt_problem* prob, const mpxkkt_params* prm,
+                                mpxkkt_iterate* it, int32_t n_trials, const void* trial_g,
+                                const void* trial_f, mpxkkt_stats* stats, void* workspace,
+                                size_t workspace_bytes);
+
+/* Options of mpxkkt_solve: the iteration protocol of runOffline
+ * (mpx/utils/mpc_wrapper.py:425-467). */
+typedef struct {
+  int32_t max_iter;    /* at most max_iter + 1 iterations (runOffline stops when i > max_iter) */
+  int32_t check_every; /* 0: enqueue all max_iter + 1 iterations with no host synchronisation
+                          (stream-ordered, capturable in a HIP graph; converged instances skip
+                          the kernels of later iterations); k > 0: after every k iterations wait
+                          for the stream and stop enqueueing once no instance is running */
+  double dcost_tol;    /* stop when last - l2 < dcost_tol (1e-3) ...                      */
+  double cc_tol;       /* ... and sum |c|^2 < cc_tol (1e-5), l2 = sum_t g_t^2              */
+} mpxkkt_solve_opts;
+
+mpxkkt_status mpxkkt_default_solve_opts(mpxkkt_solve_opts* out);
+
+/* SQP iterations until runOffline's stop rule holds for every instance
+ * (mpx/utils/mpc_wrapper.py:425-467, the loop of optimizers.mpc calls and
+ * optimizers.model_evaluator_helper at :441-466): after iteration i (from 1)
+ * (g, c) = model evaluation at the new iterate, l2 = sum g^2, and an
+ * instance stops when i > max_iter or (last - l2 < dcost_tol and
+ * sum c^2 < cc_tol), with last = 1e10 before the first iteration and l2 after
+ * each later one.  A stopped instance's iterate and stats are left as they
+ * are.  Outputs (device, [batch]): iters (int32, iterations run), cost (l2 of
+ * the last iteration, dtype, may be NULL), cc (sum c^2, dtype, may be NULL).
+ * `launched` (host, may be NULL) receives the iterations enqueued.  Same
+ * workspace as mpxkkt_sqp_iterate. */
+mpxkkt_status mpxkkt_solve(void* stream, const mpxkkt_problem* prob, const mpxkkt_params* prm,
+                           const mpxkkt_solve_opts* opts, mpxkkt_iterate* it, mpxkkt_stats* stats,
+                           int32_t* iters, void* cost, void* cc, void* workspace, size_t workspace_bytes,
+                           int32_t* launched);
 
 /* KKT residual 2-norm per instance at `it` (out: [batch], dtype).  The
  * stop test of runOffline, mpx/utils/mpc_wrapper.py:452-466, in KKT form. */

@@ -13,13 +13,17 @@ LIB_PATH = os.path.join(_HERE, "lib", "libmpx_kkt.so")
 
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_WORKSPACE, ERR_DEVICE = 0, 1, 2, 3, 4
 F32, F64 = 0, 1
-MODEL_LTV, MODEL_TANH = 0, 1
+MODEL_LTV, MODEL_TANH, MODEL_CALLER = 0, 1, 2
 PHASES = 4          # MPXKKT_PHASES
 
 EXPORTED = (
     "mpxkkt_default_params",
     "mpxkkt_workspace_size",
     "mpxkkt_sqp_iterate",
+    "mpxkkt_sqp_direction",
+    "mpxkkt_sqp_accept",
+    "mpxkkt_default_solve_opts",
+    "mpxkkt_solve",
     "mpxkkt_kkt_residual",
     "mpxkkt_model_eval",
     "mpxkkt_shift_warmstart",
@@ -28,6 +32,10 @@ EXPORTED = (
     "mpxkkt_last_error",
     "mpxkkt_version",
 )
+
+
+# views of the caller-linearised form only (MODEL_CALLER)
+CALLER_VIEWS = ("fbar", "M", "q", "r", "g")
 
 
 class View(C.Structure):
@@ -45,6 +53,7 @@ class Problem(C.Structure):
         ("D", View), ("E", View), ("e", View),
         ("ulo", View), ("uhi", View),
         ("x0", View),
+        ("fbar", View), ("M", View), ("q", View), ("r", View), ("g", View),
     ]
 
 
@@ -69,6 +78,15 @@ class Stats(C.Structure):
                 ("dmerit", C.c_void_p), ("status", C.c_void_p), ("trials", C.c_void_p)]
 
 
+class Direction(C.Structure):
+    _fields_ = [("dX", C.c_void_p), ("dU", C.c_void_p), ("alpha_max", C.c_void_p)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("max_iter", C.c_int32), ("check_every", C.c_int32),
+                ("dcost_tol", C.c_double), ("cc_tol", C.c_double)]
+
+
 class KKTError(RuntimeError):
     pass
 
@@ -90,6 +108,14 @@ def load():
     lib.mpxkkt_workspace_size.argtypes = [P(Problem), P(Params), P(C.c_size_t)]
     lib.mpxkkt_sqp_iterate.argtypes = [C.c_void_p, P(Problem), P(Params), P(Iterate), P(Stats),
                                        C.c_void_p, C.c_size_t]
+    lib.mpxkkt_sqp_direction.argtypes = [C.c_void_p, P(Problem), P(Params), P(Iterate), P(Direction),
+                                         C.c_void_p, C.c_size_t]
+    lib.mpxkkt_sqp_accept.argtypes = [C.c_void_p, P(Problem), P(Params), P(Iterate), C.c_int32, C.c_void_p,
+                                      C.c_void_p, P(Stats), C.c_void_p, C.c_size_t]
+    lib.mpxkkt_default_solve_opts.argtypes = [P(SolveOpts)]
+    lib.mpxkkt_solve.argtypes = [C.c_void_p, P(Problem), P(Params), P(SolveOpts), P(Iterate), P(Stats),
+                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                 P(C.c_int32)]
     lib.mpxkkt_kkt_residual.argtypes = [C.c_void_p, P(Problem), P(Params), P(Iterate), C.c_void_p,
                                         C.c_void_p, C.c_size_t]
     lib.mpxkkt_model_eval.argtypes = [C.c_void_p, P(Problem), C.c_void_p, C.c_void_p, C.c_void_p,
@@ -118,6 +144,14 @@ def default_params(**over):
     for k, v in over.items():
         setattr(p, k, v)
     return p
+
+
+def default_solve_opts(**over):
+    o = SolveOpts()
+    check(load().mpxkkt_default_solve_opts(C.byref(o)))
+    for k, v in over.items():
+        setattr(o, k, v)
+    return o
 
 
 def profile_enable(on=True):

@@ -20,7 +20,10 @@ struct Prob {
   int B, n, m, N, p, box, model;
   T h, kappa;
   View<T> A, Bm, W, Q, R, QN, xref, uref, D, E, e, ulo, uhi, x0;
+  View<T> fbar, M, q, r, g;  // model 2 (caller linearisation): f(x_t,u_t), cross Hessian, gradients, costs
 };
+
+enum { MODEL_LTV = 0, MODEL_TANH = 1, MODEL_CALLER = 2 };
 
 template <typename T>
 struct Prm {
@@ -69,7 +72,19 @@ struct Ws {
   T* lsp;    // [B][8]          line-search scalars (rho, phi(0), phi'(0), step caps, slack)
   T* tpart;  // [B][N1][2*MAX_TRIALS] per-stage parts of phi at each trial step
   T* hq;     // [B][N][2p]      equality rows at the iterate and their slopes along the step
+  T* gpart;  // [B][N1][2]      per-stage cost g_t and squared defect rows (stop test)
+  T* last;   // [B]             previous iteration's sum g_t^2 (stop test)
+  int* run;  // [B]             1 while the instance iterates; nullptr: every instance runs
+  int* nrun; // [1]             instances still running (host check of the solve loop)
 };
+
+// Instances whose run flag is 0 (converged in a multi-iteration solve) leave
+// every kernel at once; b is uniform over the block, so no barrier is skipped
+// by part of a block.
+template <typename T>
+__device__ __forceinline__ bool skip_instance(const Ws<T>& ws, int b) {
+  return ws.run != nullptr && ws.run[b] == 0;
+}
 
 template <typename T>
 __device__ __forceinline__ T dev_tanh(T x) { return tanh(x); }

@@ -305,6 +305,7 @@ __device__ void terminal_terms(const Prob<T>& P, const Ws<T>& ws, int b, const T
 template <typename T>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3))) k_linearize_inst(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws) {
   const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+  if (skip_instance(ws, b)) return;
   const int lane = tid & 63, wv = tid >> 6;
   const int n = P.n, m = P.m, N = P.N, N1 = N + 1;
   const InstLds L = inst_lds(n, m, N, P.model == 1, P.Q.st == 0, P.R.st == 0, false);
@@ -383,6 +384,7 @@ __host__ __device__ inline int lin_stage_wave_elems(int n, int m) {
 template <typename T>
 __global__ void __launch_bounds__(512) k_linearize_stage(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws) {
   const int b = blockIdx.y, tid = threadIdx.x, nt = blockDim.x;
+  if (skip_instance(ws, b)) return;
   const int lane = tid & 63, wv = tid >> 6, wpb = nt >> 6;
   const int n = P.n, m = P.m, N = P.N, N1 = N + 1;
   const InstLds L = inst_lds(n, m, N, P.model == 1, P.Q.st == 0, P.R.st == 0, false);
@@ -474,6 +476,7 @@ __global__ void __launch_bounds__(256) k_linesearch_inst(Prob<T> P, Prm<T> prm, 
                                                          T* o_rho, T* o_phi0, T* o_dphi, int* o_status,
                                                          int* o_trials) {
   const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+  if (skip_instance(ws, b)) return;
   const int lane = tid & 63, wv = tid >> 6;
   const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
   const InstLds L = inst_lds(n, m, N, P.model == 1, P.Q.st == 0, P.R.st == 0, true);

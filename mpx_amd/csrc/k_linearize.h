@@ -8,6 +8,8 @@
 // The block t == N handles the terminal cost and the initial-state row
 // c_init = x0 - X_0.  The box projection of U and the replacement of
 // non-positive box multipliers by mu_b / s happen here, in place.
+// Model 2 (the caller's linearisation) takes f(x_t, u_t), the gradients and the
+// stage costs from the caller's views instead of evaluating them.
 #pragma once
 #include "common.h"
 
@@ -16,6 +18,7 @@ namespace mpxk {
 template <typename T>
 __global__ void k_linearize(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws) {
   const int t = blockIdx.x, b = blockIdx.y;
+  if (skip_instance(ws, b)) return;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int n = P.n, m = P.m, N = P.N, p = P.p, N1 = N + 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -65,12 +68,20 @@ __global__ void k_linearize(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws) {
       const T* E = p > 0 ? P.E.at(b, t) : nullptr;
       T* rh = ws.rh + ((size_t)b * N + t) * m;
       T* rd = ws.rd + ((size_t)b * N + t) * m;
+      const bool caller = P.model == MODEL_CALLER;
+      const T* rc = caller ? P.r.at(b, t) : nullptr;
+      if (caller && tid == 0) acc[MP_COST] += P.g.at(b, t)[0];
       for (int i = tid; i < m; i += nt) {
-        T Ru = 0;
-        for (int j = 0; j < m; ++j) Ru += R[i * m + j] * (su[j] - ur[j]);
-        T r = Ru;
+        T r;
+        if (caller) {
+          r = rc[i];
+        } else {
+          T Ru = 0;
+          for (int j = 0; j < m; ++j) Ru += R[i * m + j] * (su[j] - ur[j]);
+          r = Ru;
+          acc[MP_COST] += T(0.5) * (su[i] - ur[i]) * Ru;
+        }
         for (int k = 0; k < p; ++k) r += E[k * m + i] * sw[k];
-        acc[MP_COST] += T(0.5) * (su[i] - ur[i]) * Ru;
         T dg = 0;
         if (P.box) {
           const T lo = P.ulo.at(b, t)[i], hi = P.uhi.at(b, t)[i];
@@ -96,15 +107,23 @@ __global__ void k_linearize(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws) {
       T* cd = ws.cdef + ((size_t)b * N1 + t + 1) * n;
       const T* xn = Xb + (size_t)(t + 1) * n;
       const T* vn = Vb + (size_t)(t + 1) * n;
+      const bool caller = P.model == MODEL_CALLER;
       for (int i = tid; i < n; i += nt) {
-        T Qx = 0;
-        for (int j = 0; j < n; ++j) Qx += Q[i * n + j] * (sx[j] - xr[j]);
-        T q = Qx;
+        T q;
+        if (caller) {
+          q = P.q.at(b, t)[i];
+        } else {
+          T Qx = 0;
+          for (int j = 0; j < n; ++j) Qx += Q[i * n + j] * (sx[j] - xr[j]);
+          q = Qx;
+          acc[MP_COST] += T(0.5) * (sx[i] - xr[i]) * Qx;
+        }
         for (int k = 0; k < p; ++k) q += D[k * n + i] * sw[k];
         qh[i] = q;
-        acc[MP_COST] += T(0.5) * (sx[i] - xr[i]) * Qx;
         T f;
-        if (P.model == 1) {
+        if (caller) {
+          f = P.fbar.at(b, t)[i];
+        } else if (P.model == 1) {
           const T* A = P.A.at(b, 0);
           const T* Bm = P.Bm.at(b, 0);
           const T* W = P.W.at(b, 0);
@@ -153,11 +172,17 @@ __global__ void k_linearize(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws) {
     __syncthreads();
     T* qh = ws.qh + ((size_t)b * N1 + N) * n;
     T* c0 = ws.cdef + (size_t)b * N1 * n;
+    const bool caller = P.model == MODEL_CALLER;
+    if (caller && tid == 0) acc[MP_COST] += P.g.at(b, N)[0];
     for (int i = tid; i < n; i += nt) {
-      T Qx = 0;
-      for (int j = 0; j < n; ++j) Qx += QN[i * n + j] * (sx[j] - xr[j]);
-      qh[i] = Qx;
-      acc[MP_COST] += T(0.5) * (sx[i] - xr[i]) * Qx;
+      if (caller) {
+        qh[i] = P.q.at(b, N)[i];
+      } else {
+        T Qx = 0;
+        for (int j = 0; j < n; ++j) Qx += QN[i * n + j] * (sx[j] - xr[j]);
+        qh[i] = Qx;
+        acc[MP_COST] += T(0.5) * (sx[i] - xr[i]) * Qx;
+      }
       const T c = x0[i] - Xb[i];
       c0[i] = c;
       acc[MP_VC] += Vb[i] * c;

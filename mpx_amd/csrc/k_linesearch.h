@@ -99,6 +99,7 @@ template <typename T>
 __global__ void __launch_bounds__(512) k_linesearch(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, LqrIn<T> in, T* o_alpha,
                              T* o_rho, T* o_phi0, T* o_dphi, int* o_status, int* o_trials) {
   const int b = blockIdx.x;
+  if (skip_instance(ws, b)) return;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];

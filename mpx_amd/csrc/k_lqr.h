@@ -27,19 +27,23 @@ struct LqrIn {
 template <typename T>
 struct RicSmem {
   T *S, *A, *T1, *B, *SB, *G, *Kb, *H, *D, *E, *s, *Sc, *c, *rd, *q, *r, *piv, *red;
+  T* Mt;  // M_t' (m x n) of a caller-given cross Hessian, or nullptr
   int ldab;
 };
 
+// hasM: room for the transposed cross Hessian M_t' (m x n) after the rest.
 template <typename T>
-__host__ __device__ inline size_t ric_smem_elems(int n, int m, int p, int nt) {
+__host__ __device__ inline size_t ric_smem_elems(int n, int m, int p, int nt, bool hasM = false) {
   const size_t kb = (size_t)m * (n + 1 > m ? n + 1 : m);  // [K | k], or m x m scratch
   return (size_t)n * n + 2 * (size_t)n * (n + m) + (size_t)m * (n + 1) + kb + (size_t)m * m +
-         (size_t)p * (n + m) + 6 * (size_t)n + 2 * (size_t)m + 128 + 4 * (size_t)nt + 8;
+         (size_t)p * (n + m) + 6 * (size_t)n + 2 * (size_t)m + 128 + 4 * (size_t)nt + 8 +
+         (hasM ? (size_t)m * n : 0);
 }
 
 template <typename T>
-__device__ RicSmem<T> ric_carve(T* base, int n, int m, int p, int nt) {
+__device__ RicSmem<T> ric_carve(T* base, int n, int m, int p, int nt, bool hasM = false) {
   RicSmem<T> L;
+  L.Mt = hasM ? base + ric_smem_elems<T>(n, m, p, nt, false) : nullptr;
   T* q = base;
   L.ldab = n + m;
   L.S = q; q += n * n;
@@ -110,6 +114,14 @@ __device__ void load_stage(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& ws,
       else if (e < tot) L.rd[e - o6] = v[u];
     }
   }
+  if (L.Mt) {
+    // M_t (n x m, row-major) lands transposed: Mt[r][i] = M[i][r]
+    const T* M = P.M.at(b, t);
+    for (int e = tid; e < n * m; e += nt) {
+      const int i = e / m, r = e - i * m;
+      L.Mt[r * n + i] = M[e];
+    }
+  }
 }
 
 template <typename T>
@@ -145,7 +157,8 @@ __device__ void spd_inverse(T* H, T* tmp, int m, T floor) {
 // One backward Riccati step in LDS: (S, s) = V_{t+1} -> V_t; [K | k] in Kb;
 // F_t = A - B K and f_t = c - B k go straight to the workspace.
 //   [T1 | SB] = S [A | B],  Sc = S c + s
-//   H = R + diag(rd) + rho_c E'E + B'SB,  [G | g] = [rho_c E'D + B'T1 | r + B'Sc]
+//   H = R + diag(rd) + rho_c E'E + B'SB,  [G | g] = [M' + rho_c E'D + B'T1 | r + B'Sc]
+//   (M: the caller's cross Hessian, 0 for the device model families)
 //   [K | k] = H^{-1} [G | g]
 //   S_t = Q + rho_c D'D + A'T1 - G'K,  s_t = q + A'Sc - G'k
 // FP32 forms the products on the matrix cores (unless flags bit 0), FP64 on VALU.
@@ -181,7 +194,7 @@ __device__ void riccati_step(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& w
       const MSeg h0{L.B, L.SB, ldab, ldab, n, true, false, 1.f}, h1{L.E, L.E, m, m, p, true, false, 1.f};
       mfma_gemm<false>(m, m, p > 0 ? 2 : 1, h0, h1, h1, L.H, m, R, m, L.rd);
       const MSeg g0{L.B, L.T1, ldab, ldab, n, true, false, 1.f}, g1{L.E, L.D, m, n, p, true, false, 1.f};
-      mfma_gemm<false>(m, n, p > 0 ? 2 : 1, g0, g1, g1, L.G, ld, nullptr, 0, nullptr);
+      mfma_gemm<false>(m, n, p > 0 ? 2 : 1, g0, g1, g1, L.G, ld, L.Mt, n, nullptr);
     }
   }
   if (!mfma) {
@@ -194,7 +207,7 @@ __device__ void riccati_step(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& w
     }
     for (int e = tid; e < m * n; e += nt) {
       const int i = e / n, j = e - i * n;
-      T v = 0;
+      T v = L.Mt ? L.Mt[e] : T(0);
       for (int k = 0; k < p; ++k) v += L.E[k * m + i] * L.D[k * n + j];
       for (int k = 0; k < n; ++k) v += L.B[k * ldab + i] * L.T1[k * ldab + j];
       L.G[i * ld + j] = v;
@@ -361,6 +374,7 @@ __host__ __device__ inline size_t stage_smem_elems(int n, int nt) {
 template <typename T>
 __global__ void __launch_bounds__(256) k_fwd_stage(Prob<T> P, Iter<T> it, Ws<T> ws) {
   const int t = blockIdx.x, b = blockIdx.y;
+  if (skip_instance(ws, b)) return;
   const int n = P.n, m = P.m, N = P.N, N1 = N + 1;
   const int tid = threadIdx.x, nt = blockDim.x;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -417,11 +431,12 @@ __global__ void __launch_bounds__(256) k_fwd_stage(Prob<T> P, Iter<T> it, Ws<T> 
 template <typename T, int MMAX>
 __global__ void __launch_bounds__(512) k_lqr_seq(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, LqrIn<T> in) {
   const int b = blockIdx.x;
+  if (skip_instance(ws, b)) return;
   const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
   const int tid = threadIdx.x, nt = blockDim.x;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* base = reinterpret_cast<T*>(smem_raw);
-  RicSmem<T> L = ric_carve(base, n, m, p, nt);
+  RicSmem<T> L = ric_carve(base, n, m, p, nt, P.M.p != nullptr);
   load_block(L.S, P.QN.at(b, 0), n, n, n);
   for (int i = tid; i < n; i += nt) L.s[i] = ws.qh[((size_t)b * N1 + N) * n + i];
   __syncthreads();

@@ -354,6 +354,7 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
                                                     Ws<float> ws, LqrIn<float> in) {
   (void)it_unused;
   const int b = blockIdx.x;
+  if (skip_instance(ws, b)) return;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
   const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
   __shared__ float tbuf[16 * 17];
@@ -600,6 +601,7 @@ __device__ __forceinline__ void aop_offsets(int (&off)[NT][4], int n, int g, int
 template <int NT, bool CB>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 3))) k_fwd_wave(Prob<float> P, Ws<float> ws, LqrIn<float> in) {
   const int b = blockIdx.x;
+  if (skip_instance(ws, b)) return;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
   const int n = P.n, m = P.m, N = P.N, N1 = N + 1;
   const LaneOff<NT> L = lane_offsets<NT>(n, m, g, c);
@@ -687,6 +689,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 3)))
 template <int NT>
 __global__ void __launch_bounds__(64) k_costate_wave(Prob<float> P, Iter<float> it, Ws<float> ws) {
   const int t = blockIdx.x, b = blockIdx.y;
+  if (skip_instance(ws, b)) return;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
   const int n = P.n, m = P.m, N = P.N, N1 = N + 1;
   const LaneOff<NT> L = lane_offsets<NT>(n, m, g, c);

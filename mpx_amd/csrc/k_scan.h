@@ -295,6 +295,7 @@ __device__ void leaf_combine_up(const Prob<T>& P, const Prm<T>& prm, const Ws<T>
 template <typename T, int MMAX>
 __global__ void __launch_bounds__(512) k_scan_up(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, LqrIn<T> in) {
   const int s = blockIdx.x, b = blockIdx.y;
+  if (skip_instance(ws, b)) return;
   const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* base = reinterpret_cast<T*>(smem_raw);
@@ -320,6 +321,7 @@ __host__ __device__ inline size_t mid_smem_elems(int n, int nt) {
 template <typename T>
 __global__ void __launch_bounds__(512) k_scan_mid(Prob<T> P, Prm<T> prm, Ws<T> ws) {
   const int b = blockIdx.x;
+  if (skip_instance(ws, b)) return;
   const int n = P.n, N = P.N, N1 = N + 1, Pn = prm.P;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int ld = n + 1;
@@ -420,6 +422,7 @@ __global__ void __launch_bounds__(512) k_scan_mid(Prob<T> P, Prm<T> prm, Ws<T> w
 template <typename T, int MMAX>
 __global__ void __launch_bounds__(512) k_scan_down(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, LqrIn<T> in) {
   const int s = blockIdx.x, b = blockIdx.y;
+  if (skip_instance(ws, b)) return;
   const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1, Pn = prm.P;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* base = reinterpret_cast<T*>(smem_raw);
@@ -449,6 +452,7 @@ __host__ __device__ inline size_t fwd_smem_elems(int n, int m, int nt) {
 template <typename T>
 __global__ void __launch_bounds__(512) k_fwd_up(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, LqrIn<T> in) {
   const int s = blockIdx.x, b = blockIdx.y;
+  if (skip_instance(ws, b)) return;
   const int n = P.n, N = P.N, N1 = N + 1, Pn = prm.P;
   const int tid = threadIdx.x, nt = blockDim.x;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -495,6 +499,7 @@ __global__ void __launch_bounds__(512) k_fwd_up(Prob<T> P, Prm<T> prm, Iter<T> i
 template <typename T>
 __global__ void k_fwd_mid(Prob<T> P, Prm<T> prm, Ws<T> ws) {
   const int b = blockIdx.x;
+  if (skip_instance(ws, b)) return;
   const int n = P.n, Pn = prm.P;
   const int tid = threadIdx.x, nt = blockDim.x;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -522,6 +527,7 @@ __global__ void k_fwd_mid(Prob<T> P, Prm<T> prm, Ws<T> ws) {
 template <typename T>
 __global__ void __launch_bounds__(512) k_fwd_down(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, LqrIn<T> in) {
   const int s = blockIdx.x + 1, b = blockIdx.y;
+  if (skip_instance(ws, b)) return;
   const int n = P.n, N = P.N, Pn = prm.P;
   const int tid = threadIdx.x, nt = blockDim.x;
   extern __shared__ __align__(16) unsigned char smem_raw[];

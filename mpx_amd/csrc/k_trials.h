@@ -110,6 +110,7 @@ __device__ __forceinline__ void row_dot2_lds(const T* row, int L, const T* v0, c
 template <typename T>
 __global__ void __launch_bounds__(256) k_ls_setup(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws) {
   const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+  if (skip_instance(ws, b)) return;
   const int m = P.m, N = P.N, N1 = N + 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* sred = reinterpret_cast<T*>(smem_raw);   // 6 * blockDim.x
@@ -197,6 +198,7 @@ __global__ void __launch_bounds__(512) k_trials(Prob<T> P, Prm<T> prm, Iter<T> i
   static_assert(!V4 || IsF32<T>::value, "float4 rows are FP32 only");
   const int wpb = blockDim.x >> 6, wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int b = blockIdx.y, t = blockIdx.x * wpb + wv;
+  if (skip_instance(ws, b)) return;
   const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
   // an earlier pass accepted: the same answer in every wave of the block (one instance)
   if (j0 > 0 && ls_first_accept(prm, ws, b, N1, j0) >= 0) return;
@@ -416,6 +418,7 @@ __global__ void __launch_bounds__(256) k_ls_accept(Prob<T> P, Prm<T> prm, Iter<T
                                                    T* o_rho, T* o_phi0, T* o_dphi, int* o_status, int* o_trials) {
   const int wpb = blockDim.x >> 6, wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int b = blockIdx.y, t = blockIdx.x * wpb + wv;
+  if (skip_instance(ws, b)) return;
   const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
   if (t > N) return;
   const T* ls = ws.lsp + (size_t)b * NLS;

@@ -21,6 +21,7 @@
 #include "k_linesearch.h"
 #include "k_trials.h"
 #include "k_misc.h"
+#include "k_caller.h"
 
 using namespace mpxk;
 
@@ -61,10 +62,14 @@ Prob<T> make_prob(const mpxkkt_problem* pr) {
   P.xref = view<T>(pr->xref); P.uref = view<T>(pr->uref);
   P.D = view<T>(pr->D); P.E = view<T>(pr->E); P.e = view<T>(pr->e);
   P.ulo = view<T>(pr->ulo); P.uhi = view<T>(pr->uhi); P.x0 = view<T>(pr->x0);
+  P.fbar = view<T>(pr->fbar); P.M = view<T>(pr->M); P.q = view<T>(pr->q); P.r = view<T>(pr->r);
+  P.g = view<T>(pr->g);
   return P;
 }
 
 int effective_segments(const mpxkkt_problem* pr, const mpxkkt_params* prm) {
+  // the caller's linearisation (cross term M) runs the sequential Riccati recursion
+  if (pr->model == MPXKKT_MODEL_CALLER) return 1;
   int P = prm->segments < 1 ? 1 : prm->segments;
   const int maxP = (pr->horizon + 1) / 2;  // at least two elements per segment
   if (P > maxP) P = maxP < 1 ? 1 : maxP;
@@ -76,7 +81,7 @@ int effective_segments(const mpxkkt_problem* pr, const mpxkkt_params* prm) {
 // them off); they never form the closed-loop maps F_t = A_t - B_t K_t.
 bool wave_path(const mpxkkt_problem* pr, int flags, int P) {
   return pr->dtype == MPXKKT_F32 && (flags & 3) == 0 && P <= 1 && pr->n + 1 <= 48 && pr->m <= 16 &&
-         pr->n_eq <= 16;
+         pr->n_eq <= 16 && pr->model != MPXKKT_MODEL_CALLER;
 }
 
 template <typename T>
@@ -115,6 +120,10 @@ size_t carve(const mpxkkt_problem* pr, int P, bool wave, Ws<T>* ws, char* base) 
   ws->lsp = take(B * NLS);
   ws->tpart = take(B * N1 * 2 * MAX_TRIALS);
   ws->hq = take(B * N * 2 * (size_t)pr->n_eq);
+  ws->gpart = take(B * N1 * 2);
+  ws->last = take(B);
+  ws->run = reinterpret_cast<int*>(take(B));
+  ws->nrun = reinterpret_cast<int*>(take(1));
   return off + 256;
 }
 
@@ -139,7 +148,7 @@ mpxkkt_status check_view(const mpxkkt_view& v, const char* name, bool needed) {
 mpxkkt_status check_problem(const mpxkkt_problem* pr) {
   if (!pr) return fail(MPXKKT_ERR_ARG, "null problem");
   if (pr->dtype != MPXKKT_F32 && pr->dtype != MPXKKT_F64) return fail(MPXKKT_ERR_ARG, "bad dtype");
-  if (pr->model != MPXKKT_MODEL_LTV && pr->model != MPXKKT_MODEL_TANH)
+  if (pr->model != MPXKKT_MODEL_LTV && pr->model != MPXKKT_MODEL_TANH && pr->model != MPXKKT_MODEL_CALLER)
     return fail(MPXKKT_ERR_ARG, "bad model");
   if (pr->batch < 1 || pr->batch > 65535) return fail(MPXKKT_ERR_ARG, "batch must be in [1, 65535]");
   if (pr->n < 1 || pr->m < 1 || pr->horizon < 1 || pr->n_eq < 0)
@@ -151,11 +160,19 @@ mpxkkt_status check_problem(const mpxkkt_problem* pr) {
   if ((s = check_view(pr->A, "A", true)) != MPXKKT_OK) return s;
   if ((s = check_view(pr->B, "B", true)) != MPXKKT_OK) return s;
   if ((s = check_view(pr->W, "W", pr->model == MPXKKT_MODEL_TANH)) != MPXKKT_OK) return s;
+  const bool caller = pr->model == MPXKKT_MODEL_CALLER;
   if ((s = check_view(pr->Q, "Q", true)) != MPXKKT_OK) return s;
   if ((s = check_view(pr->R, "R", true)) != MPXKKT_OK) return s;
   if ((s = check_view(pr->QN, "QN", true)) != MPXKKT_OK) return s;
-  if ((s = check_view(pr->xref, "xref", true)) != MPXKKT_OK) return s;
-  if ((s = check_view(pr->uref, "uref", true)) != MPXKKT_OK) return s;
+  if ((s = check_view(pr->xref, "xref", !caller)) != MPXKKT_OK) return s;
+  if ((s = check_view(pr->uref, "uref", !caller)) != MPXKKT_OK) return s;
+  if ((s = check_view(pr->fbar, "fbar", caller)) != MPXKKT_OK) return s;
+  if ((s = check_view(pr->M, "M", false)) != MPXKKT_OK) return s;
+  if (!caller && (pr->M.ptr || pr->fbar.ptr || pr->q.ptr || pr->r.ptr || pr->g.ptr))
+    return fail(MPXKKT_ERR_ARG, "fbar, M, q, r, g are views of MPXKKT_MODEL_CALLER only");
+  if ((s = check_view(pr->q, "q", caller)) != MPXKKT_OK) return s;
+  if ((s = check_view(pr->r, "r", caller)) != MPXKKT_OK) return s;
+  if ((s = check_view(pr->g, "g", caller)) != MPXKKT_OK) return s;
   if ((s = check_view(pr->D, "D", eq)) != MPXKKT_OK) return s;
   if ((s = check_view(pr->E, "E", eq)) != MPXKKT_OK) return s;
   if ((s = check_view(pr->e, "e", eq)) != MPXKKT_OK) return s;
@@ -169,6 +186,15 @@ mpxkkt_status check_iterate(const mpxkkt_problem* pr, const mpxkkt_iterate* it) 
   if (!it || !it->X || !it->U || !it->V) return fail(MPXKKT_ERR_ARG, "null iterate array");
   if (pr->n_eq > 0 && !it->Y_eq) return fail(MPXKKT_ERR_ARG, "Y_eq required when n_eq > 0");
   if (pr->has_box && !it->Y_box) return fail(MPXKKT_ERR_ARG, "Y_box required when has_box");
+  return MPXKKT_OK;
+}
+
+mpxkkt_status check_params(const mpxkkt_params* prm) {
+  if (prm->max_trials < 1 || prm->max_trials > MAX_TRIALS)
+    return fail(MPXKKT_ERR_ARG, "max_trials must be in [1, 16]");
+  if (!(prm->beta > 0.0 && prm->beta < 1.0) || !(prm->tau > 0.0 && prm->tau < 1.0) || !(prm->mu_b > 0.0) ||
+      !(prm->rho_c >= 0.0) || !(prm->hess_floor >= 0.0 && prm->hess_floor < 1.0))
+    return fail(MPXKKT_ERR_ARG, "line-search / barrier parameters out of range");
   return MPXKKT_OK;
 }
 
@@ -342,7 +368,8 @@ mpxkkt_status run_fwd_stage(hipStream_t st, const mpxkkt_problem* pr, const Prob
 template <typename T>
 mpxkkt_status run_linearize(hipStream_t st, const mpxkkt_problem* pr, const Prob<T>& P, const Prm<T>& R,
                             const Iter<T>& I, const Ws<T>& W) {
-  if ((R.flags & 8) != 0 && pr->n <= 64 && pr->m <= 64 && pr->n_eq <= 64) {
+  const bool caller = pr->model == MPXKKT_MODEL_CALLER;
+  if (!caller && (R.flags & 8) != 0 && pr->n <= 64 && pr->m <= 64 && pr->n_eq <= 64) {
     const InstLds L = inst_lds(pr->n, pr->m, pr->horizon, pr->model == MPXKKT_MODEL_TANH, pr->Q.stage_stride == 0,
                                pr->R.stage_stride == 0, false);
     const size_t lds =
@@ -356,7 +383,7 @@ mpxkkt_status run_linearize(hipStream_t st, const mpxkkt_problem* pr, const Prob
       return launch_check("k_linearize_stage");
     }
   }
-  if (use_inst<T>(pr, false)) {
+  if (!caller && use_inst<T>(pr, false)) {
     const InstLds L = inst_lds(pr->n, pr->m, pr->horizon, pr->model == MPXKKT_MODEL_TANH, pr->Q.stage_stride == 0,
                                pr->R.stage_stride == 0, false);
     const size_t lds = (size_t)L.total * sizeof(T);
@@ -410,7 +437,7 @@ mpxkkt_status run_lqr_m(hipStream_t st, const mpxkkt_problem* pr, int nt, const 
   } else if (use_wave_lqr<T>(pr, R)) {
     return run_lqr_wave(st, pr, P, R, I, W, in);
   } else {
-    const size_t lds = ric_smem_elems<T>(pr->n, pr->m, pr->n_eq, nt) * sizeof(T);
+    const size_t lds = ric_smem_elems<T>(pr->n, pr->m, pr->n_eq, nt, pr->M.ptr != nullptr) * sizeof(T);
     if ((s = prep_lds(k_lqr_seq<T, MMAX>, lds)) != MPXKKT_OK) return s;
     hipLaunchKernelGGL((k_lqr_seq<T, MMAX>), dim3(pr->batch), dim3(nt), lds, st, P, R, I, W, in);
     if ((s = launch_check("k_lqr_seq")) != MPXKKT_OK) return s;
@@ -567,12 +594,16 @@ void prof_mark(hipStream_t st, const PhaseEvents* pe, int k) {
   if (pe) (void)hipEventRecord(pe->ev[k], st);
 }
 
+// gated: instances whose run flag (workspace) is 0 skip every kernel -- the
+// multi-iteration solve's converged instances; plain iterations run them all.
 template <typename T>
 mpxkkt_status sqp_iterate_t(hipStream_t st, const mpxkkt_problem* pr, const mpxkkt_params* prm,
-                            mpxkkt_iterate* it, mpxkkt_stats* so, void* wsp, bool profile_ok = true) {
+                            mpxkkt_iterate* it, mpxkkt_stats* so, void* wsp, bool profile_ok = true,
+                            bool gated = false) {
   const int Pseg = effective_segments(pr, prm);
   Ws<T> W;
   carve<T>(pr, Pseg, wave_path(pr, prm->flags, Pseg), &W, static_cast<char*>(wsp));
+  if (!gated) W.run = nullptr;
   const Prob<T> P = make_prob<T>(pr);
   const Prm<T> R = make_prm<T>(prm, Pseg);
   const Iter<T> I = make_iter<T>(it);
@@ -626,6 +657,101 @@ mpxkkt_status kkt_t(hipStream_t st, const mpxkkt_problem* pr, const mpxkkt_param
   return launch_check("k_kkt_reduce");
 }
 
+// Caller-dynamics iteration, first half: K1-K3 and the line-search set-up on
+// the caller's linearisation, then the step to the caller.
+template <typename T>
+mpxkkt_status direction_t(hipStream_t st, const mpxkkt_problem* pr, const mpxkkt_params* prm,
+                          mpxkkt_iterate* it, mpxkkt_direction* dir, void* wsp) {
+  Ws<T> W;
+  carve<T>(pr, 1, false, &W, static_cast<char*>(wsp));
+  W.run = nullptr;
+  const Prob<T> P = make_prob<T>(pr);
+  const Prm<T> R = make_prm<T>(prm, 1);
+  const Iter<T> I = make_iter<T>(it);
+  const LqrIn<T> in = make_lqr_in<T>(pr, W);
+  mpxkkt_status s;
+  if ((s = run_linearize<T>(st, pr, P, R, I, W)) != MPXKKT_OK) return s;
+  if ((s = run_lqr<T>(st, pr, prm, P, R, I, W, in)) != MPXKKT_OK) return s;
+  if ((s = run_fwd_stage<T>(st, pr, P, R, I, W, in)) != MPXKKT_OK) return s;
+  hipLaunchKernelGGL(k_ls_setup<T>, dim3(pr->batch), dim3(256), (size_t)6 * 256 * sizeof(T), st, P, R, I, W);
+  if ((s = launch_check("k_ls_setup")) != MPXKKT_OK) return s;
+  hipLaunchKernelGGL(k_export_dir<T>, dim3(pr->batch), dim3(256), 0, st, P, W, static_cast<T*>(dir->dX),
+                     static_cast<T*>(dir->dU), static_cast<T*>(dir->alpha_max));
+  return launch_check("k_export_dir");
+}
+
+// Second half: the trials' merits from the caller's evaluations, Armijo
+// choice, multiplier and iterate update.
+template <typename T>
+mpxkkt_status accept_t(hipStream_t st, const mpxkkt_problem* pr, const mpxkkt_params* prm, mpxkkt_iterate* it,
+                       int L, const void* tg, const void* tf, mpxkkt_stats* so, void* wsp) {
+  Ws<T> W;
+  carve<T>(pr, 1, false, &W, static_cast<char*>(wsp));
+  W.run = nullptr;
+  const Prob<T> P = make_prob<T>(pr);
+  Prm<T> R = make_prm<T>(prm, 1);
+  R.max_trials = L;
+  R.group = L;
+  const Iter<T> I = make_iter<T>(it);
+  const int B = pr->batch, N1 = pr->horizon + 1;
+  const dim3 grid((N1 + kTrialWaves - 1) / kTrialWaves, B);
+  hipLaunchKernelGGL(k_trials_caller<T>, grid, dim3(64 * kTrialWaves), 0, st, P, R, I, W, L,
+                     static_cast<const T*>(tg), static_cast<const T*>(tf));
+  mpxkkt_status s = launch_check("k_trials_caller");
+  if (s != MPXKKT_OK) return s;
+  T* a = so ? static_cast<T*>(so->alpha) : nullptr;
+  T* r = so ? static_cast<T*>(so->rho) : nullptr;
+  T* f0 = so ? static_cast<T*>(so->merit0) : nullptr;
+  T* df = so ? static_cast<T*>(so->dmerit) : nullptr;
+  int* stt = so ? so->status : nullptr;
+  int* tr = so ? so->trials : nullptr;
+  hipLaunchKernelGGL(k_ls_accept<T>, grid, dim3(64 * kTrialWaves), 0, st, P, R, I, W, a, r, f0, df, stt, tr);
+  return launch_check("k_ls_accept");
+}
+
+// SQP iterations until runOffline's stop rule (mpx/utils/mpc_wrapper.py:425-467)
+// holds for every instance: after each iteration the per-stage (g_t, |c|^2)
+// partials and the per-instance stop test; converged instances skip the
+// kernels of later iterations.
+template <typename T>
+mpxkkt_status solve_t(hipStream_t st, const mpxkkt_problem* pr, const mpxkkt_params* prm,
+                      const mpxkkt_solve_opts* o, mpxkkt_iterate* it, mpxkkt_stats* so, int32_t* iters,
+                      void* cost, void* cc, void* wsp, int32_t* launched) {
+  Ws<T> W;
+  const int Pseg = effective_segments(pr, prm);
+  carve<T>(pr, Pseg, wave_path(pr, prm->flags, Pseg), &W, static_cast<char*>(wsp));
+  const Prob<T> P = make_prob<T>(pr);
+  const Iter<T> I = make_iter<T>(it);
+  const int B = pr->batch, N1 = pr->horizon + 1;
+  const dim3 gB((B + 255) / 256);
+  hipLaunchKernelGGL(k_solve_init<T>, gB, dim3(256), 0, st, B, W, iters);
+  mpxkkt_status s = launch_check("k_solve_init");
+  if (s != MPXKKT_OK) return s;
+  const int nt = 64;
+  const int total = o->max_iter + 1;
+  int done = 0;
+  for (int i = 0; i < total; ++i) {
+    if ((s = sqp_iterate_t<T>(st, pr, prm, it, so, wsp, true, true)) != MPXKKT_OK) return s;
+    hipLaunchKernelGGL(k_eval_partials<T>, dim3(N1, B), dim3(nt), 2 * nt * sizeof(T), st, P, I, W);
+    if ((s = launch_check("k_eval_partials")) != MPXKKT_OK) return s;
+    hipLaunchKernelGGL(k_stop<T>, gB, dim3(256), 0, st, B, N1, W, o->max_iter, (T)o->dcost_tol, (T)o->cc_tol,
+                       iters, static_cast<T*>(cost), static_cast<T*>(cc));
+    if ((s = launch_check("k_stop")) != MPXKKT_OK) return s;
+    done = i + 1;
+    if (o->check_every > 0 && done % o->check_every == 0 && done < total) {
+      hipLaunchKernelGGL(k_count_running<T>, dim3(1), dim3(256), 0, st, B, W);
+      if ((s = launch_check("k_count_running")) != MPXKKT_OK) return s;
+      int running = 0;
+      if (hipMemcpyAsync(&running, W.nrun, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
+        return fail(MPXKKT_ERR_DEVICE, "reading the running-instance count failed");
+      if (running == 0) break;
+    }
+  }
+  if (launched) *launched = done;
+  return MPXKKT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -664,18 +790,85 @@ mpxkkt_status mpxkkt_sqp_iterate(void* stream, const mpxkkt_problem* prob, const
   mpxkkt_status s = check_problem(prob);
   if (s != MPXKKT_OK) return s;
   if (!prm) return fail(MPXKKT_ERR_ARG, "null params");
+  if (prob->model == MPXKKT_MODEL_CALLER)
+    return fail(MPXKKT_ERR_ARG, "caller dynamics: use mpxkkt_sqp_direction and mpxkkt_sqp_accept");
   if ((s = check_iterate(prob, it)) != MPXKKT_OK) return s;
-  if (prm->max_trials < 1 || prm->max_trials > MAX_TRIALS)
-    return fail(MPXKKT_ERR_ARG, "max_trials must be in [1, 16]");
-  if (!(prm->beta > 0.0 && prm->beta < 1.0) || !(prm->tau > 0.0 && prm->tau < 1.0) || !(prm->mu_b > 0.0) ||
-      !(prm->rho_c >= 0.0) || !(prm->hess_floor >= 0.0 && prm->hess_floor < 1.0))
-    return fail(MPXKKT_ERR_ARG, "line-search / barrier parameters out of range");
+  if ((s = check_params(prm)) != MPXKKT_OK) return s;
   const size_t need = workspace_bytes(prob, prm);
   if (!workspace || workspace_bytes_ < need)
     return fail(MPXKKT_ERR_WORKSPACE, "workspace needs " + std::to_string(need) + " bytes");
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (prob->dtype == MPXKKT_F64) return sqp_iterate_t<double>(st, prob, prm, it, stats, workspace);
   return sqp_iterate_t<float>(st, prob, prm, it, stats, workspace);
+}
+
+mpxkkt_status mpxkkt_sqp_direction(void* stream, const mpxkkt_problem* prob, const mpxkkt_params* prm,
+                                   mpxkkt_iterate* it, mpxkkt_direction* dir, void* workspace,
+                                   size_t workspace_bytes_) {
+  mpxkkt_status s = check_problem(prob);
+  if (s != MPXKKT_OK) return s;
+  if (prob->model != MPXKKT_MODEL_CALLER)
+    return fail(MPXKKT_ERR_ARG, "mpxkkt_sqp_direction takes MPXKKT_MODEL_CALLER problems");
+  if (!prm || !dir || !dir->dX || !dir->dU || !dir->alpha_max)
+    return fail(MPXKKT_ERR_ARG, "null params or direction output");
+  if ((s = check_iterate(prob, it)) != MPXKKT_OK) return s;
+  if ((s = check_params(prm)) != MPXKKT_OK) return s;
+  const size_t need = workspace_bytes(prob, prm);
+  if (!workspace || workspace_bytes_ < need)
+    return fail(MPXKKT_ERR_WORKSPACE, "workspace needs " + std::to_string(need) + " bytes");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (prob->dtype == MPXKKT_F64) return direction_t<double>(st, prob, prm, it, dir, workspace);
+  return direction_t<float>(st, prob, prm, it, dir, workspace);
+}
+
+mpxkkt_status mpxkkt_sqp_accept(void* stream, const mpxkkt_problem* prob, const mpxkkt_params* prm,
+                                mpxkkt_iterate* it, int32_t n_trials, const void* trial_g, const void* trial_f,
+                                mpxkkt_stats* stats, void* workspace, size_t workspace_bytes_) {
+  mpxkkt_status s = check_problem(prob);
+  if (s != MPXKKT_OK) return s;
+  if (prob->model != MPXKKT_MODEL_CALLER)
+    return fail(MPXKKT_ERR_ARG, "mpxkkt_sqp_accept takes MPXKKT_MODEL_CALLER problems");
+  if (!prm || !trial_g || !trial_f) return fail(MPXKKT_ERR_ARG, "null params or trial evaluations");
+  if (n_trials < 1 || n_trials > MAX_TRIALS) return fail(MPXKKT_ERR_ARG, "n_trials must be in [1, 16]");
+  if ((s = check_iterate(prob, it)) != MPXKKT_OK) return s;
+  if ((s = check_params(prm)) != MPXKKT_OK) return s;
+  const size_t need = workspace_bytes(prob, prm);
+  if (!workspace || workspace_bytes_ < need)
+    return fail(MPXKKT_ERR_WORKSPACE, "workspace needs " + std::to_string(need) + " bytes");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (prob->dtype == MPXKKT_F64)
+    return accept_t<double>(st, prob, prm, it, n_trials, trial_g, trial_f, stats, workspace);
+  return accept_t<float>(st, prob, prm, it, n_trials, trial_g, trial_f, stats, workspace);
+}
+
+mpxkkt_status mpxkkt_default_solve_opts(mpxkkt_solve_opts* out) {
+  if (!out) return fail(MPXKKT_ERR_ARG, "null options");
+  out->max_iter = 100;
+  out->check_every = 0;
+  out->dcost_tol = 1e-3;
+  out->cc_tol = 1e-5;
+  return MPXKKT_OK;
+}
+
+mpxkkt_status mpxkkt_solve(void* stream, const mpxkkt_problem* prob, const mpxkkt_params* prm,
+                           const mpxkkt_solve_opts* opts, mpxkkt_iterate* it, mpxkkt_stats* stats,
+                           int32_t* iters, void* cost, void* cc, void* workspace, size_t workspace_bytes_,
+                           int32_t* launched) {
+  mpxkkt_status s = check_problem(prob);
+  if (s != MPXKKT_OK) return s;
+  if (!prm || !opts || !iters) return fail(MPXKKT_ERR_ARG, "null params, options or iteration counts");
+  if (prob->model == MPXKKT_MODEL_CALLER)
+    return fail(MPXKKT_ERR_ARG, "caller dynamics: the solve loop needs the device model families");
+  if ((s = check_iterate(prob, it)) != MPXKKT_OK) return s;
+  if (opts->max_iter < 0 || opts->max_iter > 100000 || opts->check_every < 0)
+    return fail(MPXKKT_ERR_ARG, "max_iter must be in [0, 100000] and check_every >= 0");
+  if ((s = check_params(prm)) != MPXKKT_OK) return s;
+  const size_t need = workspace_bytes(prob, prm);
+  if (!workspace || workspace_bytes_ < need)
+    return fail(MPXKKT_ERR_WORKSPACE, "workspace needs " + std::to_string(need) + " bytes");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (prob->dtype == MPXKKT_F64) return solve_t<double>(st, prob, prm, opts, it, stats, iters, cost, cc, workspace, launched);
+  return solve_t<float>(st, prob, prm, opts, it, stats, iters, cost, cc, workspace, launched);
 }
 
 mpxkkt_status mpxkkt_kkt_residual(void* stream, const mpxkkt_problem* prob, const mpxkkt_params* prm,

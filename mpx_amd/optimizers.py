@@ -10,17 +10,31 @@ The reference imports ``mpx.primal_dual_ilqr.primal_dual_ilqr.optimizers``
   (mpx/utils/mpc_wrapper.py:422).
 
 Here the same calls run one batched primal-dual SQP iteration of the HIP
-library.  ``dynamics`` is a device model family (:mod:`mpx_amd.models`);
-``cost`` is a :class:`~mpx_amd.models.TrackingCost` or None, in which case
-``W`` supplies Q and R as its diagonal blocks (the reference's weight matrix);
-``reference`` is the state reference (N+1, n) (or a (x_ref, u_ref) pair);
-``hessian_approx`` and ``parameter`` are accepted for signature
-compatibility (the Gauss-Newton Hessian of the tracking cost is exact; stage
-parameters are carried by the model family).  A leading batch axis on x0 /
-X_in / U_in / V_in batches the call, as ``jax.vmap`` does at
-mpx/utils/mpc_wrapper_srbd.py:77.  As in the reference, outputs are fresh
-arrays and numerical failure shows as NaN in U[..., 0, 0], never as an
-exception.
+library, in one of two forms:
+
+* ``dynamics`` a device model family (:mod:`mpx_amd.models`): dynamics and
+  cost are evaluated inside the kernels.  ``cost`` is a
+  :class:`~mpx_amd.models.TrackingCost` or None, in which case ``W`` supplies
+  Q and R as its diagonal blocks (the reference's weight matrix);
+  ``reference`` is the state reference (N+1, n) (or a (x_ref, u_ref) pair).
+  The tracking cost's Gauss-Newton Hessian is exact, so ``hessian_approx``
+  must be None or the cost's own; stage parameters are carried by the model
+  family, so ``parameter`` must be None.
+* ``dynamics`` a plain callable ``dynamics(x, u, t, parameter)`` with
+  ``cost(W, reference, x, u, t)`` and ``hessian_approx(W, reference, x, u,
+  t) -> (Q, R, M)`` -- the reference's callables, unchanged
+  (mpx/utils/models.py:6, mpx/utils/objectives.py:15, :75).  They are
+  evaluated on the host (numpy); the iteration runs on the device in the
+  caller-dynamics form (:mod:`mpx_amd.caller`).  The reference takes f_x, f_u
+  and the cost gradient from JAX; here pass ``dynamics_jac(x, u, t,
+  parameter) -> (f_x, f_u)`` and ``cost_grad(W, reference, x, u, t) -> (q,
+  r)``, or leave them out for central differences (step 1e-6 (1 + |z_i|)).
+
+A leading batch axis on x0 / X_in / U_in / V_in batches the call, as
+``jax.vmap`` does at mpx/utils/mpc_wrapper_srbd.py:77 (in the callable form
+``W``, ``reference`` and ``parameter`` then carry the batch axis too).  As in
+the reference, outputs are fresh arrays and numerical failure shows as NaN in
+U[..., 0, 0], never as an exception.
 
 Optional keyword ``constraints`` adds equality rows and boxes, whose
 multipliers ``V_eq`` (N, p) and ``V_ineq`` (N, 2m) are then returned as well
@@ -77,11 +91,124 @@ def _instances(cost, dynamics, W, reference, x0, constraints):
     return insts, batched
 
 
+def _central(fn, z, h=1e-6):
+    """Central-difference Jacobian of fn at z (columns: d fn / d z_j)."""
+    z = np.asarray(z, dtype=np.float64)
+    cols = []
+    for j in range(z.size):
+        e = np.zeros_like(z)
+        e[j] = h * (1.0 + abs(z[j]))
+        cols.append((np.asarray(fn(z + e), np.float64) - np.asarray(fn(z - e), np.float64)) / (2.0 * e[j]))
+    return np.stack(cols, axis=-1)
+
+
+def _callable_iteration(cost, dynamics, hessian_approx, reference, parameter, W, x0, X_in, U_in, V_in,
+                        constraints, V_eq, V_ineq, dtype, dynamics_jac, cost_grad, tune):
+    from .caller import CallerBatch, CallerSolver
+    x0 = _as_np(x0)
+    batched = x0.ndim == 2
+    X, U, V = _as_np(X_in), _as_np(U_in), _as_np(V_in)
+    if not batched:
+        x0, X, U, V = x0[None], X[None], U[None], V[None]
+    B, N1, n = X.shape
+    N, m = U.shape[1], U.shape[2]
+    per = lambda a, b: (a[b] if (batched and a is not None and np.ndim(a) > 0 and np.shape(a)[0] == B
+                                 and not np.isscalar(a)) else a)
+    c = constraints
+    D = None if c is None or c.D is None else np.broadcast_to(np.asarray(c.D, np.float64), (B,) + np.shape(c.D)[-3:])
+    E = None if D is None else np.broadcast_to(np.asarray(c.E, np.float64), (B,) + np.shape(c.E)[-3:])
+    e = None if D is None else (np.zeros(D.shape[:3]) if c.e is None else
+                                np.broadcast_to(np.asarray(c.e, np.float64), D.shape[:3]))
+    box = c is not None and c.u_lo is not None
+    ulo = np.broadcast_to(np.asarray(c.u_lo, np.float64), (B, m)) if box else None
+    uhi = np.broadcast_to(np.asarray(c.u_hi, np.float64), (B, m)) if box else None
+    if box:
+        # linearise where the library iterates: U moved inside the box (include/mpx_kkt.h)
+        w = 1e-3 * (uhi - ulo)
+        U = np.where(U >= uhi[:, None], (uhi - w)[:, None], U)
+        U = np.where(U <= ulo[:, None], (ulo + w)[:, None], U)
+    batch = CallerBatch(B, n, m, N, x0, D=D, E=E, e=e, ulo=ulo, uhi=uhi, dtype=dtype)
+    tune = dict(tune)
+    solver = CallerSolver(batch, **tune)
+    lin = dict(A=np.zeros((B, N, n, n)), B=np.zeros((B, N, n, m)), fbar=np.zeros((B, N, n)),
+               Q=np.zeros((B, N, n, n)), R=np.zeros((B, N, m, m)), M=np.zeros((B, N, n, m)),
+               q=np.zeros((B, N + 1, n)), r=np.zeros((B, N, m)), g=np.zeros((B, N + 1, 1)),
+               QN=np.zeros((B, n, n)))
+    zm = np.zeros(m)
+    fns = []
+    for b in range(B):
+        Wb, refb, parb = per(W, b), per(reference, b), per(parameter, b)
+        f = (lambda Wb_, parb_: lambda x, u, t: np.asarray(dynamics(x, u, t, parb_), np.float64))(Wb, parb)
+        l = (lambda Wb_, refb_: lambda x, u, t: float(cost(Wb_, refb_, x, u, t)))(Wb, refb)
+        fns.append((f, l))
+        for t in range(N1):
+            x = X[b, t]
+            u = U[b, t] if t < N else zm
+            l_t = lambda x_, u_, t_=t: l(x_, u_, t_)
+            if cost_grad is not None:
+                q, r = cost_grad(Wb, refb, x, u, t)
+            else:
+                q = _central(lambda z: l_t(z, u), x)
+                r = _central(lambda z: l_t(x, z), u)
+            Qh, Rh, Mh = (np.asarray(a, np.float64) for a in hessian_approx(Wb, refb, x, u, t))
+            lin["q"][b, t], lin["g"][b, t, 0] = q, l_t(x, u)
+            if t == N:
+                lin["QN"][b] = Qh
+                continue
+            lin["r"][b, t], lin["Q"][b, t], lin["R"][b, t], lin["M"][b, t] = r, Qh, Rh, Mh
+            lin["fbar"][b, t] = f(x, u, t)
+            if dynamics_jac is not None:
+                fx, fu = dynamics_jac(x, u, t, parb)
+            else:
+                fx = _central(lambda z: f(z, u, t), x)
+                fu = _central(lambda z: f(x, z, t), u)
+            lin["A"][b, t], lin["B"][b, t] = fx, fu
+    batch.set_linearization(**lin)
+    Ye = None if V_eq is None else (_as_np(V_eq) if batched else _as_np(V_eq)[None])
+    Yb = None if V_ineq is None else (_as_np(V_ineq) if batched else _as_np(V_ineq)[None])
+    it = batch.iterate_from(X, U, V, Ye, Yb)
+    dX, dU, _ = solver.direction(it)
+    steps = solver.trial_steps().double().cpu().numpy()
+    dX, dU = dX.double().cpu().numpy(), dU.double().cpu().numpy()
+    Ltr = steps.shape[1]
+    tg = np.zeros((B, Ltr, N1))
+    tf = np.zeros((B, Ltr, N, n))
+    for b, (f, l) in enumerate(fns):
+        for j in range(Ltr):
+            xs, us = X[b] + steps[b, j] * dX[b], U[b] + steps[b, j] * dU[b]
+            for t in range(N):
+                tg[b, j, t] = l(xs[t], us[t], t)
+                tf[b, j, t] = f(xs[t], us[t], t)
+            tg[b, j, N] = l(xs[N], zm, N)
+    solver.accept(it, tg, tf)
+    out = [it[k].double().cpu().numpy() for k in ("X", "U", "V", "Yeq", "Ybox")]
+    if not batched:
+        out = [o[0] for o in out]
+    if constraints is not None:
+        return out[0], out[1], out[2], out[3], out[4]
+    return out[0], out[1], out[2]
+
+
 def mpc(cost, dynamics, hessian_approx, limited_memory, reference, parameter, W, x0, X_in, U_in, V_in,
-        constraints=None, V_eq=None, V_ineq=None, dtype=torch.float32, **tune):
+        constraints=None, V_eq=None, V_ineq=None, dtype=torch.float32, dynamics_jac=None, cost_grad=None,
+        **tune):
     """One primal-dual SQP iteration; returns (X, U, V) (plus (V_eq, V_ineq)
     when ``constraints`` is given), as numpy arrays."""
-    del hessian_approx, parameter
+    tune = dict(tune)
+    tune.setdefault("limited_memory", 1 if limited_memory else 0)
+    if callable(dynamics) and not hasattr(dynamics, "model"):
+        if hessian_approx is None or not callable(cost):
+            raise ValueError("callable dynamics need a callable cost and hessian_approx "
+                             "(the reference's objectives.py:15 / :75 contracts)")
+        return _callable_iteration(cost, dynamics, hessian_approx, reference, parameter, W, x0, X_in, U_in,
+                                   V_in, constraints, V_eq, V_ineq, dtype, dynamics_jac, cost_grad, tune)
+    if parameter is not None:
+        raise ValueError("stage parameters belong to the model family here; pass parameter=None "
+                         "or use callable dynamics(x, u, t, parameter)")
+    if hessian_approx is not None and not (cost is not None and getattr(hessian_approx, "__self__", None) is cost):
+        raise ValueError("the device model families use the tracking cost's exact Gauss-Newton Hessian; "
+                         "pass hessian_approx=None (or cost.hessian_approx), or use callable dynamics and cost "
+                         "to supply your own Hessian")
     insts, batched = _instances(cost, dynamics, W, reference, x0, constraints)
     batch = DeviceBatch.from_instances(insts, dtype=dtype)
     X, U, V = _as_np(X_in), _as_np(U_in), _as_np(V_in)
@@ -90,8 +217,6 @@ def mpc(cost, dynamics, hessian_approx, limited_memory, reference, parameter, W,
     Ye = None if V_eq is None else (_as_np(V_eq) if batched else _as_np(V_eq)[None])
     Yb = None if V_ineq is None else (_as_np(V_ineq) if batched else _as_np(V_ineq)[None])
     it = batch.iterate_from(X, U, V, Ye, Yb)
-    tune = dict(tune)
-    tune.setdefault("limited_memory", 1 if limited_memory else 0)
     solver = SQPSolver(batch, **tune)
     solver.iterate(it)
     out = [it[k].double().cpu().numpy() for k in ("X", "U", "V", "Yeq", "Ybox")]

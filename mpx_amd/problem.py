@@ -225,6 +225,32 @@ class SQPSolver:
                                             C.c_void_p(self.ws.data_ptr()), self.ws_bytes))
         return self.stats_t
 
+    def solve(self, it, max_iter=100, dcost_tol=1e-3, cc_tol=1e-5, check_every=0):
+        """SQP iterations in place on ``it`` until runOffline's stop rule holds
+        for every instance (mpx/utils/mpc_wrapper.py:425-467).  Returns
+        (iters [B] int32, cost [B], cc [B], iterations enqueued): the
+        iterations each instance ran, sum g_t^2 and sum |c|^2 after its last
+        one.  ``check_every`` = 0 enqueues max_iter + 1 gated iterations with
+        no host synchronisation; k > 0 reads the running count every k."""
+        b = self.batch
+        iters = torch.zeros(b.B, dtype=torch.int32, device=b.device)
+        cost = torch.zeros(b.B, dtype=b.dtype, device=b.device)
+        cc = torch.zeros(b.B, dtype=b.dtype, device=b.device)
+        opts = L.default_solve_opts(max_iter=int(max_iter), dcost_tol=float(dcost_tol),
+                                    cc_tol=float(cc_tol), check_every=int(check_every))
+        launched = C.c_int32(0)
+        ci = b.c_iterate(it)
+        L.check(self.lib.mpxkkt_solve(_stream(), C.byref(b.problem), C.byref(self.params), C.byref(opts),
+                                      C.byref(ci), C.byref(self._stats), _ptr(iters), _ptr(cost), _ptr(cc),
+                                      C.c_void_p(self.ws.data_ptr()), self.ws_bytes, C.byref(launched)))
+        return iters, cost, cc, launched.value
+
+    def capture_solve(self, it, max_iter=100, dcost_tol=1e-3, cc_tol=1e-5):
+        """Record the whole solve (max_iter + 1 gated iterations and their stop
+        tests, no host synchronisation) as one HIP graph on the iterate ``it``;
+        ``replay()`` re-runs it from whatever ``it`` then holds."""
+        return SolveGraph(self, it, max_iter, dcost_tol, cc_tol)
+
     def kkt(self, it):
         out = torch.zeros(self.batch.B, dtype=self.batch.dtype, device=self.batch.device)
         ci = self.batch.c_iterate(it)
@@ -285,3 +311,39 @@ class SQPGraph:
     def replay(self):
         self.graph.replay()
         return self.stats
+
+
+class SolveGraph:
+    """A captured multi-iteration solve; ``iters``, ``cost`` and ``cc`` are
+    the tensors each replay writes."""
+
+    def __init__(self, solver, it, max_iter, dcost_tol, cc_tol):
+        b = solver.batch
+        dev = b.device
+        self.it = it
+        self.iters = torch.zeros(b.B, dtype=torch.int32, device=dev)
+        self.cost = torch.zeros(b.B, dtype=b.dtype, device=dev)
+        self.cc = torch.zeros(b.B, dtype=b.dtype, device=dev)
+        opts = L.default_solve_opts(max_iter=int(max_iter), dcost_tol=float(dcost_tol), cc_tol=float(cc_tol),
+                                    check_every=0)
+        launched = C.c_int32(0)
+
+        def enqueue(target):
+            ci = b.c_iterate(target)
+            L.check(solver.lib.mpxkkt_solve(_stream(), C.byref(b.problem), C.byref(solver.params),
+                                            C.byref(opts), C.byref(ci), C.byref(solver._stats),
+                                            _ptr(self.iters), _ptr(self.cost), _ptr(self.cc),
+                                            C.c_void_p(solver.ws.data_ptr()), solver.ws_bytes,
+                                            C.byref(launched)))
+        # one eager pass on a copy first: one-time set-up stays outside the capture
+        enqueue({k: v.clone() for k, v in it.items()})
+        side = torch.cuda.Stream(device=dev)
+        side.wait_stream(torch.cuda.current_stream(dev))
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph, stream=side):
+            enqueue(it)
+        torch.cuda.current_stream(dev).wait_stream(side)
+
+    def replay(self):
+        self.graph.replay()
+        return self.iters

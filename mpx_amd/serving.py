@@ -188,7 +188,7 @@ def _workspace_bytes_for(proto, k, dtype, params):
     pr.batch, pr.n, pr.m, pr.horizon, pr.n_eq = k, n, m, N, p
     pr.has_box = 1 if proto.get("ulo") is not None else 0
     for name, typ in L.Problem._fields_:
-        if typ is L.View:
+        if typ is L.View and name not in L.CALLER_VIEWS:
             getattr(pr, name).ptr = 256     # never dereferenced by the size query
     nbytes = C.c_size_t(0)
     L.check(L.load().mpxkkt_workspace_size(C.byref(pr), C.byref(params), C.byref(nbytes)))
@@ -282,7 +282,7 @@ class MPCServer:
                     + N * p * (n + m + 1) + (2 * m if box else 0) + n)
             state = 2 * (N + 1) * n + N * m + N * p + 2 * N * m
             ws = _workspace_bytes_for(inst, self.max_group, self.dtype, self.params_for(inst))
-            ns = self.n_streams_slow if self._bucket_ms.get(key, 0.0) >= self.slow_ms else self.n_streams
+            ns = self.n_streams_slow if (self._bucket_ms.get(key) or 0.0) >= self.slow_ms else self.n_streams
             self._probe[key] = ((prob + state) * es, ns * ((prob + state) * es * self.max_group + ws))
         slot, fixed = self._probe[key]
         return fixed + self.slots_per_bucket * slot

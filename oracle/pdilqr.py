@@ -84,7 +84,16 @@ DEFAULT_PARAMS = dict(
 # ---------------------------------------------------------------------------
 
 def dynamics(inst, x, u, t):
-    """x_{t+1} = f(x_t, u_t, t); signature of mpx/utils/models.py:6 (x, u, t)."""
+    """x_{t+1} = f(x_t, u_t, t); signature of mpx/utils/models.py:6 (x, u, t).
+
+    ``model == "caller"``: the instance carries the caller's functions
+    ``f(x, u, t)``, ``fjac(x, u, t) -> (f_x, f_u)``, ``cost(x, u, t)`` (t = N:
+    terminal, u ignored), ``grad(x, u, t) -> (q, r)`` and
+    ``hess(x, u, t) -> (Q, R, M)`` -- what the reference solver gets from the
+    dynamics and cost callables, jax.jacobian / jax.grad and hessian_approx
+    (mpx/utils/mpc_wrapper.py:66-74, mpx/utils/objectives.py:15, :75)."""
+    if inst["model"] == "caller":
+        return inst["f"](x, u, t)
     if inst["model"] == "ltv":
         return inst["A"][t] @ x + inst["B"][t] @ u
     return x + inst["h"] * (inst["A"] @ x + inst["B"] @ u
@@ -94,6 +103,8 @@ def dynamics(inst, x, u, t):
 def dynamics_jac(inst, x, u, t):
     """(f_x, f_u).  tanh model: f_x = I + h (A + kappa diag(1 - tanh^2(Wx)) W),
     f_u = h B."""
+    if inst["model"] == "caller":
+        return inst["fjac"](x, u, t)
     if inst["model"] == "ltv":
         return inst["A"][t], inst["B"][t]
     n = inst["n"]
@@ -116,6 +127,8 @@ def R_at(inst, t):
 def stage_cost(inst, x, u, t):
     """Quadratic tracking cost; terminal form at t == N (mpx/utils/objectives.py:37)."""
     N = inst["N"]
+    if inst["model"] == "caller":
+        return float(inst["cost"](x, u, t))
     dx = x - inst["xref"][t]
     if t == N:
         return 0.5 * dx @ inst["QN"] @ dx
@@ -205,16 +218,22 @@ def linearize(inst, X, U, Yeq, Ybox, rho_c, mu_b=1e-2):
     L = dict(A=np.zeros((N, n, n)), B=np.zeros((N, n, m)), Q=np.zeros((N, n, n)),
              R=np.zeros((N, m, m)), M=np.zeros((N, n, m)), q=np.zeros((N, n)),
              r=np.zeros((N, m)), c=np.zeros((N, n)))
+    caller = inst["model"] == "caller"
     for t in range(N):
         x, u = X[t], U[t]
         fx, fu = dynamics_jac(inst, x, u, t)
         L["A"][t], L["B"][t] = fx, fu
         L["c"][t] = dynamics(inst, x, u, t) - X[t + 1]
-        Qt, Rt = Q_at(inst, t), R_at(inst, t)
-        L["Q"][t] = Qt.copy()
-        L["R"][t] = Rt.copy()
-        L["q"][t] = Qt @ (x - inst["xref"][t])
-        L["r"][t] = Rt @ (u - inst["uref"][t])
+        if caller:
+            Qt, Rt, Mt = inst["hess"](x, u, t)
+            L["Q"][t], L["R"][t], L["M"][t] = Qt.copy(), Rt.copy(), Mt.copy()
+            L["q"][t], L["r"][t] = (np.array(v, dtype=float) for v in inst["grad"](x, u, t))
+        else:
+            Qt, Rt = Q_at(inst, t), R_at(inst, t)
+            L["Q"][t] = Qt.copy()
+            L["R"][t] = Rt.copy()
+            L["q"][t] = Qt @ (x - inst["xref"][t])
+            L["r"][t] = Rt @ (u - inst["uref"][t])
         if p > 0:
             D, E = inst["D"][t], inst["E"][t]
             h = eq_rows(inst, x, u, t)
@@ -229,8 +248,12 @@ def linearize(inst, X, U, Yeq, Ybox, rho_c, mu_b=1e-2):
             s = -box_g(inst, u)
             L["r"][t] += mu_b / s[:m] - mu_b / s[m:]
             L["R"][t] += np.diag(Ybox[t][:m] / s[:m] + Ybox[t][m:] / s[m:])
-    L["QN"] = inst["QN"].copy()
-    L["qN"] = inst["QN"] @ (X[N] - inst["xref"][N])
+    if caller:
+        L["QN"] = np.array(inst["hess"](X[N], np.zeros(m), N)[0], dtype=float)
+        L["qN"] = np.array(inst["grad"](X[N], np.zeros(m), N)[0], dtype=float)
+    else:
+        L["QN"] = inst["QN"].copy()
+        L["qN"] = inst["QN"] @ (X[N] - inst["xref"][N])
     L["c0"] = inst["x0"] - X[0]
     return L
 
@@ -635,11 +658,17 @@ def kkt_residual(inst, X, U, V, Yeq, Ybox, mu_b=DEFAULT_PARAMS["mu_b"]):
     """2-norm of the KKT residual of the constrained problem (D8) at (X,U,V,mu,nu)."""
     N, n, m, p = inst["N"], inst["n"], inst["m"], inst["p"]
     parts = []
+    caller = inst["model"] == "caller"
     for t in range(N):
         x, u = X[t], U[t]
         fx, fu = dynamics_jac(inst, x, u, t)
-        gx = Q_at(inst, t) @ (x - inst["xref"][t]) + fx.T @ V[t + 1] - V[t]
-        gu = R_at(inst, t) @ (u - inst["uref"][t]) + fu.T @ V[t + 1]
+        if caller:
+            q, r = inst["grad"](x, u, t)
+            gx = q + fx.T @ V[t + 1] - V[t]
+            gu = r + fu.T @ V[t + 1]
+        else:
+            gx = Q_at(inst, t) @ (x - inst["xref"][t]) + fx.T @ V[t + 1] - V[t]
+            gu = R_at(inst, t) @ (u - inst["uref"][t]) + fu.T @ V[t + 1]
         if p > 0:
             gx = gx + inst["D"][t].T @ Yeq[t]
             gu = gu + inst["E"][t].T @ Yeq[t]
@@ -648,7 +677,10 @@ def kkt_residual(inst, X, U, V, Yeq, Ybox, mu_b=DEFAULT_PARAMS["mu_b"]):
             gu = gu + Ybox[t][:m] - Ybox[t][m:]
             parts.append(Ybox[t] * (-box_g(inst, u)) - mu_b)
         parts += [gx, gu]
-    parts.append(inst["QN"] @ (X[N] - inst["xref"][N]) - V[N])
+    if caller:
+        parts.append(inst["grad"](X[N], np.zeros(m), N)[0] - V[N])
+    else:
+        parts.append(inst["QN"] @ (X[N] - inst["xref"][N]) - V[N])
     parts.append(constraints(inst, X, U).ravel())
     r = np.concatenate([np.ravel(v) for v in parts])
     return float(np.sqrt(r @ r))
@@ -667,3 +699,33 @@ def solve(inst, X, U, V, Yeq, Ybox, iters, params=None, lqr="riccati"):
         X, U, V, Yeq, Ybox = st["X"], st["U"], st["V"], st["Yeq"], st["Ybox"]
         out.append(st)
     return out
+
+
+def solve_offline(inst, X, U, V, Yeq, Ybox, max_iter=100, dcost_tol=1e-3, cc_tol=1e-5, params=None,
+                  step=None):
+    """SQP iterations until the stop rule of ``runOffline``
+    (mpx/utils/mpc_wrapper.py:425-467): after each iteration i (counted from 1)
+    evaluate (g, c) = model_evaluator_helper at the new iterate
+    (mpx/utils/mpc_wrapper.py:452), l2 = sum g^2; stop when i > max_iter, or
+    when last - l2 < dcost_tol and sum c^2 < cc_tol, with last = 1e10 before
+    the first iteration and last = l2 after every iteration that did not stop
+    (mpx/utils/mpc_wrapper.py:429-431, :462-467).
+
+    ``step`` replaces ``sqp_step`` (same signature and result keys), so a
+    faster restatement can drive the same loop.  Returns dict with the final
+    iterate, ``iters`` (iterations run), ``l2`` and ``cc`` of the last one.
+    """
+    step = step or (lambda *a: sqp_step(*a, params=params))
+    last = 1e10
+    i = 0
+    while True:
+        st = step(inst, X, U, V, Yeq, Ybox)
+        X, U, V, Yeq, Ybox = st["X"], st["U"], st["V"], st["Yeq"], st["Ybox"]
+        g, c = model_evaluator_helper(inst, X, U)
+        l2 = float(np.sum(g * g))
+        cc = float(np.sum(c * c))
+        i += 1
+        if i > max_iter or (last - l2 < dcost_tol and cc < cc_tol):
+            break
+        last = l2
+    return dict(X=X, U=U, V=V, Yeq=Yeq, Ybox=Ybox, iters=i, l2=l2, cc=cc)

@@ -1,5 +1,5 @@
 /*
- * FP64 C restatement of one primal-dual SQP iteration (TEST INFRASTRUCTURE and
+ * FP64 (and, with -DPDILQR_F32, FP32) C restatement of one primal-dual SQP iteration (TEST INFRASTRUCTURE and
  * CPU baseline only; the product never links this).  Same algorithm and
  * decisions D1-D10 as oracle/pdilqr.py (linearise + augmented Lagrangian of the
  * equality rows + interior-point box, sequential Riccati LQR, merit line search
@@ -11,29 +11,37 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* real: double, or float with -DPDILQR_F32 (the same algorithm in FP32
+ * arithmetic: the error model of the device's FP32 path). */
+#ifdef PDILQR_F32
+typedef float real;
+#else
+typedef double real;
+#endif
+
 typedef struct {
   int n, m, N, p, model, box, q_tv, r_tv;
-  double h, kappa;
-  const double *A, *B, *W, *Q, *R, *QN, *xref, *uref, *D, *E, *e, *ulo, *uhi, *x0;
+  real h, kappa;
+  const real *A, *B, *W, *Q, *R, *QN, *xref, *uref, *D, *E, *e, *ulo, *uhi, *x0;
 } inst_t;
 
 enum { P_RHO0, P_RHOMAX, P_RHOC, P_MUB, P_TAU, P_BETA, P_C1, P_SLACK, P_TRIALS, P_HFLOOR };
 
 /* C = A (r x k) * B (k x c) */
-static void mm(const double* A, const double* B, double* C, int r, int k, int c) {
+static void mm(const real* A, const real* B, real* C, int r, int k, int c) {
   for (int i = 0; i < r; ++i)
     for (int j = 0; j < c; ++j) {
-      double s = 0;
+      real s = 0;
       for (int l = 0; l < k; ++l) s += A[i * k + l] * B[l * c + j];
       C[i * c + j] = s;
     }
 }
 
 /* C = A^T (A: k x r) * B (k x c) */
-static void mtm(const double* A, const double* B, double* C, int k, int r, int c) {
+static void mtm(const real* A, const real* B, real* C, int k, int r, int c) {
   for (int i = 0; i < r; ++i)
     for (int j = 0; j < c; ++j) {
-      double s = 0;
+      real s = 0;
       for (int l = 0; l < k; ++l) s += A[l * r + i] * B[l * c + j];
       C[i * c + j] = s;
     }
@@ -41,10 +49,10 @@ static void mtm(const double* A, const double* B, double* C, int k, int r, int c
 
 /* in-place Cholesky (lower) with pivots max(|pivot|, floor |H_jj|) (decision
  * D2 of oracle/pdilqr.py); returns 0 on success */
-static int chol(double* H, int m, double floor) {
+static int chol(real* H, int m, real floor) {
   for (int j = 0; j < m; ++j) {
-    double d = H[j * m + j];
-    const double fl = floor * fabs(H[j * m + j]);
+    real d = H[j * m + j];
+    const real fl = floor * fabs(H[j * m + j]);
     for (int k = 0; k < j; ++k) d -= H[j * m + k] * H[j * m + k];
     d = fabs(d);
     if (!(d >= fl) && !isnan(d)) d = fl;
@@ -52,7 +60,7 @@ static int chol(double* H, int m, double floor) {
     d = sqrt(d);
     H[j * m + j] = d;
     for (int i = j + 1; i < m; ++i) {
-      double v = H[i * m + j];
+      real v = H[i * m + j];
       for (int k = 0; k < j; ++k) v -= H[i * m + k] * H[j * m + k];
       H[i * m + j] = v / d;
     }
@@ -61,43 +69,43 @@ static int chol(double* H, int m, double floor) {
 }
 
 /* solve (L L^T) X = G for the ncol columns of G (m x ncol) in place */
-static void chol_solve(const double* L, double* G, int m, int ncol) {
+static void chol_solve(const real* L, real* G, int m, int ncol) {
   for (int c = 0; c < ncol; ++c) {
     for (int i = 0; i < m; ++i) {
-      double v = G[i * ncol + c];
+      real v = G[i * ncol + c];
       for (int k = 0; k < i; ++k) v -= L[i * m + k] * G[k * ncol + c];
       G[i * ncol + c] = v / L[i * m + i];
     }
     for (int i = m - 1; i >= 0; --i) {
-      double v = G[i * ncol + c];
+      real v = G[i * ncol + c];
       for (int k = i + 1; k < m; ++k) v -= L[k * m + i] * G[k * ncol + c];
       G[i * ncol + c] = v / L[i * m + i];
     }
   }
 }
 
-static const double* Qt(const inst_t* I, int t) { return I->Q + (I->q_tv ? (size_t)t * I->n * I->n : 0); }
-static const double* Rt(const inst_t* I, int t) { return I->R + (I->r_tv ? (size_t)t * I->m * I->m : 0); }
+static const real* Qt(const inst_t* I, int t) { return I->Q + (I->q_tv ? (size_t)t * I->n * I->n : 0); }
+static const real* Rt(const inst_t* I, int t) { return I->R + (I->r_tv ? (size_t)t * I->m * I->m : 0); }
 
-static void dyn(const inst_t* I, const double* x, const double* u, int t, double* f, double* fx, double* wk) {
+static void dyn(const inst_t* I, const real* x, const real* u, int t, real* f, real* fx, real* wk) {
   const int n = I->n, m = I->m;
   if (I->model == 0) {
-    const double* A = I->A + (size_t)t * n * n;
-    const double* B = I->B + (size_t)t * n * m;
+    const real* A = I->A + (size_t)t * n * n;
+    const real* B = I->B + (size_t)t * n * m;
     for (int i = 0; i < n; ++i) {
-      double v = 0;
+      real v = 0;
       for (int j = 0; j < n; ++j) v += A[i * n + j] * x[j];
       for (int j = 0; j < m; ++j) v += B[i * m + j] * u[j];
       f[i] = v;
     }
-    if (fx) memcpy(fx, A, sizeof(double) * n * n);
+    if (fx) memcpy(fx, A, sizeof(real) * n * n);
     return;
   }
   for (int i = 0; i < n; ++i) {
-    double ax = 0, wx = 0, bu = 0;
+    real ax = 0, wx = 0, bu = 0;
     for (int j = 0; j < n; ++j) { ax += I->A[i * n + j] * x[j]; wx += I->W[i * n + j] * x[j]; }
     for (int j = 0; j < m; ++j) bu += I->B[i * m + j] * u[j];
-    const double th = tanh(wx);
+    const real th = tanh(wx);
     wk[i] = 1.0 - th * th;
     f[i] = x[i] + I->h * (ax + bu + I->kappa * th);
   }
@@ -107,36 +115,36 @@ static void dyn(const inst_t* I, const double* x, const double* u, int t, double
         fx[i * n + j] = (i == j ? 1.0 : 0.0) + I->h * (I->A[i * n + j] + I->kappa * wk[i] * I->W[i * n + j]);
 }
 
-static void fu_of(const inst_t* I, int t, double* fu) {
+static void fu_of(const inst_t* I, int t, real* fu) {
   const int n = I->n, m = I->m;
-  if (I->model == 0) memcpy(fu, I->B + (size_t)t * n * m, sizeof(double) * n * m);
+  if (I->model == 0) memcpy(fu, I->B + (size_t)t * n * m, sizeof(real) * n * m);
   else for (int k = 0; k < n * m; ++k) fu[k] = I->h * I->B[k];
 }
 
 /* stage AL / barrier cost (t < N) or terminal cost (t == N) */
-static double al_cost(const inst_t* I, const double* x, const double* u, int t, const double* yeq,
-                      const double* prm) {
+static real al_cost(const inst_t* I, const real* x, const real* u, int t, const real* yeq,
+                      const real* prm) {
   const int n = I->n, m = I->m, p = I->p, N = I->N;
-  const double* xr = I->xref + (size_t)t * n;
-  double val = 0;
-  const double* Q = t == N ? I->QN : Qt(I, t);
+  const real* xr = I->xref + (size_t)t * n;
+  real val = 0;
+  const real* Q = t == N ? I->QN : Qt(I, t);
   for (int i = 0; i < n; ++i) {
-    double qx = 0;
+    real qx = 0;
     for (int j = 0; j < n; ++j) qx += Q[i * n + j] * (x[j] - xr[j]);
     val += 0.5 * (x[i] - xr[i]) * qx;
   }
   if (t == N) return val;
-  const double* R = Rt(I, t);
-  const double* ur = I->uref + (size_t)t * m;
+  const real* R = Rt(I, t);
+  const real* ur = I->uref + (size_t)t * m;
   for (int i = 0; i < m; ++i) {
-    double ru = 0;
+    real ru = 0;
     for (int j = 0; j < m; ++j) ru += R[i * m + j] * (u[j] - ur[j]);
     val += 0.5 * (u[i] - ur[i]) * ru;
   }
   for (int r = 0; r < p; ++r) {
-    const double* D = I->D + ((size_t)t * p + r) * n;
-    const double* E = I->E + ((size_t)t * p + r) * m;
-    double hv = I->e[(size_t)t * p + r];
+    const real* D = I->D + ((size_t)t * p + r) * n;
+    const real* E = I->E + ((size_t)t * p + r) * m;
+    real hv = I->e[(size_t)t * p + r];
     for (int j = 0; j < n; ++j) hv += D[j] * x[j];
     for (int j = 0; j < m; ++j) hv += E[j] * u[j];
     val += yeq[r] * hv + 0.5 * prm[P_RHOC] * hv * hv;
@@ -146,20 +154,20 @@ static double al_cost(const inst_t* I, const double* x, const double* u, int t, 
   return val;
 }
 
-static double merit(const inst_t* I, const double* X, const double* U, const double* V, const double* Yeq,
-                    double rho, const double* prm, double* wk) {
+static real merit(const inst_t* I, const real* X, const real* U, const real* V, const real* Yeq,
+                    real rho, const real* prm, real* wk) {
   const int n = I->n, m = I->m, N = I->N, p = I->p;
-  double val = 0, *f = wk, *tmp = wk + n;
+  real val = 0, *f = wk, *tmp = wk + n;
   for (int t = 0; t <= N; ++t) val += al_cost(I, X + (size_t)t * n, t < N ? U + (size_t)t * m : NULL, t,
                                               t < N ? Yeq + (size_t)t * p : NULL, prm);
   for (int i = 0; i < n; ++i) {
-    const double c = I->x0[i] - X[i];
+    const real c = I->x0[i] - X[i];
     val += V[i] * c + 0.5 * rho * c * c;
   }
   for (int t = 0; t < N; ++t) {
     dyn(I, X + (size_t)t * n, U + (size_t)t * m, t, f, NULL, tmp);
     for (int i = 0; i < n; ++i) {
-      const double c = f[i] - X[(size_t)(t + 1) * n + i];
+      const real c = f[i] - X[(size_t)(t + 1) * n + i];
       val += V[(size_t)(t + 1) * n + i] * c + 0.5 * rho * c * c;
     }
   }
@@ -167,50 +175,50 @@ static double merit(const inst_t* I, const double* X, const double* U, const dou
 }
 
 /* one SQP iteration of one instance, in place; returns the accepted step */
-static double sqp_one(const inst_t* I, double* X, double* U, double* V, double* Yeq, double* Ybox,
-                      const double* prm) {
+static real sqp_one(const inst_t* I, real* X, real* U, real* V, real* Yeq, real* Ybox,
+                      const real* prm) {
   const int n = I->n, m = I->m, N = I->N, p = I->p, N1 = N + 1;
-  const double rc = prm[P_RHOC], mub = prm[P_MUB], tau = prm[P_TAU];
+  const real rc = prm[P_RHOC], mub = prm[P_MUB], tau = prm[P_TAU];
   const size_t nn = (size_t)n * n, nm = (size_t)n * m;
   const size_t tsz = (size_t)(n + m + 1) * (n + m + 1);  /* any n x n, m x (n+1), m x m temporary */
-  double* mem = (double*)calloc((size_t)N * (2 * nn + 3 * nm + (size_t)m * m) + (size_t)N1 * (nn + 4 * n) +
+  real* mem = (real*)calloc((size_t)N * (2 * nn + 3 * nm + (size_t)m * m) + (size_t)N1 * (nn + 4 * n) +
                                     (size_t)N * 7 * m + 5 * tsz + 8 * (size_t)(n + m + 4),
-                                sizeof(double));
-  double* q = mem;
-  double* Af = q;  q += (size_t)N * nn;      /* A_t */
-  double* Bf = q;  q += (size_t)N * nm;      /* B_t */
-  double* Kf = q;  q += (size_t)N * nm;      /* K_t (m x n) */
-  double* Mh = q;  q += (size_t)N * nm;      /* M^ (n x m) */
-  double* Qh = q;  q += (size_t)N * nn;      /* Q^ */
-  double* Rh = q;  q += (size_t)N * m * m;   /* R^ */
-  double* S = q;   q += (size_t)N1 * nn;
-  double* s = q;   q += (size_t)N1 * n;
-  double* qg = q;  q += (size_t)N1 * n;      /* gradients (row N: terminal) */
-  double* cd = q;  q += (size_t)N1 * n;      /* constraint rows */
-  double* dx = q;  q += (size_t)N1 * n;
-  double* rg = q;  q += (size_t)N * m;
-  double* kf = q;  q += (size_t)N * m;
-  double* du = q;  q += (size_t)N * m;
-  double* dY = q;  q += (size_t)N * 2 * m;
-  double* Sl = q;  q += (size_t)N * 2 * m;
-  double* t1 = q;  q += tsz;
-  double* t2 = q;  q += tsz;
-  double* t3 = q;  q += tsz;
-  double* H = q;   q += tsz;
-  double* G = q;   q += tsz;
-  double* w = q;   q += 8 * (size_t)(n + m + 4);
+                                sizeof(real));
+  real* q = mem;
+  real* Af = q;  q += (size_t)N * nn;      /* A_t */
+  real* Bf = q;  q += (size_t)N * nm;      /* B_t */
+  real* Kf = q;  q += (size_t)N * nm;      /* K_t (m x n) */
+  real* Mh = q;  q += (size_t)N * nm;      /* M^ (n x m) */
+  real* Qh = q;  q += (size_t)N * nn;      /* Q^ */
+  real* Rh = q;  q += (size_t)N * m * m;   /* R^ */
+  real* S = q;   q += (size_t)N1 * nn;
+  real* s = q;   q += (size_t)N1 * n;
+  real* qg = q;  q += (size_t)N1 * n;      /* gradients (row N: terminal) */
+  real* cd = q;  q += (size_t)N1 * n;      /* constraint rows */
+  real* dx = q;  q += (size_t)N1 * n;
+  real* rg = q;  q += (size_t)N * m;
+  real* kf = q;  q += (size_t)N * m;
+  real* du = q;  q += (size_t)N * m;
+  real* dY = q;  q += (size_t)N * 2 * m;
+  real* Sl = q;  q += (size_t)N * 2 * m;
+  real* t1 = q;  q += tsz;
+  real* t2 = q;  q += tsz;
+  real* t3 = q;  q += tsz;
+  real* H = q;   q += tsz;
+  real* G = q;   q += tsz;
+  real* w = q;   q += 8 * (size_t)(n + m + 4);
   /* box: interior projection and positive multipliers */
   if (I->box) {
     for (int t = 0; t < N; ++t)
       for (int i = 0; i < m; ++i) {
-        double* u = U + (size_t)t * m + i;
-        const double lo = I->ulo[i], hi = I->uhi[i], wd = 1e-3 * (hi - lo);
+        real* u = U + (size_t)t * m + i;
+        const real lo = I->ulo[i], hi = I->uhi[i], wd = 1e-3 * (hi - lo);
         if (*u >= hi) *u = hi - wd;
         if (*u <= lo) *u = lo + wd;
         Sl[(size_t)t * 2 * m + i] = hi - *u;
         Sl[(size_t)t * 2 * m + m + i] = *u - lo;
         for (int k = 0; k < 2; ++k) {
-          double* nu = Ybox + (size_t)t * 2 * m + k * m + i;
+          real* nu = Ybox + (size_t)t * 2 * m + k * m + i;
           if (!(*nu > 0)) *nu = mub / Sl[(size_t)t * 2 * m + k * m + i];
         }
       }
@@ -218,31 +226,31 @@ static double sqp_one(const inst_t* I, double* X, double* U, double* V, double* 
   /* linearise + augment */
   for (int i = 0; i < n; ++i) cd[i] = I->x0[i] - X[i];
   for (int t = 0; t < N; ++t) {
-    const double *x = X + (size_t)t * n, *u = U + (size_t)t * m;
+    const real *x = X + (size_t)t * n, *u = U + (size_t)t * m;
     dyn(I, x, u, t, w, Af + (size_t)t * nn, w + n);
     fu_of(I, t, Bf + (size_t)t * nm);
     for (int i = 0; i < n; ++i) cd[(size_t)(t + 1) * n + i] = w[i] - X[(size_t)(t + 1) * n + i];
-    const double* Q = Qt(I, t);
-    const double* R = Rt(I, t);
-    memcpy(Qh + (size_t)t * nn, Q, sizeof(double) * nn);
-    memcpy(Rh + (size_t)t * m * m, R, sizeof(double) * m * m);
+    const real* Q = Qt(I, t);
+    const real* R = Rt(I, t);
+    memcpy(Qh + (size_t)t * nn, Q, sizeof(real) * nn);
+    memcpy(Rh + (size_t)t * m * m, R, sizeof(real) * m * m);
     for (int i = 0; i < n; ++i) {
-      double v = 0;
+      real v = 0;
       for (int j = 0; j < n; ++j) v += Q[i * n + j] * (x[j] - I->xref[(size_t)t * n + j]);
       qg[(size_t)t * n + i] = v;
     }
     for (int i = 0; i < m; ++i) {
-      double v = 0;
+      real v = 0;
       for (int j = 0; j < m; ++j) v += R[i * m + j] * (u[j] - I->uref[(size_t)t * m + j]);
       rg[(size_t)t * m + i] = v;
     }
     for (int r = 0; r < p; ++r) {
-      const double* D = I->D + ((size_t)t * p + r) * n;
-      const double* E = I->E + ((size_t)t * p + r) * m;
-      double hv = I->e[(size_t)t * p + r];
+      const real* D = I->D + ((size_t)t * p + r) * n;
+      const real* E = I->E + ((size_t)t * p + r) * m;
+      real hv = I->e[(size_t)t * p + r];
       for (int j = 0; j < n; ++j) hv += D[j] * x[j];
       for (int j = 0; j < m; ++j) hv += E[j] * u[j];
-      const double wv = Yeq[(size_t)t * p + r] + rc * hv;
+      const real wv = Yeq[(size_t)t * p + r] + rc * hv;
       for (int i = 0; i < n; ++i) {
         qg[(size_t)t * n + i] += D[i] * wv;
         for (int j = 0; j < n; ++j) Qh[(size_t)t * nn + i * n + j] += rc * D[i] * D[j];
@@ -255,25 +263,25 @@ static double sqp_one(const inst_t* I, double* X, double* U, double* V, double* 
     }
     if (I->box)
       for (int i = 0; i < m; ++i) {
-        const double shi = Sl[(size_t)t * 2 * m + i], slo = Sl[(size_t)t * 2 * m + m + i];
-        const double nhi = Ybox[(size_t)t * 2 * m + i], nlo = Ybox[(size_t)t * 2 * m + m + i];
+        const real shi = Sl[(size_t)t * 2 * m + i], slo = Sl[(size_t)t * 2 * m + m + i];
+        const real nhi = Ybox[(size_t)t * 2 * m + i], nlo = Ybox[(size_t)t * 2 * m + m + i];
         rg[(size_t)t * m + i] += mub / shi - mub / slo;
         Rh[(size_t)t * m * m + i * m + i] += nhi / shi + nlo / slo;
       }
   }
   for (int i = 0; i < n; ++i) {
-    double v = 0;
+    real v = 0;
     for (int j = 0; j < n; ++j) v += I->QN[i * n + j] * (X[(size_t)N * n + j] - I->xref[(size_t)N * n + j]);
     qg[(size_t)N * n + i] = v;
   }
   /* backward Riccati */
-  memcpy(S + (size_t)N * nn, I->QN, sizeof(double) * nn);
-  memcpy(s + (size_t)N * n, qg + (size_t)N * n, sizeof(double) * n);
+  memcpy(S + (size_t)N * nn, I->QN, sizeof(real) * nn);
+  memcpy(s + (size_t)N * n, qg + (size_t)N * n, sizeof(real) * n);
   for (int t = N - 1; t >= 0; --t) {
-    const double *A = Af + (size_t)t * nn, *B = Bf + (size_t)t * nm, *Sn = S + (size_t)(t + 1) * nn;
-    double* Sc = w;
+    const real *A = Af + (size_t)t * nn, *B = Bf + (size_t)t * nm, *Sn = S + (size_t)(t + 1) * nn;
+    real* Sc = w;
     for (int i = 0; i < n; ++i) {
-      double v = s[(size_t)(t + 1) * n + i];
+      real v = s[(size_t)(t + 1) * n + i];
       for (int j = 0; j < n; ++j) v += Sn[i * n + j] * cd[(size_t)(t + 1) * n + j];
       Sc[i] = v;
     }
@@ -282,51 +290,51 @@ static double sqp_one(const inst_t* I, double* X, double* U, double* V, double* 
     mtm(B, t2, H, n, m, m);          /* B'SB */
     for (int k = 0; k < m * m; ++k) H[k] += Rh[(size_t)t * m * m + k];
     mtm(B, t1, G, n, m, n);          /* B'SA (m x n) -> columns 0..n-1 of [G | g] below */
-    double* Gg = t3;                 /* m x (n+1) */
+    real* Gg = t3;                 /* m x (n+1) */
     for (int i = 0; i < m; ++i) {
       for (int j = 0; j < n; ++j) Gg[i * (n + 1) + j] = G[i * n + j] + Mh[(size_t)t * nm + j * m + i];
-      double v = rg[(size_t)t * m + i];
+      real v = rg[(size_t)t * m + i];
       for (int k = 0; k < n; ++k) v += B[k * m + i] * Sc[k];
       Gg[i * (n + 1) + n] = v;
     }
     if (chol(H, m, prm[P_HFLOOR])) { free(mem); return NAN; }
-    double* Y = G;                   /* keep G' = [G | g] for S update */
-    memcpy(Y, Gg, sizeof(double) * m * (n + 1));
+    real* Y = G;                   /* keep G' = [G | g] for S update */
+    memcpy(Y, Gg, sizeof(real) * m * (n + 1));
     chol_solve(H, Gg, m, n + 1);     /* [K | k] */
     for (int i = 0; i < m; ++i) {
       for (int j = 0; j < n; ++j) Kf[(size_t)t * nm + i * n + j] = Gg[i * (n + 1) + j];
       kf[(size_t)t * m + i] = Gg[i * (n + 1) + n];
     }
-    double* St = S + (size_t)t * nn;
+    real* St = S + (size_t)t * nn;
     mtm(A, t1, St, n, n, n);         /* A'SA */
     for (int i = 0; i < n; ++i)
       for (int j = 0; j < n; ++j) {
-        double v = Qh[(size_t)t * nn + i * n + j] + St[i * n + j];
+        real v = Qh[(size_t)t * nn + i * n + j] + St[i * n + j];
         for (int r = 0; r < m; ++r) v -= Y[r * (n + 1) + i] * Gg[r * (n + 1) + j];
         St[i * n + j] = v;
       }
     for (int i = 0; i < n; ++i)
       for (int j = i + 1; j < n; ++j) {
-        const double a = 0.5 * (St[i * n + j] + St[j * n + i]);
+        const real a = 0.5 * (St[i * n + j] + St[j * n + i]);
         St[i * n + j] = St[j * n + i] = a;
       }
     for (int i = 0; i < n; ++i) {
-      double v = qg[(size_t)t * n + i];
+      real v = qg[(size_t)t * n + i];
       for (int k = 0; k < n; ++k) v += A[k * n + i] * Sc[k];
       for (int r = 0; r < m; ++r) v -= Y[r * (n + 1) + i] * Gg[r * (n + 1) + n];
       s[(size_t)t * n + i] = v;
     }
   }
   /* forward pass, costates, slope terms */
-  double d_f = 0, d_c = 0, lc = 0, cc = 0;
-  memcpy(dx, cd, sizeof(double) * n);
-  double* dV = (double*)malloc(sizeof(double) * N1 * n);
+  real d_f = 0, d_c = 0, lc = 0, cc = 0;
+  memcpy(dx, cd, sizeof(real) * n);
+  real* dV = (real*)malloc(sizeof(real) * N1 * n);
   for (int t = 0; t <= N; ++t) {
-    const double* St = S + (size_t)t * nn;
+    const real* St = S + (size_t)t * nn;
     for (int i = 0; i < n; ++i) {
-      double lam = s[(size_t)t * n + i];
+      real lam = s[(size_t)t * n + i];
       for (int j = 0; j < n; ++j) lam += St[i * n + j] * dx[(size_t)t * n + j];
-      const double v = V[(size_t)t * n + i], c = cd[(size_t)t * n + i];
+      const real v = V[(size_t)t * n + i], c = cd[(size_t)t * n + i];
       dV[(size_t)t * n + i] = lam - v;
       d_c += (lam - 2 * v) * c;
       lc += lam * c;
@@ -335,45 +343,45 @@ static double sqp_one(const inst_t* I, double* X, double* U, double* V, double* 
     }
     if (t == N) break;
     for (int i = 0; i < m; ++i) {
-      double v = kf[(size_t)t * m + i];
+      real v = kf[(size_t)t * m + i];
       for (int j = 0; j < n; ++j) v += Kf[(size_t)t * nm + i * n + j] * dx[(size_t)t * n + j];
       du[(size_t)t * m + i] = -v;
       d_f -= rg[(size_t)t * m + i] * v;
     }
-    const double *A = Af + (size_t)t * nn, *B = Bf + (size_t)t * nm;
+    const real *A = Af + (size_t)t * nn, *B = Bf + (size_t)t * nm;
     for (int i = 0; i < n; ++i) {
-      double v = cd[(size_t)(t + 1) * n + i];
+      real v = cd[(size_t)(t + 1) * n + i];
       for (int j = 0; j < n; ++j) v += A[i * n + j] * dx[(size_t)t * n + j];
       for (int j = 0; j < m; ++j) v += B[i * m + j] * du[(size_t)t * m + j];
       dx[(size_t)(t + 1) * n + i] = v;
     }
   }
-  const double kap = lc - d_f;
-  double rho = prm[P_RHO0];
-  if (cc > 0) { const double r2 = 2 * (d_f + d_c + 0.5 * kap) / cc; if (r2 > rho) rho = r2; }
+  const real kap = lc - d_f;
+  real rho = prm[P_RHO0];
+  if (cc > 0) { const real r2 = 2 * (d_f + d_c + 0.5 * kap) / cc; if (r2 > rho) rho = r2; }
   if (rho > prm[P_RHOMAX]) rho = prm[P_RHOMAX];
-  const double dphi = d_f + d_c - rho * cc;
-  double amax = 1, adual = 1;
+  const real dphi = d_f + d_c - rho * cc;
+  real amax = 1, adual = 1;
   if (I->box)
     for (int t = 0; t < N; ++t)
       for (int i = 0; i < m; ++i) {
-        const double d = du[(size_t)t * m + i];
-        const double shi = Sl[(size_t)t * 2 * m + i], slo = Sl[(size_t)t * 2 * m + m + i];
-        const double nhi = Ybox[(size_t)t * 2 * m + i], nlo = Ybox[(size_t)t * 2 * m + m + i];
+        const real d = du[(size_t)t * m + i];
+        const real shi = Sl[(size_t)t * 2 * m + i], slo = Sl[(size_t)t * 2 * m + m + i];
+        const real nhi = Ybox[(size_t)t * 2 * m + i], nlo = Ybox[(size_t)t * 2 * m + m + i];
         if (d > 0 && tau * shi / d < amax) amax = tau * shi / d;
         if (d < 0 && tau * slo / (-d) < amax) amax = tau * slo / (-d);
-        const double dhi = mub / shi - nhi + nhi * d / shi, dlo = mub / slo - nlo - nlo * d / slo;
+        const real dhi = mub / shi - nhi + nhi * d / shi, dlo = mub / slo - nlo - nlo * d / slo;
         dY[(size_t)t * 2 * m + i] = dhi;
         dY[(size_t)t * 2 * m + m + i] = dlo;
         if (dhi < 0 && -tau * nhi / dhi < adual) adual = -tau * nhi / dhi;
         if (dlo < 0 && -tau * nlo / dlo < adual) adual = -tau * nlo / dlo;
       }
-  double* wk = (double*)malloc(sizeof(double) * 4 * (n + m + 4));
-  const double phi0 = merit(I, X, U, V, Yeq, rho, prm, wk);
-  const double slack = prm[P_SLACK] * (1 + fabs(phi0));
-  double* Xt = (double*)malloc(sizeof(double) * (N1 * n * 2 + N * m));
-  double *Vt = Xt + N1 * n, *Ut = Vt + N1 * n;
-  double alpha = -1, a = amax;
+  real* wk = (real*)malloc(sizeof(real) * 4 * (n + m + 4));
+  const real phi0 = merit(I, X, U, V, Yeq, rho, prm, wk);
+  const real slack = prm[P_SLACK] * (1 + fabs(phi0));
+  real* Xt = (real*)malloc(sizeof(real) * (N1 * n * 2 + N * m));
+  real *Vt = Xt + N1 * n, *Ut = Vt + N1 * n;
+  real alpha = -1, a = amax;
   const int trials = (int)prm[P_TRIALS];
   for (int j = 0; j < trials; ++j, a *= prm[P_BETA]) {
     for (int k = 0; k < N1 * n; ++k) { Xt[k] = X[k] + a * dx[k]; Vt[k] = V[k] + a * dV[k]; }
@@ -385,9 +393,9 @@ static double sqp_one(const inst_t* I, double* X, double* U, double* V, double* 
   for (int k = 0; k < N * m; ++k) U[k] += alpha * du[k];
   for (int t = 0; t < N; ++t)
     for (int r = 0; r < p; ++r) {
-      const double* D = I->D + ((size_t)t * p + r) * n;
-      const double* E = I->E + ((size_t)t * p + r) * m;
-      double hv = I->e[(size_t)t * p + r];
+      const real* D = I->D + ((size_t)t * p + r) * n;
+      const real* E = I->E + ((size_t)t * p + r) * m;
+      real hv = I->e[(size_t)t * p + r];
       for (int j = 0; j < n; ++j) hv += D[j] * X[(size_t)t * n + j];
       for (int j = 0; j < m; ++j) hv += E[j] * U[(size_t)t * m + j];
       Yeq[(size_t)t * p + r] += rc * hv;
@@ -403,12 +411,12 @@ static double sqp_one(const inst_t* I, double* X, double* U, double* V, double* 
 
 /* iters SQP iterations for B instances (arrays instance-major, as the device
  * layout); returns 0.  Q/R are [B][Tq][n][n] with q_tv = (Tq == N). */
-int pdilqr_solve_batch(int B, int n, int m, int N, int p, int model, int box, int q_tv, int r_tv, double h,
-                       double kappa, const double* A, const double* Bm, const double* W, const double* Q,
-                       const double* R, const double* QN, const double* xref, const double* uref,
-                       const double* D, const double* E, const double* e, const double* ulo, const double* uhi,
-                       const double* x0, double* X, double* U, double* V, double* Yeq, double* Ybox,
-                       const double* prm, int iters, double* alpha_out, int nthreads) {
+int pdilqr_solve_batch(int B, int n, int m, int N, int p, int model, int box, int q_tv, int r_tv, real h,
+                       real kappa, const real* A, const real* Bm, const real* W, const real* Q,
+                       const real* R, const real* QN, const real* xref, const real* uref,
+                       const real* D, const real* E, const real* e, const real* ulo, const real* uhi,
+                       const real* x0, real* X, real* U, real* V, real* Yeq, real* Ybox,
+                       const real* prm, int iters, real* alpha_out, int nthreads) {
   const size_t nA = model == 0 ? (size_t)N * n * n : (size_t)n * n;
   const size_t nB = model == 0 ? (size_t)N * n * m : (size_t)n * m;
   const size_t nQ = (size_t)(q_tv ? N : 1) * n * n, nR = (size_t)(r_tv ? N : 1) * m * m;
@@ -426,7 +434,7 @@ int pdilqr_solve_batch(int B, int n, int m, int N, int p, int model, int box, in
     I.ulo = box ? ulo + (size_t)b * m : NULL; I.uhi = box ? uhi + (size_t)b * m : NULL;
     I.x0 = x0 + (size_t)b * n;
     for (int k = 0; k < iters; ++k) {
-      const double a = sqp_one(&I, X + (size_t)b * (N + 1) * n, U + (size_t)b * N * m, V + (size_t)b * (N + 1) * n,
+      const real a = sqp_one(&I, X + (size_t)b * (N + 1) * n, U + (size_t)b * N * m, V + (size_t)b * (N + 1) * n,
                                p ? Yeq + (size_t)b * N * p : NULL, box ? Ybox + (size_t)b * N * 2 * m : NULL, prm);
       if (alpha_out) alpha_out[(size_t)b * iters + k] = a;
     }

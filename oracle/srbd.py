@@ -1,0 +1,336 @@
+"""FP64 restatement of the reference's single-rigid-body quadruped model and
+its Gauss-Newton objective -- the caller functions the SRBD controller hands
+the solver (mpx/utils/mpc_wrapper_srbd.py:58-68).
+
+TEST INFRASTRUCTURE ONLY (the checker of the device SRBD kernels): nothing in
+the product imports it.
+
+Restated from
+* ``quadruped_srbd_dynamics`` (mpx/utils/models.py:6-37): state x = [p (3),
+  quat (4, w first), dp (3), omega (3)], control u = ground reaction forces
+  (3 per contact), stage parameter [contact (4), p_legs (12)];
+* ``penalty``, ``quadruped_srbd_obj``, ``quadruped_srbd_hessian_gn``
+  (mpx/utils/objectives.py:7-75);
+* the quaternion helpers they call from mujoco-mjx 3.3.0 (pinned at
+  pyproject.toml:17-18, absent here): ``quat_to_mat``, ``quat_mul``,
+  ``quat_inv``, ``quat_integrate`` (axis-angle exponential map), ``quat_sub``
+  (axis-angle logarithm of inv(v) u, angle wrapped into (-pi, pi]),
+  ``normalize_with_norm`` (x / (|x| + 1e-6 [|x| == 0])).  These follow the
+  published mjx/_src/math.py; restated from the mathematics, not copied.
+
+Reference layout used here (per stage t = 0..N): reference row
+[p_ref (3), quat_ref (4), dp_ref (3), omega_ref (3), contact (4)] (the
+contact columns 13.. are what quadruped_srbd_hessian_gn reads,
+objectives.py:70); parameter row [contact (4), p_legs (12)] (models.py:16-17).
+Derivatives are analytic; tests/test_srbd.py pins them by central differences.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+GRAV = np.array([0.0, 0.0, -9.81])
+MU = 0.7
+NC = 4
+N_STATE = 13
+N_CTRL = 3 * NC
+
+
+# --------------------------------------------------------------------------
+# quaternion helpers (mujoco-mjx math, restated)
+# --------------------------------------------------------------------------
+
+def quat_mul(u, v):
+    return np.array([
+        u[0] * v[0] - u[1] * v[1] - u[2] * v[2] - u[3] * v[3],
+        u[0] * v[1] + u[1] * v[0] + u[2] * v[3] - u[3] * v[2],
+        u[0] * v[2] - u[1] * v[3] + u[2] * v[0] + u[3] * v[1],
+        u[0] * v[3] + u[1] * v[2] - u[2] * v[1] + u[3] * v[0],
+    ])
+
+
+def quat_mul_left(u):
+    """L(u) with quat_mul(u, v) = L(u) v."""
+    return np.array([[u[0], -u[1], -u[2], -u[3]],
+                     [u[1], u[0], -u[3], u[2]],
+                     [u[2], u[3], u[0], -u[1]],
+                     [u[3], -u[2], u[1], u[0]]])
+
+
+def quat_mul_right(v):
+    """R(v) with quat_mul(u, v) = R(v) u."""
+    return np.array([[v[0], -v[1], -v[2], -v[3]],
+                     [v[1], v[0], v[3], -v[2]],
+                     [v[2], -v[3], v[0], v[1]],
+                     [v[3], v[2], -v[1], v[0]]])
+
+
+def quat_inv(q):
+    return q * np.array([1.0, -1.0, -1.0, -1.0])
+
+
+def quat_to_mat(q):
+    """Rotation matrix of a (not necessarily unit) quaternion, quadratic in q."""
+    w, x, y, z = q
+    return np.array([
+        [w * w + x * x - y * y - z * z, 2 * (x * y - w * z), 2 * (x * z + w * y)],
+        [2 * (x * y + w * z), w * w - x * x + y * y - z * z, 2 * (y * z - w * x)],
+        [2 * (x * z - w * y), 2 * (y * z + w * x), w * w - x * x - y * y + z * z],
+    ])
+
+
+def quat_to_mat_grad(q):
+    """dR/dq_k, k = 0..3 (each 3 x 3)."""
+    w, x, y, z = q
+    dw = 2 * np.array([[w, -z, y], [z, w, -x], [-y, x, w]])
+    dx = 2 * np.array([[x, y, z], [y, -x, -w], [z, w, -x]])
+    dy = 2 * np.array([[-y, x, w], [x, y, z], [-w, z, -y]])
+    dz = 2 * np.array([[-z, -w, x], [w, -z, y], [x, y, z]])
+    return [dw, dx, dy, dz]
+
+
+def _norm_safe(v):
+    n = float(np.sqrt(v @ v))
+    return n, v / (n + (1e-6 if n == 0.0 else 0.0))
+
+
+def quat_integrate(q, v, dt):
+    """q * exp(v dt / 2), normalised."""
+    n, axis = _norm_safe(v)
+    ang = n * dt
+    qr = np.concatenate([[np.cos(ang / 2)], axis * np.sin(ang / 2)])
+    qn = quat_mul(q, qr)
+    return _norm_safe(qn)[1]
+
+
+def _sinc_half(n, dt):
+    """sin(n dt / 2) / n and its derivative in n (series near n = 0)."""
+    a = 0.5 * dt
+    if n < 1e-4:
+        s = a - (a ** 3) * n * n / 6.0
+        ds = -(a ** 3) * n / 3.0
+    else:
+        s = np.sin(a * n) / n
+        ds = (a * np.cos(a * n) * n - np.sin(a * n)) / (n * n)
+    return s, ds
+
+
+def quat_integrate_jac(q, v, dt):
+    """(d quat_integrate / dq (4 x 4), d / dv (4 x 3)) for v != 0 (generic)."""
+    n = float(np.sqrt(v @ v))
+    a = 0.5 * dt
+    s, ds = _sinc_half(n, dt)
+    qr = np.concatenate([[np.cos(a * n)], v * s])
+    vhat = v / n if n > 0 else np.zeros(3)
+    dqr = np.zeros((4, 3))
+    dqr[0] = -np.sin(a * n) * a * vhat
+    dqr[1:] = s * np.eye(3) + np.outer(v, ds * vhat)
+    qn = quat_mul(q, qr)
+    nn = float(np.sqrt(qn @ qn))
+    dnorm = (np.eye(4) - np.outer(qn, qn) / (nn * nn)) / nn
+    return dnorm @ quat_mul_right(qr), dnorm @ quat_mul_left(q) @ dqr
+
+
+def quat_sub(u, v):
+    """Axis-angle of inv(v) u, angle wrapped into (-pi, pi]."""
+    q = quat_mul(quat_inv(v), u)
+    s, axis = _norm_safe(q[1:])
+    ang = 2.0 * np.arctan2(s, q[0])
+    if ang > np.pi:
+        ang -= 2.0 * np.pi
+    return axis * ang
+
+
+def quat_sub_jac(u, v):
+    """d quat_sub(u, v) / du (3 x 4) for a generic (non-zero vector part) q."""
+    q = quat_mul(quat_inv(v), u)
+    dq = quat_mul_left(quat_inv(v))
+    qv, w = q[1:], q[0]
+    s = float(np.sqrt(qv @ qv))
+    ang = 2.0 * np.arctan2(s, w)
+    if ang > np.pi:
+        ang -= 2.0 * np.pi
+    d2 = s * s + w * w
+    J = np.zeros((3, 4))
+    J[:, 0] = -2.0 * qv / d2                       # d(axis ang)/dw = axis * (-2 s / d2)
+    J[:, 1:] = (ang / s) * np.eye(3) + np.outer(qv, qv) * ((2.0 * w / d2) / (s * s) - ang / s ** 3)
+    return J @ dq
+
+
+# --------------------------------------------------------------------------
+# dynamics (mpx/utils/models.py:6-37)
+# --------------------------------------------------------------------------
+
+def _skew(a):
+    return np.array([[0.0, -a[2], a[1]], [a[2], 0.0, -a[0]], [-a[1], a[0], 0.0]])
+
+
+def dynamics(c, x, u, t, parameter):
+    """x_{t+1}; ``c`` holds mass, inertia, inertia_inv, dt."""
+    mass, I, Iinv, dt = c["mass"], c["inertia"], c["inertia_inv"], c["dt"]
+    p, quat, dp, om = x[:3], x[3:7], x[7:10], x[10:13]
+    contact, legs = parameter[t, :4], parameter[t, 4:]
+    f = [u[3 * i:3 * i + 3] for i in range(NC)]
+    force = sum(f[i] * contact[i] for i in range(NC))
+    dp_next = dp + (GRAV + force / mass) * dt
+    wrench = sum(np.cross(legs[3 * i:3 * i + 3] - p, f[i]) * contact[i] for i in range(NC))
+    Rm = quat_to_mat(quat)
+    om_next = om + Iinv @ (Rm.T @ wrench - np.cross(om, I @ om)) * dt
+    p_new = p + dp_next * dt
+    quat_new = quat_integrate(quat, om_next, dt)
+    return np.concatenate([p_new, quat_new, dp_next, om_next])
+
+
+def dynamics_jac(c, x, u, t, parameter):
+    """(f_x 13 x 13, f_u 13 x 12), analytic."""
+    mass, I, Iinv, dt = c["mass"], c["inertia"], c["inertia_inv"], c["dt"]
+    p, quat, om = x[:3], x[3:7], x[10:13]
+    contact, legs = parameter[t, :4], parameter[t, 4:]
+    f = [u[3 * i:3 * i + 3] for i in range(NC)]
+    wrench = sum(np.cross(legs[3 * i:3 * i + 3] - p, f[i]) * contact[i] for i in range(NC))
+    Rm = quat_to_mat(quat)
+    dR = quat_to_mat_grad(quat)
+    fx = np.zeros((13, 13))
+    fu = np.zeros((13, 12))
+    # dp_next
+    fx[7:10, 7:10] = np.eye(3)
+    for i in range(NC):
+        fu[7:10, 3 * i:3 * i + 3] = np.eye(3) * contact[i] * dt / mass
+    # omega_next = om + Iinv (R' w - om x I om) dt
+    dw_dp = sum(_skew(f[i]) * contact[i] for i in range(NC))          # d((l - p) x f)/dp = [f]x
+    Ji = Iinv * dt
+    fx[10:13, 0:3] = Ji @ Rm.T @ dw_dp
+    for k in range(4):
+        fx[10:13, 3 + k] = Ji @ (dR[k].T @ wrench)
+    fx[10:13, 10:13] = np.eye(3) + Ji @ (-_skew(om) @ I + _skew(I @ om))
+    for i in range(NC):
+        fu[10:13, 3 * i:3 * i + 3] = Ji @ Rm.T @ _skew(legs[3 * i:3 * i + 3] - p) * contact[i]
+    # p_new = p + dp_next dt
+    fx[0:3, 0:3] = np.eye(3)
+    fx[0:3, :] += dt * fx[7:10, :]
+    fu[0:3, :] = dt * fu[7:10, :]
+    # quat_new = quat_integrate(quat, omega_next, dt)
+    om_next = dynamics(c, x, u, t, parameter)[10:13]
+    Jq, Jv = quat_integrate_jac(quat, om_next, dt)
+    fx[3:7, 3:7] = Jq
+    fx[3:7, :] += Jv @ fx[10:13, :]
+    fu[3:7, :] = Jv @ fu[10:13, :]
+    return fx, fu
+
+
+# --------------------------------------------------------------------------
+# objective (mpx/utils/objectives.py:7-75)
+# --------------------------------------------------------------------------
+
+ALPHA_P, SIGMA_P = 0.1, 5.0
+
+
+def _safe_log(x):
+    return np.log(np.clip(x, 1e-10, 1e6))
+
+
+def penalty(cv, alpha=ALPHA_P, sigma=SIGMA_P):
+    """Relaxed log barrier of objectives.py:7-13, elementwise."""
+    quad = alpha / 2 * (np.square((cv - 2 * sigma) / sigma) - 1.0)
+    logb = -alpha * _safe_log(cv)
+    return np.clip(np.where(cv > sigma, logb, quad + logb), 0.0, 1e8)
+
+
+def penalty_d(cv, alpha=ALPHA_P, sigma=SIGMA_P):
+    """(penalty', penalty'') elementwise, with the derivative conventions of
+    the clips: zero where a clip is active (value outside its range), the
+    log's own derivative inside."""
+    cv = np.asarray(cv, dtype=float)
+    val = penalty(cv, alpha, sigma)
+    inlog = (cv > 1e-10) & (cv < 1e6)
+    dlog = np.where(inlog, 1.0 / cv, 0.0)
+    d2log = np.where(inlog, -1.0 / (cv * cv), 0.0)
+    d1 = np.where(cv > sigma, -alpha * dlog, alpha * (cv - 2 * sigma) / sigma ** 2 - alpha * dlog)
+    d2 = np.where(cv > sigma, -alpha * d2log, alpha / sigma ** 2 - alpha * d2log)
+    live = (val > 0.0) & (val < 1e8)
+    return np.where(live, d1, 0.0), np.where(live, d2, 0.0)
+
+
+def friction_cone(u):
+    fx, fy, fz = u[0::3], u[1::3], u[2::3]
+    return MU * fz - np.sqrt(fx * fx + fy * fy + 1e-2)
+
+
+def friction_cone_jac(u):
+    J = np.zeros((NC, N_CTRL))
+    for i in range(NC):
+        fx, fy = u[3 * i], u[3 * i + 1]
+        r = np.sqrt(fx * fx + fy * fy + 1e-2)
+        J[i, 3 * i] = -fx / r
+        J[i, 3 * i + 1] = -fy / r
+        J[i, 3 * i + 2] = MU
+    return J
+
+
+def residual(x, u, ref_t):
+    return np.concatenate([x[:3] - ref_t[:3], quat_sub(x[3:7], ref_t[3:7]), x[7:10] - ref_t[7:10],
+                           x[10:13] - ref_t[10:13], u])
+
+
+def cost(W, reference, x, u, t, N):
+    """l_t of quadruped_srbd_obj (0.5 x the weighted residual, stage penalties)."""
+    r = residual(x, u, reference[t])
+    if t == N:
+        rx = r[:12]
+        return 0.5 * rx @ W[:12, :12] @ rx
+    pen = np.clip(penalty(friction_cone(u)), 1e-6, 1e6)
+    return 0.5 * (r @ W @ r + np.sum(pen))
+
+
+def cost_grad(W, reference, x, u, t, N):
+    """(grad_x l_t, grad_u l_t), analytic."""
+    r = residual(x, u, reference[t])
+    Jx = np.zeros((24, 13))
+    Jx[0:3, 0:3] = np.eye(3)
+    Jx[3:6, 3:7] = quat_sub_jac(x[3:7], reference[t][3:7])
+    Jx[6:9, 7:10] = np.eye(3)
+    Jx[9:12, 10:13] = np.eye(3)
+    if t == N:
+        return Jx[:12].T @ (W[:12, :12] @ r[:12]), np.zeros(N_CTRL)
+    gx = Jx.T @ (W @ r)
+    fc = friction_cone(u)
+    pv = penalty(fc)
+    d1, _ = penalty_d(fc)
+    live = (pv > 1e-6) & (pv < 1e6)          # the outer clip of objectives.py:26
+    gu = (W @ r)[12:] + 0.5 * friction_cone_jac(u).T @ np.where(live, d1, 0.0)
+    return gx, gu
+
+
+def hessian_gn(W, reference, x, u, t):
+    """quadruped_srbd_hessian_gn: (J_x'W J_x, J_u'W J_u + H_c, J_x'W J_u)."""
+    Jx = np.zeros((24, 13))
+    Jx[0:3, 0:3] = np.eye(3)
+    Jx[3:6, 3:7] = quat_sub_jac(x[3:7], reference[t][3:7])
+    Jx[6:9, 7:10] = np.eye(3)
+    Jx[9:12, 10:13] = np.eye(3)
+    Ju = np.zeros((24, 12))
+    Ju[12:] = np.eye(12)
+    contact = reference[t][13:13 + NC]
+    _, d2 = penalty_d(friction_cone(u))
+    h = np.clip(d2, 1e-6, 1e6) * contact
+    Jf = friction_cone_jac(u)
+    Hc = Jf.T @ np.diag(h) @ Jf
+    return Jx.T @ W @ Jx, Ju.T @ W @ Ju + Hc, Jx.T @ W @ Ju
+
+
+def caller_instance(consts, W, reference, parameter, x0, N, p=0, D=None, E=None, e=None):
+    """The SRBD problem as an oracle caller instance (pdilqr ``model == "caller"``)."""
+    inst = dict(n=N_STATE, m=N_CTRL, N=N, p=p, model="caller", x0=np.asarray(x0, float),
+                ulo=None, uhi=None)
+    inst["f"] = lambda x, u, t: dynamics(consts, x, u, t, parameter)
+    inst["fjac"] = lambda x, u, t: dynamics_jac(consts, x, u, t, parameter)
+    inst["cost"] = lambda x, u, t: cost(W, reference, x, u, t, N)
+    inst["grad"] = lambda x, u, t: cost_grad(W, reference, x, u, t, N)
+
+    def hess(x, u, t):
+        Qh, Rh, Mh = hessian_gn(W, reference, x, u, t)
+        return Qh, Rh, Mh
+    inst["hess"] = hess
+    inst["D"] = np.zeros((N, p, N_STATE)) if D is None else D
+    inst["E"] = np.zeros((N, p, N_CTRL)) if E is None else E
+    inst["e"] = np.zeros((N, p)) if e is None else e
+    return inst

@@ -28,3 +28,15 @@ def test_c_port_matches_numpy_oracle(wl, shape, start, iters):
             if b.size:
                 assert np.max(np.abs(out[k][i] - b)) / max(1.0, np.max(np.abs(b))) < 1e-11, key
         assert np.allclose(out[5][i], [r["alpha"] for r in ref], rtol=0, atol=1e-14)
+
+
+def test_fp32_build_is_fp32_arithmetic():
+    """The FP32 build of the restatement (the error model of the device's FP32
+    path) differs from FP64 at FP32 rounding level, neither 0 nor large."""
+    insts = [syn.make_instance(3, i, n=10, m=4, N=12, p=3) for i in range(4)]
+    starts = [syn.initial_iterate(d, "rollout") for d in insts]
+    a = cport.solve_batch(insts, starts, 2)
+    b = cport.solve_batch(insts, starts, 2, f32=True)
+    for x, y in zip(a[:3], b[:3]):
+        err = np.max(np.abs(x - y)) / max(1.0, np.max(np.abs(x)))
+        assert 1e-9 < err < 1e-3, err

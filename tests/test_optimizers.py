@@ -119,3 +119,52 @@ def test_model_evaluator_helper():
     g_ref, c_ref = orc.model_evaluator_helper(d, X, U)
     assert np.allclose(g, g_ref, rtol=1e-12, atol=1e-12)
     assert np.allclose(c, c_ref, rtol=1e-12, atol=1e-12)
+
+
+def test_mpc_refuses_dropped_hessian_and_parameter():
+    """The device model families take no caller Hessian and no stage
+    parameters: passing them is an error, not a silently different problem."""
+    from mpx_amd import optimizers
+    d = syn.make_instance(2, 0, n=5, m=2, N=4)
+    X, U, V, _, _ = syn.initial_iterate(d, "rollout")
+    dyn, cost, _ = _parts(d)
+    with pytest.raises(ValueError, match="Hessian"):
+        optimizers.mpc(cost, dyn, lambda *a: None, False, (d["xref"], d["uref"]), None, None, d["x0"], X, U, V)
+    with pytest.raises(ValueError, match="parameter"):
+        optimizers.mpc(cost, dyn, None, False, (d["xref"], d["uref"]), np.zeros((5, 3)), None, d["x0"], X, U, V)
+
+
+def _reference_style(inst):
+    """The caller problem as the reference's callables: dynamics(x, u, t,
+    parameter), cost(W, reference, x, u, t), hessian_approx(W, reference, x,
+    u, t) (mpx/utils/models.py:6, mpx/utils/objectives.py:15, :75)."""
+    dyn = lambda x, u, t, parameter: inst["f"](x, u, t)
+    cost = lambda W, reference, x, u, t: inst["cost"](x, u, t)
+    hess = lambda W, reference, x, u, t: inst["hess"](x, u, t)
+    jac = lambda x, u, t, parameter: inst["fjac"](x, u, t)
+    grad = lambda W, reference, x, u, t: inst["grad"](x, u, t)
+    return dyn, cost, hess, jac, grad
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("exact", [True, False])
+def test_mpc_callable_dynamics_matches_oracle(exact):
+    """The reference's own callables (nonlinear dynamics, non-quadratic cost,
+    cross term M) through ``mpc``: equal to the oracle's step on the same
+    functions, to 1e-9 with exact derivatives and to 1e-5 with the central
+    differences used when none are given."""
+    from caller_problem import make_caller_instance, start
+    from mpx_amd import optimizers
+    from mpx_amd.models import Constraints
+    inst = make_caller_instance(1, n=7, m=3, N=8, p=2, box=True)
+    X, U, V, Ye, Yb = start(inst)
+    dyn, cost, hess, jac, grad = _reference_style(inst)
+    cons = Constraints(D=inst["D"], E=inst["E"], e=inst["e"], u_lo=inst["ulo"], u_hi=inst["uhi"])
+    out = optimizers.mpc(cost, dyn, hess, False, None, None, None, inst["x0"], X, U, V, constraints=cons,
+                         V_eq=Ye, V_ineq=Yb, dtype=torch.float64,
+                         dynamics_jac=jac if exact else None, cost_grad=grad if exact else None)
+    ref = orc.sqp_step(inst, X, U, V, Ye, Yb)
+    tol = 1e-9 if exact else 1e-5
+    for name, got in zip(("X", "U", "V", "Yeq", "Ybox"), out):
+        want = ref[name]
+        assert np.max(np.abs(got - want)) <= tol * max(1.0, np.max(np.abs(want))), name

@@ -322,3 +322,87 @@ def test_riccati_with_floor_matches_dense_kkt_of_modified_problem():
     ref = orc.lqr_dense(L2)
     for k in ("dx", "du", "lam"):
         assert _rel(sol[k], ref[k]) < 1e-9, k
+
+
+# ---------------------------------------------------------------------------
+# the caller-function form (MPXKKT_MODEL_CALLER)
+# ---------------------------------------------------------------------------
+
+def test_caller_problem_derivatives_finite_difference():
+    """The test problem's own f_x, f_u and gradients against central differences."""
+    from caller_problem import make_caller_instance
+    inst = make_caller_instance(0, n=6, m=3, N=4)
+    rng = np.random.default_rng(3)
+    x, u, h = rng.standard_normal(6), rng.standard_normal(3), 1e-6
+    fx, fu = inst["fjac"](x, u, 1)
+    q, r = inst["grad"](x, u, 1)
+    for j in range(6):
+        e = np.zeros(6); e[j] = h
+        np.testing.assert_allclose(fx[:, j], (inst["f"](x + e, u, 1) - inst["f"](x - e, u, 1)) / (2 * h),
+                                   atol=1e-8)
+        assert abs(q[j] - (inst["cost"](x + e, u, 1) - inst["cost"](x - e, u, 1)) / (2 * h)) < 1e-6
+    for j in range(3):
+        e = np.zeros(3); e[j] = h
+        np.testing.assert_allclose(fu[:, j], (inst["f"](x, u + e, 1) - inst["f"](x, u - e, 1)) / (2 * h),
+                                   atol=1e-8)
+        assert abs(r[j] - (inst["cost"](x, u + e, 1) - inst["cost"](x, u - e, 1)) / (2 * h)) < 1e-6
+
+
+def _caller_from(inst):
+    """The same instance written as caller functions (exact Hessian, M = 0)."""
+    N, n, m = inst["N"], inst["n"], inst["m"]
+    c = dict(n=n, m=m, N=N, p=inst["p"], model="caller", x0=inst["x0"], D=inst["D"], E=inst["E"],
+             e=inst["e"], ulo=inst.get("ulo"), uhi=inst.get("uhi"))
+    c["f"] = lambda x, u, t: orc.dynamics(inst, x, u, t)
+    c["fjac"] = lambda x, u, t: orc.dynamics_jac(inst, x, u, t)
+    c["cost"] = lambda x, u, t: orc.stage_cost(inst, x, u, t)
+
+    def grad(x, u, t):
+        if t == N:
+            return inst["QN"] @ (x - inst["xref"][N]), np.zeros(m)
+        return orc.Q_at(inst, t) @ (x - inst["xref"][t]), orc.R_at(inst, t) @ (u - inst["uref"][t])
+
+    def hess(x, u, t):
+        if t == N:
+            return inst["QN"], np.zeros((m, m)), np.zeros((n, m))
+        return orc.Q_at(inst, t), orc.R_at(inst, t), np.zeros((n, m))
+    c["grad"], c["hess"] = grad, hess
+    return c
+
+
+@pytest.mark.parametrize("wl,shape", [(2, dict(n=6, m=3, N=8)), (4, dict(n=8, m=4, N=6, p=2))])
+def test_caller_form_equals_device_model_family(wl, shape):
+    """A tanh-family instance given as caller functions takes the same SQP step
+    as the family itself (pins the caller form's plumbing: gradients, Hessians,
+    defects, the merit's cost terms)."""
+    inst = syn.make_instance(wl, 0, **shape)
+    start = syn.initial_iterate(inst, "rollout")
+    a = orc.sqp_step(inst, *start)
+    b = orc.sqp_step(_caller_from(inst), *start)
+    for k in ("X", "U", "V", "Yeq", "Ybox"):
+        np.testing.assert_allclose(b[k], a[k], rtol=0, atol=1e-12)
+    assert a["alpha"] == b["alpha"] and a["trials"] == b["trials"]
+
+
+def test_caller_lq_with_cross_term_one_step_exact():
+    """Linear dynamics and a quadratic cost with a cross term M (the Hessian
+    triple is exact): one SQP step from any start reaches the optimum."""
+    from caller_problem import make_caller_instance
+    rng = np.random.default_rng(5)
+    n, m, N = 5, 3, 7
+    A = np.eye(n) + 0.05 * rng.standard_normal((n, n))
+    B = 0.1 * rng.standard_normal((n, m))
+    Q = np.diag(rng.uniform(1, 5, n))
+    R = np.diag(rng.uniform(0.5, 1, m))
+    M = 0.2 * rng.standard_normal((n, m))
+    inst = make_caller_instance(0, n=n, m=m, N=N)
+    inst["f"] = lambda x, u, t: A @ x + B @ u
+    inst["fjac"] = lambda x, u, t: (A, B)
+    inst["cost"] = lambda x, u, t: (0.5 * x @ Q @ x if t == N else 0.5 * x @ Q @ x + 0.5 * u @ R @ u + x @ M @ u)
+    inst["grad"] = lambda x, u, t: ((Q @ x, np.zeros(m)) if t == N else (Q @ x + M @ u, R @ u + M.T @ x))
+    inst["hess"] = lambda x, u, t: ((Q, np.zeros((m, m)), np.zeros((n, m))) if t == N else (Q, R, M))
+    X = rng.standard_normal((N + 1, n))
+    U = rng.standard_normal((N, m))
+    st = orc.sqp_step(inst, X, U, np.zeros((N + 1, n)), np.zeros((N, 0)), np.zeros((N, 2 * m)))
+    assert st["alpha"] == 1.0
+    assert orc.kkt_residual(inst, st["X"], st["U"], st["V"], st["Yeq"], st["Ybox"]) < 1e-10
