@@ -84,6 +84,57 @@ def cpu_baseline(insts, starts, budget_s):
                 sample=f"{done} instance-iterations of the numpy FP64 oracle, 1 thread")
 
 
+def config2_leg(plain_sample=16, iters=5, seed=0):
+    """Config 2: B=1024, n=24, m=12, N=100, |u| <= 1, tanh dynamics, FP32, 5 SQP
+    iterations from the rollout warm start; batched (one launch sequence for
+    all instances) against plain (one instance per launch sequence, timed on
+    a sample).  Returns SQP iterations per second over the whole batch."""
+    import torch
+    from mpx_amd import synthetic as syn
+    from mpx_amd.problem import DeviceBatch, SQPSolver
+    B = syn.WORKLOADS[2]["batch"]
+    insts = [syn.make_instance(2, i, seed) for i in range(B)]
+    starts = [syn.initial_iterate(d, "rollout") for d in insts]
+    batch = DeviceBatch.from_instances(insts, dtype=torch.float32)
+    host = [np.stack([s[k] for s in starts]) for k in range(5)]
+    solver = SQPSolver(batch)
+    it = batch.iterate_from(*host)
+    solver.iterate(it)                          # one-time set-up outside the timing
+    reps = 3
+    ms = []
+    for _ in range(reps):
+        it = batch.iterate_from(*host)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(iters):
+            solver.iterate(it)
+        e1.record()
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1) / iters)
+    ms_iter = float(np.median(ms))
+    S = min(plain_sample, B)
+    singles = []
+    for i in range(S):
+        b1 = DeviceBatch.from_instances([insts[i]], dtype=torch.float32)
+        singles.append((SQPSolver(b1), b1.iterate_from(*[s[None] for s in starts[i]])))
+    for s1, it1 in singles:
+        s1.iterate(it1)
+    torch.cuda.synchronize()
+    p0 = time.perf_counter()
+    for s1, it1 in singles:
+        for _ in range(iters):
+            s1.iterate(it1)
+    torch.cuda.synchronize()
+    plain_per_inst_iter = (time.perf_counter() - p0) / (iters * S)
+    batched = 1e3 / ms_iter
+    plain = 1.0 / (plain_per_inst_iter * B)
+    return {"workload": "B=1024 n=24 m=12 N=100 |u|<=1, tanh dynamics, 5 iterations from the rollout start",
+            "ms_per_iteration": ms_iter, "batched_iters_per_s": batched, "plain_iters_per_s": plain,
+            "batched_vs_plain": batched / plain,
+            "plain_sample": f"{S} instances x {iters} iterations, one launch sequence each"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +152,7 @@ def main():
     ap.add_argument("--serve-sweep", default="10000,40000",
                     help="further arrival rates (comma list, '' for none), each served for --serve seconds "
                          "with a wall-clock cap; reported in serving_sweep")
+    ap.add_argument("--no-config2", action="store_true", help="skip the config-2 batched-vs-plain leg")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB: only copy a buffer of this size twice (reference for the HBM byte counters)")
     args = ap.parse_args()
@@ -208,6 +260,12 @@ def main():
     log(f"plain: {plain_per_inst_iter * 1e6:.1f} us per instance-iteration (tune {tune1}); "
         f"batched/plain = {value / plain_iters_per_s:.1f}x")
 
+    config2 = None
+    if not args.no_config2:
+        config2 = config2_leg(seed=args.seed)
+        log(f"config 2: {config2['ms_per_iteration']:.3f} ms per batched iteration, "
+            f"batched/plain = {config2['batched_vs_plain']:.1f}x")
+
     cpu = None
     if not args.no_cpu:
         c = cpu_baseline(insts, starts, args.cpu_budget)
@@ -254,6 +312,7 @@ def main():
         "mean_trials": trials, "nonzero_status": status_bad,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "config2": config2,
         "serving": serving,
         "serving_sweep": serving_sweep,
     }

@@ -191,6 +191,33 @@ mpxkkt_status mpxkkt_sqp_accept(void* stream, const mpxkkt_problem* prob, const 
                                 const void* trial_f, mpxkkt_stats* stats, void* workspace,
                                 size_t workspace_bytes);
 
+/* The single-rigid-body quadruped of the reference's SRBD controller,
+ * evaluated on the device: dynamics quadruped_srbd_dynamics
+ * (mpx/utils/models.py:6-37), cost quadruped_srbd_obj and Hessian triple
+ * quadruped_srbd_hessian_gn (mpx/utils/objectives.py:7-75), bound as at
+ * mpx/utils/mpc_wrapper_srbd.py:58-68.  State x = [p, quat (w first), dp,
+ * omega] (n = 13), control = ground reaction forces of 4 feet (m = 12).
+ * Per instance: W (24 x 24 residual weight, stage stride 0), reference
+ * [N+1][17] = [p_ref, quat_ref, dp_ref, omega_ref, contact (4)], parameter
+ * [N+1][16] = [contact (4), p_legs (12)]. */
+typedef struct {
+  double mass, dt;
+  double inertia[9], inertia_inv[9];  /* row-major */
+  mpxkkt_view W, reference, parameter;
+} mpxkkt_srbd;
+
+/* One SQP iteration of the SRBD problem with no host involvement: the
+ * linearisation (analytic Jacobians, gradient, Gauss-Newton Hessian triple)
+ * into the caller views of `prob` (MPXKKT_MODEL_CALLER, n = 13, m = 12; its
+ * A, B, fbar, Q, R, QN, q, r, g -- and M when non-NULL -- must point at
+ * writable device buffers, overwritten here), then mpxkkt_sqp_direction's
+ * work, the model's cost and dynamics at the max_trials trial points, and
+ * mpxkkt_sqp_accept's work.  Replaces the jit(vmap(optimizers.mpc)) call of
+ * the SRBD controller (mpx/utils/mpc_wrapper_srbd.py:68, :77). */
+mpxkkt_status mpxkkt_srbd_iterate(void* stream, const mpxkkt_problem* prob, const mpxkkt_srbd* model,
+                                  const mpxkkt_params* prm, mpxkkt_iterate* it, mpxkkt_stats* stats,
+                                  void* workspace, size_t workspace_bytes);
+
 /* Options of mpxkkt_solve: the iteration protocol of runOffline
  * (mpx/utils/mpc_wrapper.py:425-467). */
 typedef struct {

@@ -22,6 +22,7 @@ EXPORTED = (
     "mpxkkt_sqp_iterate",
     "mpxkkt_sqp_direction",
     "mpxkkt_sqp_accept",
+    "mpxkkt_srbd_iterate",
     "mpxkkt_default_solve_opts",
     "mpxkkt_solve",
     "mpxkkt_kkt_residual",
@@ -82,6 +83,11 @@ class Direction(C.Structure):
     _fields_ = [("dX", C.c_void_p), ("dU", C.c_void_p), ("alpha_max", C.c_void_p)]
 
 
+class Srbd(C.Structure):
+    _fields_ = [("mass", C.c_double), ("dt", C.c_double), ("inertia", C.c_double * 9),
+                ("inertia_inv", C.c_double * 9), ("W", View), ("reference", View), ("parameter", View)]
+
+
 class SolveOpts(C.Structure):
     _fields_ = [("max_iter", C.c_int32), ("check_every", C.c_int32),
                 ("dcost_tol", C.c_double), ("cc_tol", C.c_double)]
@@ -112,6 +118,8 @@ def load():
                                          C.c_void_p, C.c_size_t]
     lib.mpxkkt_sqp_accept.argtypes = [C.c_void_p, P(Problem), P(Params), P(Iterate), C.c_int32, C.c_void_p,
                                       C.c_void_p, P(Stats), C.c_void_p, C.c_size_t]
+    lib.mpxkkt_srbd_iterate.argtypes = [C.c_void_p, P(Problem), P(Srbd), P(Params), P(Iterate), P(Stats),
+                                        C.c_void_p, C.c_size_t]
     lib.mpxkkt_default_solve_opts.argtypes = [P(SolveOpts)]
     lib.mpxkkt_solve.argtypes = [C.c_void_p, P(Problem), P(Params), P(SolveOpts), P(Iterate), P(Stats),
                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,

@@ -76,6 +76,8 @@ struct Ws {
   T* last;   // [B]             previous iteration's sum g_t^2 (stop test)
   int* run;  // [B]             1 while the instance iterates; nullptr: every instance runs
   int* nrun; // [1]             instances still running (host check of the solve loop)
+  T* trial_g; // [B][L][N1]     trial stage costs of a device-evaluated caller model (SRBD)
+  T* trial_f; // [B][L][N][n]   trial next states of that model
 };
 
 // Instances whose run flag is 0 (converged in a multi-iteration solve) leave

@@ -22,6 +22,7 @@
 #include "k_trials.h"
 #include "k_misc.h"
 #include "k_caller.h"
+#include "k_srbd.h"
 
 using namespace mpxk;
 
@@ -124,6 +125,9 @@ size_t carve(const mpxkkt_problem* pr, int P, bool wave, Ws<T>* ws, char* base) 
   ws->last = take(B);
   ws->run = reinterpret_cast<int*>(take(B));
   ws->nrun = reinterpret_cast<int*>(take(1));
+  const bool caller = pr->model == MPXKKT_MODEL_CALLER;
+  ws->trial_g = take(caller ? B * MAX_TRIALS * N1 : 0);
+  ws->trial_f = take(caller ? B * MAX_TRIALS * N * n : 0);
   return off + 256;
 }
 
@@ -675,6 +679,7 @@ mpxkkt_status direction_t(hipStream_t st, const mpxkkt_problem* pr, const mpxkkt
   if ((s = run_fwd_stage<T>(st, pr, P, R, I, W, in)) != MPXKKT_OK) return s;
   hipLaunchKernelGGL(k_ls_setup<T>, dim3(pr->batch), dim3(256), (size_t)6 * 256 * sizeof(T), st, P, R, I, W);
   if ((s = launch_check("k_ls_setup")) != MPXKKT_OK) return s;
+  if (!dir->dX) return MPXKKT_OK;
   hipLaunchKernelGGL(k_export_dir<T>, dim3(pr->batch), dim3(256), 0, st, P, W, static_cast<T*>(dir->dX),
                      static_cast<T*>(dir->dU), static_cast<T*>(dir->alpha_max));
   return launch_check("k_export_dir");
@@ -707,6 +712,45 @@ mpxkkt_status accept_t(hipStream_t st, const mpxkkt_problem* pr, const mpxkkt_pa
   int* tr = so ? so->trials : nullptr;
   hipLaunchKernelGGL(k_ls_accept<T>, grid, dim3(64 * kTrialWaves), 0, st, P, R, I, W, a, r, f0, df, stt, tr);
   return launch_check("k_ls_accept");
+}
+
+template <typename T>
+Srbd<T> make_srbd(const mpxkkt_srbd* m) {
+  Srbd<T> S;
+  S.mass = (T)m->mass;
+  S.dt = (T)m->dt;
+  for (int k = 0; k < 9; ++k) { S.I[k] = (T)m->inertia[k]; S.Iinv[k] = (T)m->inertia_inv[k]; }
+  S.W = view<T>(m->W);
+  S.ref = view<T>(m->reference);
+  S.par = view<T>(m->parameter);
+  return S;
+}
+
+// One SQP iteration of the SRBD quadruped entirely on the device: its
+// linearisation into the problem's caller views, the direction, the trial
+// evaluations and the acceptance.
+template <typename T>
+mpxkkt_status srbd_iterate_t(hipStream_t st, const mpxkkt_problem* pr, const mpxkkt_srbd* model,
+                             const mpxkkt_params* prm, mpxkkt_iterate* it, mpxkkt_stats* so, void* wsp) {
+  Ws<T> W;
+  carve<T>(pr, 1, false, &W, static_cast<char*>(wsp));
+  W.run = nullptr;
+  const Prob<T> P = make_prob<T>(pr);
+  const Srbd<T> S = make_srbd<T>(model);
+  const Iter<T> I = make_iter<T>(it);
+  const int B = pr->batch, N1 = pr->horizon + 1;
+  const int L = std::max(1, std::min(prm->max_trials, MAX_TRIALS));
+  hipLaunchKernelGGL(k_srbd_linearize<T>, dim3((B * N1 + 63) / 64), dim3(64), 0, st, P, S, I);
+  mpxkkt_status s = launch_check("k_srbd_linearize");
+  if (s != MPXKKT_OK) return s;
+  mpxkkt_direction dir;
+  dir.dX = nullptr;
+  // the step stays in the workspace (direction_t exports only when asked)
+  if ((s = direction_t<T>(st, pr, prm, it, &dir, wsp)) != MPXKKT_OK) return s;
+  hipLaunchKernelGGL(k_srbd_trials<T>, dim3((B * L * N1 + 63) / 64), dim3(64), 0, st, P, S, I, W,
+                     (T)prm->beta, L);
+  if ((s = launch_check("k_srbd_trials")) != MPXKKT_OK) return s;
+  return accept_t<T>(st, pr, prm, it, L, W.trial_g, W.trial_f, so, wsp);
 }
 
 // SQP iterations until runOffline's stop rule (mpx/utils/mpc_wrapper.py:425-467)
@@ -839,6 +883,30 @@ mpxkkt_status mpxkkt_sqp_accept(void* stream, const mpxkkt_problem* prob, const 
   if (prob->dtype == MPXKKT_F64)
     return accept_t<double>(st, prob, prm, it, n_trials, trial_g, trial_f, stats, workspace);
   return accept_t<float>(st, prob, prm, it, n_trials, trial_g, trial_f, stats, workspace);
+}
+
+mpxkkt_status mpxkkt_srbd_iterate(void* stream, const mpxkkt_problem* prob, const mpxkkt_srbd* model,
+                                  const mpxkkt_params* prm, mpxkkt_iterate* it, mpxkkt_stats* stats,
+                                  void* workspace, size_t workspace_bytes_) {
+  mpxkkt_status s = check_problem(prob);
+  if (s != MPXKKT_OK) return s;
+  if (prob->model != MPXKKT_MODEL_CALLER || prob->n != 13 || prob->m != 12)
+    return fail(MPXKKT_ERR_ARG, "the SRBD model needs a MPXKKT_MODEL_CALLER problem with n = 13, m = 12");
+  if (!prm || !model) return fail(MPXKKT_ERR_ARG, "null params or model");
+  if (!model->W.ptr || !model->reference.ptr || !model->parameter.ptr)
+    return fail(MPXKKT_ERR_ARG, "SRBD model needs W, reference and parameter views");
+  if (model->W.stage_stride != 0 || model->W.batch_stride < 0 || model->reference.batch_stride < 0 ||
+      model->reference.stage_stride < 0 || model->parameter.batch_stride < 0 || model->parameter.stage_stride < 0)
+    return fail(MPXKKT_ERR_ARG, "bad SRBD view strides (W is per instance: stage stride 0)");
+  if (!(model->mass > 0.0) || !(model->dt > 0.0)) return fail(MPXKKT_ERR_ARG, "mass and dt must be positive");
+  if ((s = check_iterate(prob, it)) != MPXKKT_OK) return s;
+  if ((s = check_params(prm)) != MPXKKT_OK) return s;
+  const size_t need = workspace_bytes(prob, prm);
+  if (!workspace || workspace_bytes_ < need)
+    return fail(MPXKKT_ERR_WORKSPACE, "workspace needs " + std::to_string(need) + " bytes");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (prob->dtype == MPXKKT_F64) return srbd_iterate_t<double>(st, prob, model, prm, it, stats, workspace);
+  return srbd_iterate_t<float>(st, prob, model, prm, it, stats, workspace);
 }
 
 mpxkkt_status mpxkkt_default_solve_opts(mpxkkt_solve_opts* out) {

@@ -15,9 +15,15 @@
  * arithmetic: the error model of the device's FP32 path). */
 #ifdef PDILQR_F32
 typedef float real;
+/* FP32 library functions and constants in the FP32 build */
+#define tanh tanhf
+#define log logf
+#define sqrt sqrtf
+#define fabs fabsf
 #else
 typedef double real;
 #endif
+#define RC(x) ((real)(x))
 
 typedef struct {
   int n, m, N, p, model, box, q_tv, r_tv;
@@ -106,13 +112,13 @@ static void dyn(const inst_t* I, const real* x, const real* u, int t, real* f, r
     for (int j = 0; j < n; ++j) { ax += I->A[i * n + j] * x[j]; wx += I->W[i * n + j] * x[j]; }
     for (int j = 0; j < m; ++j) bu += I->B[i * m + j] * u[j];
     const real th = tanh(wx);
-    wk[i] = 1.0 - th * th;
+    wk[i] = RC(1.0) - th * th;
     f[i] = x[i] + I->h * (ax + bu + I->kappa * th);
   }
   if (fx)
     for (int i = 0; i < n; ++i)
       for (int j = 0; j < n; ++j)
-        fx[i * n + j] = (i == j ? 1.0 : 0.0) + I->h * (I->A[i * n + j] + I->kappa * wk[i] * I->W[i * n + j]);
+        fx[i * n + j] = (i == j ? RC(1.0) : RC(0.0)) + I->h * (I->A[i * n + j] + I->kappa * wk[i] * I->W[i * n + j]);
 }
 
 static void fu_of(const inst_t* I, int t, real* fu) {
@@ -131,7 +137,7 @@ static real al_cost(const inst_t* I, const real* x, const real* u, int t, const 
   for (int i = 0; i < n; ++i) {
     real qx = 0;
     for (int j = 0; j < n; ++j) qx += Q[i * n + j] * (x[j] - xr[j]);
-    val += 0.5 * (x[i] - xr[i]) * qx;
+    val += RC(0.5) * (x[i] - xr[i]) * qx;
   }
   if (t == N) return val;
   const real* R = Rt(I, t);
@@ -139,7 +145,7 @@ static real al_cost(const inst_t* I, const real* x, const real* u, int t, const 
   for (int i = 0; i < m; ++i) {
     real ru = 0;
     for (int j = 0; j < m; ++j) ru += R[i * m + j] * (u[j] - ur[j]);
-    val += 0.5 * (u[i] - ur[i]) * ru;
+    val += RC(0.5) * (u[i] - ur[i]) * ru;
   }
   for (int r = 0; r < p; ++r) {
     const real* D = I->D + ((size_t)t * p + r) * n;
@@ -147,7 +153,7 @@ static real al_cost(const inst_t* I, const real* x, const real* u, int t, const 
     real hv = I->e[(size_t)t * p + r];
     for (int j = 0; j < n; ++j) hv += D[j] * x[j];
     for (int j = 0; j < m; ++j) hv += E[j] * u[j];
-    val += yeq[r] * hv + 0.5 * prm[P_RHOC] * hv * hv;
+    val += yeq[r] * hv + RC(0.5) * prm[P_RHOC] * hv * hv;
   }
   if (I->box)
     for (int i = 0; i < m; ++i) val -= prm[P_MUB] * (log(I->uhi[i] - u[i]) + log(u[i] - I->ulo[i]));
@@ -162,13 +168,13 @@ static real merit(const inst_t* I, const real* X, const real* U, const real* V, 
                                               t < N ? Yeq + (size_t)t * p : NULL, prm);
   for (int i = 0; i < n; ++i) {
     const real c = I->x0[i] - X[i];
-    val += V[i] * c + 0.5 * rho * c * c;
+    val += V[i] * c + RC(0.5) * rho * c * c;
   }
   for (int t = 0; t < N; ++t) {
     dyn(I, X + (size_t)t * n, U + (size_t)t * m, t, f, NULL, tmp);
     for (int i = 0; i < n; ++i) {
       const real c = f[i] - X[(size_t)(t + 1) * n + i];
-      val += V[(size_t)(t + 1) * n + i] * c + 0.5 * rho * c * c;
+      val += V[(size_t)(t + 1) * n + i] * c + RC(0.5) * rho * c * c;
     }
   }
   return val;
@@ -212,7 +218,7 @@ static real sqp_one(const inst_t* I, real* X, real* U, real* V, real* Yeq, real*
     for (int t = 0; t < N; ++t)
       for (int i = 0; i < m; ++i) {
         real* u = U + (size_t)t * m + i;
-        const real lo = I->ulo[i], hi = I->uhi[i], wd = 1e-3 * (hi - lo);
+        const real lo = I->ulo[i], hi = I->uhi[i], wd = RC(1e-3) * (hi - lo);
         if (*u >= hi) *u = hi - wd;
         if (*u <= lo) *u = lo + wd;
         Sl[(size_t)t * 2 * m + i] = hi - *u;
@@ -315,7 +321,7 @@ static real sqp_one(const inst_t* I, real* X, real* U, real* V, real* Yeq, real*
       }
     for (int i = 0; i < n; ++i)
       for (int j = i + 1; j < n; ++j) {
-        const real a = 0.5 * (St[i * n + j] + St[j * n + i]);
+        const real a = RC(0.5) * (St[i * n + j] + St[j * n + i]);
         St[i * n + j] = St[j * n + i] = a;
       }
     for (int i = 0; i < n; ++i) {
@@ -358,7 +364,7 @@ static real sqp_one(const inst_t* I, real* X, real* U, real* V, real* Yeq, real*
   }
   const real kap = lc - d_f;
   real rho = prm[P_RHO0];
-  if (cc > 0) { const real r2 = 2 * (d_f + d_c + 0.5 * kap) / cc; if (r2 > rho) rho = r2; }
+  if (cc > 0) { const real r2 = 2 * (d_f + d_c + RC(0.5) * kap) / cc; if (r2 > rho) rho = r2; }
   if (rho > prm[P_RHOMAX]) rho = prm[P_RHOMAX];
   const real dphi = d_f + d_c - rho * cc;
   real amax = 1, adual = 1;

@@ -88,8 +88,14 @@ def quat_to_mat_grad(q):
     return [dw, dx, dy, dz]
 
 
+def _is_zero(v):
+    """mjx's norm treats a vector as zero when jnp.allclose(v, 0) holds (every
+    |v_i| <= 1e-8, allclose's default atol); its value and gradient are then 0."""
+    return bool(np.all(np.abs(v) <= 1e-8))
+
+
 def _norm_safe(v):
-    n = float(np.sqrt(v @ v))
+    n = 0.0 if _is_zero(v) else float(np.sqrt(v @ v))
     return n, v / (n + (1e-6 if n == 0.0 else 0.0))
 
 
@@ -115,7 +121,15 @@ def _sinc_half(n, dt):
 
 
 def quat_integrate_jac(q, v, dt):
-    """(d quat_integrate / dq (4 x 4), d / dv (4 x 3)) for v != 0 (generic)."""
+    """(d quat_integrate / dq (4 x 4), d / dv (4 x 3)).  For a zero v in mjx's
+    sense (_is_zero) the rotation is the identity and, as JAX's autodiff of
+    the safe norm gives, d / dv = 0."""
+    if _is_zero(v):
+        qr = np.array([1.0, 0.0, 0.0, 0.0])
+        qn = quat_mul(q, qr)
+        nn = float(np.sqrt(qn @ qn))
+        dnorm = (np.eye(4) - np.outer(qn, qn) / (nn * nn)) / nn
+        return dnorm @ quat_mul_right(qr), np.zeros((4, 3))
     n = float(np.sqrt(v @ v))
     a = 0.5 * dt
     s, ds = _sinc_half(n, dt)
@@ -141,10 +155,13 @@ def quat_sub(u, v):
 
 
 def quat_sub_jac(u, v):
-    """d quat_sub(u, v) / du (3 x 4) for a generic (non-zero vector part) q."""
+    """d quat_sub(u, v) / du (3 x 4); 0 when the vector part of inv(v) u is
+    zero in mjx's sense (JAX's autodiff through the safe norm)."""
     q = quat_mul(quat_inv(v), u)
     dq = quat_mul_left(quat_inv(v))
     qv, w = q[1:], q[0]
+    if _is_zero(qv):
+        return np.zeros((3, 4))
     s = float(np.sqrt(qv @ qv))
     ang = 2.0 * np.arctan2(s, w)
     if ang > np.pi:

@@ -153,3 +153,19 @@ def test_gauss_newton_hessian_structure():
     np.testing.assert_allclose(Q[3:7, 3:7], Jq.T @ W[3:6, 3:6] @ Jq, atol=1e-9)
     # swing legs (contact 0) add nothing beyond W_grf
     np.testing.assert_allclose(R[3:6, 3:6], W[15:18, 15:18], atol=1e-15)
+
+
+def test_zero_vector_conventions():
+    """At a rotation rate or orientation error that is zero in mjx's sense
+    (every component <= 1e-8) the value is the identity rotation / zero error
+    and the derivative through mjx's safe norm is 0, as JAX's autodiff gives."""
+    q = np.array([1.0, 0.0, 0.0, 0.0])
+    Jq, Jv = srbd.quat_integrate_jac(q, np.zeros(3), 0.02)
+    assert np.all(Jv == 0)
+    np.testing.assert_allclose(srbd.quat_integrate(q, np.full(3, 1e-9), 0.02), q, atol=0)
+    assert np.all(srbd.quat_sub_jac(q, q) == 0)
+    assert np.all(srbd.quat_sub(q, q) == 0)
+    # just above the threshold the analytic derivative holds
+    v = np.array([2e-8, 0.0, 0.0])
+    _, Jv = srbd.quat_integrate_jac(q, v, 0.02)
+    np.testing.assert_allclose(Jv[1:], 0.01 * np.eye(3), atol=1e-12)
