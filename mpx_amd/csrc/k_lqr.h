@@ -129,28 +129,54 @@ struct IsF32 { static constexpr bool value = false; };
 template <>
 struct IsF32<float> { static constexpr bool value = true; };
 
-// H <- H^{-1} for the SPD m x m matrix in LDS.  16 < m <= 32: wave 0 factors
-// H = U'U and forms Y = U^{-1} in tmp with the columns in registers
-// (chol_upper_inv_wave), then the block forms H^{-1} = Y Y' on VALU (m^3
-// flops; one more matrix-core product site here pushes the FP32 kernels into
-// register spills).  Other m: Cholesky and the identity solved through both
-// triangles, block-wide in LDS (the register version spills at those template
-// sizes inside the Riccati kernels).
+// [G | g] <- W = Z [G | g] and Kb <- [K | k] = Z' W for H = Z^{-1} Z^{-T}
+// (Z the inverse of H's lower Cholesky factor, pivots floored): H^{-1}[G | g]
+// through the factor, never forming H^{-1} (whose FP32 rounding error is a
+// factor cond(H)^(1/2) larger), and G'H^{-1}G = W'W.  16 < m <= 32: wave 0
+// factors H = U'U and forms Y = U^{-1} = Z' in registers
+// (chol_upper_inv_wave), then two products; other m: Cholesky and the two
+// triangular solves block-wide in LDS (the register version spills at those
+// template sizes inside the Riccati kernels).
 template <typename T, int MMAX>
-__device__ void spd_inverse(T* H, T* tmp, int m, T floor) {
+__device__ void spd_apply_inverse(T* H, T* G, T* Kb, int m, int ncol, int ld, T floor, bool mfma) {
   if constexpr (MMAX == 32) {
-    chol_upper_inv_wave<T, MMAX>(H, tmp, m, floor);
+    chol_upper_inv_wave<T, MMAX>(H, H, m, floor);   // Y = U^{-1} (upper) in H
     lds_sync();
-    gemm_lds<T, false, true, 0>(m, m, m, tmp, m, tmp, m, H, m);
+    if constexpr (IsF32<T>::value) {
+      if (mfma) {
+        const MSeg w0{H, G, m, ld, m, true, false, 1.f};
+        mfma_gemm<false>(m, ncol, 1, w0, w0, w0, Kb, ld, nullptr, 0, nullptr);
+      }
+    }
+    if (!mfma) gemm_lds<T, true, false, 0>(m, ncol, m, H, m, G, ld, Kb, ld);   // W = Y' G
+    lds_sync();
+    for (int e = threadIdx.x; e < m * ncol; e += blockDim.x) {
+      const int i = e / ncol, j = e - i * ncol;
+      G[i * ld + j] = Kb[i * ld + j];
+    }
+    lds_sync();
+    if constexpr (IsF32<T>::value) {
+      if (mfma) {
+        const MSeg k0{H, G, m, ld, m, false, false, 1.f};
+        mfma_gemm<false>(m, ncol, 1, k0, k0, k0, Kb, ld, nullptr, 0, nullptr);
+      }
+    }
+    if (!mfma) gemm_lds<T, false, false, 0>(m, ncol, m, H, m, G, ld, Kb, ld);  // K = Y W
   } else {
-    chol_lds(H, m, floor);
-    for (int e = threadIdx.x; e < m * m; e += blockDim.x) tmp[e] = (e / m == e % m) ? T(1) : T(0);
+    chol_lds(H, m, floor);                           // H = L L', L = Z^{-1}
+    for (int e = threadIdx.x; e < m * ncol; e += blockDim.x) {
+      const int i = e / ncol, j = e - i * ncol;
+      Kb[i * ld + j] = G[i * ld + j];
+    }
     lds_sync();
-    trsm_lower(H, m, tmp, m, m);
+    trsm_lower(H, m, Kb, ncol, ld);                  // W = L^{-1} G
     lds_sync();
-    trsm_upper_t(H, m, tmp, m, m);
+    for (int e = threadIdx.x; e < m * ncol; e += blockDim.x) {
+      const int i = e / ncol, j = e - i * ncol;
+      G[i * ld + j] = Kb[i * ld + j];
+    }
     lds_sync();
-    for (int e = threadIdx.x; e < m * m; e += blockDim.x) H[e] = tmp[e];
+    trsm_upper_t(H, m, Kb, ncol, ld);                // K = L^{-T} W
   }
 }
 
@@ -159,8 +185,8 @@ __device__ void spd_inverse(T* H, T* tmp, int m, T floor) {
 //   [T1 | SB] = S [A | B],  Sc = S c + s
 //   H = R + diag(rd) + rho_c E'E + B'SB,  [G | g] = [M' + rho_c E'D + B'T1 | r + B'Sc]
 //   (M: the caller's cross Hessian, 0 for the device model families)
-//   [K | k] = H^{-1} [G | g]
-//   S_t = Q + rho_c D'D + A'T1 - G'K,  s_t = q + A'Sc - G'k
+//   [W | w] = Z [G | g], [K | k] = Z' [W | w]  (H^{-1} = Z' Z; W replaces G in LDS)
+//   S_t = Q + rho_c D'D + A'T1 - W'W,  s_t = q + A'Sc - W'w
 // FP32 forms the products on the matrix cores (unless flags bit 0), FP64 on VALU.
 template <typename T, int MMAX>
 __device__ void riccati_step(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& ws, int b, int t,
@@ -220,23 +246,14 @@ __device__ void riccati_step(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& w
     L.G[i * ld + n] = v;
   }
   lds_sync();
-  // C: H <- H^{-1}
-  spd_inverse<T, MMAX>(L.H, L.Kb, m, prm.hfloor);
-  lds_sync();
-  // D: [K | k] = H^{-1} [G | g]
-  if constexpr (IsF32<T>::value) {
-    if (mfma) {
-      const MSeg k0{L.H, L.G, m, ld, m, false, false, 1.f};
-      mfma_gemm<false>(m, n + 1, 1, k0, k0, k0, L.Kb, ld, nullptr, 0, nullptr);
-    }
-  }
-  if (!mfma) gemm_lds<T, false, false, 0>(m, n + 1, m, L.H, m, L.G, ld, L.Kb, ld);
+  // C, D: [W | w] = Z [G | g] in G's place, [K | k] = Z' [W | w] in Kb
+  spd_apply_inverse<T, MMAX>(L.H, L.G, L.Kb, m, n + 1, ld, prm.hfloor, mfma);
   lds_sync();
   // E: S_t, s_t ; F_t = A - B K and f_t = c - B k (to the workspace)
   if constexpr (IsF32<T>::value) {
     if (mfma) {
       const MSeg s0{L.A, L.T1, ldab, ldab, n, true, false, 1.f};
-      const MSeg s1{L.G, L.Kb, ld, ld, m, true, false, -1.f};
+      const MSeg s1{L.G, L.G, ld, ld, m, true, false, -1.f};
       const MSeg s2{L.D, L.D, n, n, p, true, false, 1.f};
       mfma_gemm<true>(n, n, p > 0 ? 3 : 2, s0, s1, s2, L.S, n, Q, n, nullptr);
       const MSeg f0{L.B, L.Kb, ldab, ld, m, false, false, -1.f};
@@ -253,7 +270,7 @@ __device__ void riccati_step(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& w
       T v = Q[e];
       for (int k = 0; k < p; ++k) v += L.D[k * n + i] * L.D[k * n + j];
       for (int k = 0; k < n; ++k) v += L.A[k * ldab + i] * L.T1[k * ldab + j];
-      for (int r = 0; r < m; ++r) v -= L.G[r * ld + i] * L.Kb[r * ld + j];
+      for (int r = 0; r < m; ++r) v -= L.G[r * ld + i] * L.G[r * ld + j];
       L.S[i * n + j] = v;
       L.S[j * n + i] = v;
     }
@@ -265,9 +282,8 @@ __device__ void riccati_step(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& w
     T w = L.c[i];
 #pragma unroll 4
     for (int r = 0; r < m; ++r) {
-      const T kr = L.Kb[r * ld + n];
-      v -= L.G[r * ld + i] * kr;
-      w -= L.B[i * ldab + r] * kr;
+      v -= L.G[r * ld + i] * L.G[r * ld + n];
+      w -= L.B[i * ldab + r] * L.Kb[r * ld + n];
     }
     L.s[i] = v;
     fo[i] = w;

@@ -17,18 +17,19 @@
 // and one Riccati step
 //   H  = R + diag(rd) + rho_c E'E + B~' S~ B~
 //   G~ = B~' S~ A~ + rho_c E' [D | 0] + [0 | r]
-//   K~ = H^{-1} G~
-//   S~ <- Q~ + rho_c [D|0]'[D|0] + A~' S~ A~ - G~' K~ ,
+//   W~ = Z G~,  K~ = Z' W~   (H = L L', Z = L^{-1}, H^{-1} = Z' Z)
+//   S~ <- Q~ + rho_c [D|0]'[D|0] + A~' S~ A~ - W~' W~ ,
 //   F~ = A~ - B~ K~ = [[A - B K, c - B k], [0, 1]]
 // is the affine step of k_lqr.h (S_t = Q + rho_c D'D + A'SA - G'K,
 // s_t = q + A'(S c + s) - G'k, F_t = A - BK, f_t = c - Bk) with the vectors
 // riding in column n.  The corner of S~ is reset to 0 every step.
 //
-// The m x m inverse is a Cholesky factorisation with the forward substitution
-// of L^{-1} in the same sweep over one tile (broadcasts of the pivot column by
-// DPP and of the pivot row by ds_bpermute), then H^{-1} = L^{-T} L^{-1} on the
-// matrix cores; the symmetric value tiles are mirrored through a 1 KB LDS
-// transpose.  The forward pass
+// The m x m factor is a Cholesky factorisation with the forward substitution
+// of Z = L^{-1} in the same sweep over one tile (broadcasts of the pivot column
+// by DPP and of the pivot row by ds_bpermute); H^{-1} is applied as Z' (Z G~)
+// on the matrix cores and never formed (in FP32 forming it first costs a
+// factor of about cond(H)^(1/2) in accuracy); the symmetric value tiles are
+// mirrored through a 1 KB LDS transpose.  The forward pass
 // (k_fwd_wave) runs on the matrix cores too, with x~ as column 0 of a
 // B-operand tile: every product's result tile is again in that layout, so
 // x~_{t+1} = A~ x~ + B~ du and du = -K~ x~ chain from registers and
@@ -107,7 +108,7 @@ __device__ __forceinline__ float row_bcast(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x150 + J, 0xf, 0xf, false));
 }
 
-// Step J of chol_inverse.  Lane (g, c) holds H[4 g + v][c] and Z[4 g + v][c]
+// Step J of chol_lower_inv.  Lane (g, c) holds H[4 g + v][c] and Z[4 g + v][c]
 // in register v.  The pivot a = H[J][J] comes from one lane (a scalar read)
 // and is floored at floor |H0[J][J]| (H0: the tile before the factorisation);
 // column J of L = H[:, J] / sqrt(a) below the diagonal comes from lane J of
@@ -147,23 +148,17 @@ __device__ __forceinline__ void chol_steps(f4& h, f4& z, const f4& h0, float flo
   }
 }
 
-// H <- H^{-1} for the symmetric tile H (m x m in the top-left corner, the
-// identity elsewhere): Cholesky H = L L' with floored pivots and Z = L^{-1}
-// in one sweep of 16 steps (straight-line code that the scheduler can place
-// among independent matrix products; past the m x m block every step is the
-// identity), then H^{-1} = Z' Z on the matrix cores.  Z' Z forms entries
-// (i, j) and (j, i) from the same products, so the inverse is symmetric and
-// serves as its own transpose (no mirror through LDS).
-__device__ __forceinline__ void chol_inverse(f4& h, float floor, int g, int c) {
+// Z = L^{-1} for the symmetric tile H (m x m in the top-left corner, the
+// identity elsewhere; H = L L' with floored pivots), in one sweep of 16 steps
+// (straight-line code that the scheduler can place among independent matrix
+// products; past the m x m block every step is the identity).  H is consumed.
+__device__ __forceinline__ f4 chol_lower_inv(f4& h, float floor, int g, int c) {
   const f4 h0 = h;
   f4 z;
 #pragma unroll
   for (int v = 0; v < 4; ++v) z[v] = (4 * g + v == c) ? 1.f : 0.f;
   chol_steps<0>(h, z, h0, floor, g, c);
-  f4 acc = zero4();
-#pragma unroll
-  for (int s = 0; s < 4; ++s) acc = mma(z[s], z[s], acc);
-  h = acc;
+  return z;
 }
 
 // Loop-invariant lane offsets (bytes) of one instance's shapes.
@@ -425,8 +420,9 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
 #pragma unroll
         for (int s = 0; s < 4; ++s) G[j] = mma(cur.E[s], cur.D[j][s], G[j]);
     }
-    // H <- H^{-1} (Cholesky; exactly symmetric)
-    chol_inverse(H, prm.hfloor, g, c);
+    // Z = L^{-1} (H = L L'); Z' in C layout is the A operand of Z G~
+    const f4 Z = chol_lower_inv(H, prm.hfloor, g, c);
+    const f4 Zt = tile_transpose(Z, tbuf, g, c);
     f4 Pm[NT][NT];
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
@@ -466,24 +462,28 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
       }
     }
 
-    // K~ = H^{-1} G~
+    // W~ = Z G~ (in G's registers), K~ = Z' W~
     f4 K[NT];
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
+      f4 w = zero4();
+#pragma unroll
+      for (int s = 0; s < 4; ++s) w = mma(Zt[s], G[j][s], w);
+      G[j] = w;
       f4 acc = zero4();
 #pragma unroll
-      for (int s = 0; s < 4; ++s) acc = mma(H[s], G[j][s], acc);
+      for (int s = 0; s < 4; ++s) acc = mma(Z[s], w[s], acc);
       K[j] = acc;
     }
 
-    // S~ <- Pm + rho_c [D|0]'[D|0] - G~' K~ (tiles i <= j, then mirrored)
+    // S~ <- Pm + rho_c [D|0]'[D|0] - W~' W~ (tiles i <= j, then mirrored)
 #pragma unroll
     for (int i = 0; i < NT; ++i)
 #pragma unroll
       for (int j = i; j < NT; ++j) {
         f4 acc = Pm[i][j];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) acc = mma(-G[i][s], K[j][s], acc);
+        for (int s = 0; s < 4; ++s) acc = mma(-G[i][s], G[j][s], acc);
         if (p > 0) {
 #pragma unroll
           for (int s = 0; s < 4; ++s) acc = mma(cur.D[i][s], cur.D[j][s], acc);
