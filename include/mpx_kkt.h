@@ -160,6 +160,22 @@ mpxkkt_status mpxkkt_sqp_iterate(void* stream, const mpxkkt_problem* prob,
                                  const mpxkkt_params* prm, mpxkkt_iterate* it,
                                  mpxkkt_stats* stats, void* workspace, size_t workspace_bytes);
 
+/* mpxkkt_sqp_iterate without a workspace argument and without any allocation:
+ * the workspace is a static device array of the library (one copy of
+ * MPXKKT_NOALLOC_WORKSPACE_BYTES per device, held until the process ends).
+ * No counterpart in the reference: it is for callers that cannot allocate --
+ * code under HIP graph capture, serving loops and harnesses that forbid
+ * hipMalloc.  Problems whose mpxkkt_workspace_size exceeds the array are
+ * refused (MPXKKT_ERR_WORKSPACE).  Every call on one device shares that
+ * device's array and no kernel reads workspace it has not written in the same
+ * call, so nothing needs clearing between calls; calls that overlap on one
+ * device (other streams, other threads) race and must be serialised by the
+ * caller.  No phase events are recorded. */
+#define MPXKKT_NOALLOC_WORKSPACE_BYTES (64u << 20)
+mpxkkt_status mpxkkt_sqp_iterate_noalloc(void* stream, const mpxkkt_problem* prob,
+                                         const mpxkkt_params* prm, mpxkkt_iterate* it,
+                                         mpxkkt_stats* stats);
+
 /* The caller-dynamics SQP iteration (MPXKKT_MODEL_CALLER) in two calls, the
  * merit line search split where the reference solver evaluates the caller's
  * functions at the trial points:

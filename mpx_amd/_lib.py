@@ -15,11 +15,13 @@ OK, ERR_ARG, ERR_UNSUPPORTED, ERR_WORKSPACE, ERR_DEVICE = 0, 1, 2, 3, 4
 F32, F64 = 0, 1
 MODEL_LTV, MODEL_TANH, MODEL_CALLER = 0, 1, 2
 PHASES = 4          # MPXKKT_PHASES
+NOALLOC_WORKSPACE_BYTES = 64 << 20   # MPXKKT_NOALLOC_WORKSPACE_BYTES
 
 EXPORTED = (
     "mpxkkt_default_params",
     "mpxkkt_workspace_size",
     "mpxkkt_sqp_iterate",
+    "mpxkkt_sqp_iterate_noalloc",
     "mpxkkt_sqp_direction",
     "mpxkkt_sqp_accept",
     "mpxkkt_srbd_iterate",
@@ -114,6 +116,7 @@ def load():
     lib.mpxkkt_workspace_size.argtypes = [P(Problem), P(Params), P(C.c_size_t)]
     lib.mpxkkt_sqp_iterate.argtypes = [C.c_void_p, P(Problem), P(Params), P(Iterate), P(Stats),
                                        C.c_void_p, C.c_size_t]
+    lib.mpxkkt_sqp_iterate_noalloc.argtypes = [C.c_void_p, P(Problem), P(Params), P(Iterate), P(Stats)]
     lib.mpxkkt_sqp_direction.argtypes = [C.c_void_p, P(Problem), P(Params), P(Iterate), P(Direction),
                                          C.c_void_p, C.c_size_t]
     lib.mpxkkt_sqp_accept.argtypes = [C.c_void_p, P(Problem), P(Params), P(Iterate), C.c_int32, C.c_void_p,

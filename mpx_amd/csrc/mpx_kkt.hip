@@ -222,6 +222,30 @@ mpxkkt_status lds_limit(int* out) {
   return MPXKKT_OK;
 }
 
+// Workspace of mpxkkt_sqp_iterate_noalloc: a static device array, one copy per
+// device, held while the process runs.  On the host the array's name is only a
+// stand-in for its device copy: host code uses the address hipGetSymbolAddress
+// returns, taken once per device with that device current.
+static __device__ __align__(256) unsigned char scratch[MPXKKT_NOALLOC_WORKSPACE_BYTES];
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {nullptr};
+
+mpxkkt_status scratch_address(void** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return fail(MPXKKT_ERR_DEVICE, "hipGetDevice failed");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      return fail(MPXKKT_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[dev] = p;
+  }
+  *out = g_scratch[dev];
+  return MPXKKT_OK;
+}
+
 template <typename K>
 mpxkkt_status prep_lds(K kernel, size_t bytes) {
   int lim = 0;
@@ -844,6 +868,30 @@ mpxkkt_status mpxkkt_sqp_iterate(void* stream, const mpxkkt_problem* prob, const
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (prob->dtype == MPXKKT_F64) return sqp_iterate_t<double>(st, prob, prm, it, stats, workspace);
   return sqp_iterate_t<float>(st, prob, prm, it, stats, workspace);
+}
+
+mpxkkt_status mpxkkt_sqp_iterate_noalloc(void* stream, const mpxkkt_problem* prob, const mpxkkt_params* prm,
+                                         mpxkkt_iterate* it, mpxkkt_stats* stats) {
+  mpxkkt_status s = check_problem(prob);
+  if (s != MPXKKT_OK) return s;
+  if (!prm) return fail(MPXKKT_ERR_ARG, "null params");
+  if (prob->model == MPXKKT_MODEL_CALLER)
+    return fail(MPXKKT_ERR_ARG, "caller dynamics: use mpxkkt_sqp_direction and mpxkkt_sqp_accept");
+  if ((s = check_iterate(prob, it)) != MPXKKT_OK) return s;
+  if ((s = check_params(prm)) != MPXKKT_OK) return s;
+  const size_t need = workspace_bytes(prob, prm);
+  if (need > (size_t)MPXKKT_NOALLOC_WORKSPACE_BYTES)
+    return fail(MPXKKT_ERR_WORKSPACE, "this problem needs " + std::to_string(need) +
+                                          " bytes of workspace; the static workspace of "
+                                          "mpxkkt_sqp_iterate_noalloc holds " +
+                                          std::to_string((size_t)MPXKKT_NOALLOC_WORKSPACE_BYTES) +
+                                          " (use mpxkkt_sqp_iterate with a caller workspace)");
+  void* ws = nullptr;
+  if ((s = scratch_address(&ws)) != MPXKKT_OK) return s;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  // no phase events here: creating them is a runtime allocation
+  if (prob->dtype == MPXKKT_F64) return sqp_iterate_t<double>(st, prob, prm, it, stats, ws, false);
+  return sqp_iterate_t<float>(st, prob, prm, it, stats, ws, false);
 }
 
 mpxkkt_status mpxkkt_sqp_direction(void* stream, const mpxkkt_problem* prob, const mpxkkt_params* prm,
