@@ -219,12 +219,15 @@ class MPCServer:
 
     def __init__(self, hbm_budget_bytes, max_group=256, slots_per_bucket=64, n_streams=1,
                  dtype=torch.float32, device="cuda", shift=1, tune=None, n_streams_slow=2, slow_ms=2.0,
-                 max_slots_per_bucket=1 << 16):
+                 max_slots_per_bucket=1 << 16, max_pending=2048):
         self.budget = int(hbm_budget_bytes)
+        # ticks queued and not yet launched: beyond this a request is refused
+        # (an overloaded server sheds load instead of letting every latency grow)
+        self.max_pending = int(max_pending)
         self.max_group = max_group
         self.slots_per_bucket = slots_per_bucket
         self.max_slots = max_slots_per_bucket
-        self.turned_away = dict(budget=0, slots=0)
+        self.turned_away = dict(budget=0, slots=0, queue=0)
         # staging areas (each with its stream) per bucket: n_streams, or
         # n_streams_slow for a bucket whose SQP iteration takes slow_ms or more
         # (known with tune='auto'): a fast bucket serves a launch in well under
@@ -336,9 +339,15 @@ class MPCServer:
         c.bucket.free.append(c.slot)
 
     # --- requests ----------------------------------------------------------
+    @property
+    def accepting(self):
+        """Whether a request would be queued now (the pending queue has room)."""
+        return len(self.pending) < self.max_pending
+
     def request(self, cid, x0, iters, t_arrive=None):
         """Queue one tick of controller ``cid`` (open) with measured state x0 and
-        ``iters`` >= 1 SQP iterations."""
+        ``iters`` >= 1 SQP iterations; returns the request, or None when the
+        pending queue is full (the tick is refused: counted in turned_away)."""
         if cid not in self.controllers:
             raise KeyError(f"controller {cid} is not open")
         iters = int(iters)
@@ -348,6 +357,9 @@ class MPCServer:
         if x0.shape != (self.controllers[cid].bucket.slots.n,):
             raise ValueError(f"x0 has shape {x0.shape}, controller {cid} expects "
                              f"({self.controllers[cid].bucket.slots.n},)")
+        if not self.accepting:
+            self.turned_away["queue"] += 1
+            return None
         self._rid += 1
         r = Request(self._rid, cid, iters, time.perf_counter() if t_arrive is None else t_arrive, x0)
         self.pending.append(r)

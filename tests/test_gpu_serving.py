@@ -103,6 +103,22 @@ def test_request_validation():
     assert not srv.busy
 
 
+def test_pending_queue_bound():
+    """An overloaded server sheds load: beyond max_pending queued ticks a
+    request is refused (None, counted in turned_away) and the queued ones are
+    still served."""
+    d = _insts()[0]
+    srv = _server(max_pending=2)
+    cids = [srv.open(dict(d, x0=_x(d, k))) for k in range(3)]
+    rs = [srv.request(c, d["x0"], 1) for c in cids]
+    assert rs[0] is not None and rs[1] is not None and rs[2] is None
+    assert srv.turned_away["queue"] == 1 and not srv.accepting
+    done = srv.run_round()
+    assert sorted(r.rid for r in done) == sorted(r.rid for r in rs[:2])
+    assert srv.accepting and srv.request(cids[2], d["x0"], 1) is not None
+    srv.run_round()
+
+
 def _oracle_tick(inst, state, x0, iters, shift=1):
     """One served tick on the oracle: iters SQP iterations from the
     controller's warm start with the measured state x0, first control, shift."""
