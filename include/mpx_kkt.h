@@ -234,6 +234,30 @@ mpxkkt_status mpxkkt_srbd_iterate(void* stream, const mpxkkt_problem* prob, cons
                                   const mpxkkt_params* prm, mpxkkt_iterate* it, mpxkkt_stats* stats,
                                   void* workspace, size_t workspace_bytes);
 
+/* Gait of the SRBD controller's reference generator (mpx/config/config_srbd.py). */
+typedef struct {
+  double dt, mass;           /* MPC step (s) and body mass (kg)                      */
+  double duty_factor, step_freq, step_height, clearance_speed;
+  double foot0[12];          /* nominal foot positions in the base frame             */
+  double advance_dt;         /* > 0: first advance t_timer by timer_run over this
+                                step (1 / mpc_frequency), as the controller does
+                                each tick (mpx/utils/mpc_wrapper_srbd.py:106)         */
+} mpxkkt_srbd_gait;
+
+/* The SRBD controller's gait timer and reference generator on the device,
+ * reference_generator_srbd without the terrain estimator
+ * (mpx/utils/mpc_utils.py:7-14, :146-268): per instance, from the gait
+ * phases t_timer [4] (in/out), the state x [13], the feet [12], the command
+ * input [7] = [v, omega, base height], the measured contacts [4] and the
+ * liftoff positions [12] (in/out), the reference [N+1][17] and parameter
+ * [N+1][16] mpxkkt_srbd_iterate reads, and the feet's reference velocities
+ * foot_ref_dot [N+1][12] (may be NULL).  All arrays dense, instance-major,
+ * of `dtype`. */
+mpxkkt_status mpxkkt_srbd_reference(void* stream, int32_t dtype, int32_t batch, int32_t horizon,
+                                    const mpxkkt_srbd_gait* gait, void* t_timer, const void* x,
+                                    const void* foot, const void* input, const void* contact,
+                                    void* liftoff, void* reference, void* parameter, void* foot_ref_dot);
+
 /* Options of mpxkkt_solve: the iteration protocol of runOffline
  * (mpx/utils/mpc_wrapper.py:425-467). */
 typedef struct {

@@ -25,6 +25,7 @@ EXPORTED = (
     "mpxkkt_sqp_direction",
     "mpxkkt_sqp_accept",
     "mpxkkt_srbd_iterate",
+    "mpxkkt_srbd_reference",
     "mpxkkt_default_solve_opts",
     "mpxkkt_solve",
     "mpxkkt_kkt_residual",
@@ -90,6 +91,12 @@ class Srbd(C.Structure):
                 ("inertia_inv", C.c_double * 9), ("W", View), ("reference", View), ("parameter", View)]
 
 
+class SrbdGait(C.Structure):
+    _fields_ = [("dt", C.c_double), ("mass", C.c_double), ("duty_factor", C.c_double),
+                ("step_freq", C.c_double), ("step_height", C.c_double), ("clearance_speed", C.c_double),
+                ("foot0", C.c_double * 12), ("advance_dt", C.c_double)]
+
+
 class SolveOpts(C.Structure):
     _fields_ = [("max_iter", C.c_int32), ("check_every", C.c_int32),
                 ("dcost_tol", C.c_double), ("cc_tol", C.c_double)]
@@ -123,6 +130,8 @@ def load():
                                       C.c_void_p, P(Stats), C.c_void_p, C.c_size_t]
     lib.mpxkkt_srbd_iterate.argtypes = [C.c_void_p, P(Problem), P(Srbd), P(Params), P(Iterate), P(Stats),
                                         C.c_void_p, C.c_size_t]
+    lib.mpxkkt_srbd_reference.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, P(SrbdGait)] + \
+        [C.c_void_p] * 9
     lib.mpxkkt_default_solve_opts.argtypes = [P(SolveOpts)]
     lib.mpxkkt_solve.argtypes = [C.c_void_p, P(Problem), P(Params), P(SolveOpts), P(Iterate), P(Stats),
                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,

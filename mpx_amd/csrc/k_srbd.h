@@ -526,3 +526,126 @@ __global__ void __launch_bounds__(64) k_srbd_trials(Prob<T> P, Srbd<T> S, Iter<T
 }
 
 }  // namespace mpxk
+
+namespace mpxk {
+
+// Gait timer and reference generator of the SRBD controller
+// (mpx/utils/mpc_utils.py:7-14 timer_run, :146-268 reference_generator_srbd,
+// without the terrain estimator as in mpx/config/config_srbd.py), one thread
+// per instance.  Per instance in: t_timer (4), x (13), foot (12), input (7:
+// v, omega, base height), contact (4), liftoff (12, updated in place); out:
+// reference [N+1][17], parameter [N+1][16], foot_ref_dot [N+1][12] (may be
+// null).  With advance_dt > 0 the instance's t_timer is first advanced by
+// timer_run(duty, step_freq, t_timer, advance_dt) in place (the controller's
+// per-tick timer, mpx/utils/mpc_wrapper_srbd.py:106).
+template <typename T>
+struct SrbdGait {
+  T dt, mass, duty, step_freq, step_height, clearance, advance_dt;
+  T foot0[12];
+  int N, B;
+  T *t_timer, *liftoff;
+  const T *x, *foot, *input, *contact;
+  T *ref, *par, *foot_dot;
+};
+
+template <typename T>
+__device__ __forceinline__ void timer_step(T duty, T freq, T dt, T* tm, T* c) {
+  for (int i = 0; i < 4; ++i) {
+    T v = tm[i] + dt * freq;
+    if (v > T(1)) v -= T(1);
+    tm[i] = v;
+    c[i] = v < duty ? T(1) : T(0);
+  }
+}
+
+template <typename T>
+__global__ void k_srbd_reference(SrbdGait<T> G) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= G.B) return;
+  const int N = G.N, N1 = N + 1;
+  T tm[4], lo[12], foot[12], cin[4], in[7];
+  for (int i = 0; i < 4; ++i) { tm[i] = G.t_timer[b * 4 + i]; cin[i] = G.contact[b * 4 + i]; }
+  for (int i = 0; i < 12; ++i) { lo[i] = G.liftoff[b * 12 + i]; foot[i] = G.foot[b * 12 + i]; }
+  for (int i = 0; i < 7; ++i) in[i] = G.input[b * 7 + i];
+  if (G.advance_dt > T(0)) {
+    T c[4];
+    timer_step(G.duty, G.step_freq, G.advance_dt, tm, c);
+    for (int i = 0; i < 4; ++i) G.t_timer[b * 4 + i] = tm[i];
+  }
+  const T* x = G.x + (size_t)b * 13;
+  const T q0 = x[3], q1 = x[4], q2 = x[5], q3 = x[6];
+  const T yaw = dev_atan2(T(2) * (q0 * q3 + q1 * q2), T(1) - T(2) * (q2 * q2 + q3 * q3));
+  const T cy = dev_cos(yaw), sy = dev_sin(yaw);
+  const T height = in[6] + (lo[2] + lo[5] + lo[8] + lo[11]) / T(4);
+  const T p[3] = {x[0], x[1], height};
+  const T v[3] = {cy * in[0] - sy * in[1], sy * in[0] + cy * in[1], in[2]};
+  const T vn = dev_sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  const T v0[2] = {-v[0] / (vn + T(1e-6)) * G.clearance, -v[1] / (vn + T(1e-6)) * G.clearance};
+  // footholds of the swing legs: raibert-style offset from the projected nominal feet
+  const T gain = dev_sqrt(in[6] / T(9.81));
+  T fh[2][4];
+  for (int i = 0; i < 4; ++i) {
+    const T f0x = G.foot0[3 * i], f0y = G.foot0[3 * i + 1];
+    const T proj[2] = {p[0] + cy * f0x - sy * f0y, p[1] + sy * f0x + cy * f0y};
+    for (int d = 0; d < 2; ++d)
+      fh[d][i] = T(0.5) * v[d] * G.duty / G.step_freq + gain * (x[7 + d] - v[d]) + proj[d];
+  }
+  // early contact: a leg the timer lifts that still touches down late in its swing
+  T des[4], ct[4] = {tm[0], tm[1], tm[2], tm[3]}, early[4];
+  timer_step(G.duty, G.step_freq, G.dt, ct, des);
+  for (int i = 0; i < 4; ++i)
+    early[i] = (des[i] == T(0) && cin[i] == T(1) && ct[i] > T(0.5) + T(0.5) * G.duty) ? T(1) : T(0);
+  T* ref = G.ref + (size_t)b * N1 * 17;
+  T* par = G.par + (size_t)b * N1 * 16;
+  T* fd = G.foot_dot ? G.foot_dot + (size_t)b * N1 * 12 : nullptr;
+  // the recursion's previous foot row starts as the current feet; the
+  // previous contact of stage 0 is 0 (the reference reads the still-zero last
+  // row there, JAX indexing -1)
+  T prev[12], cprev[4] = {0, 0, 0, 0};
+  for (int i = 0; i < 12; ++i) prev[i] = foot[i];
+  T timer[4] = {tm[0], tm[1], tm[2], tm[3]};
+  for (int t = 0; t < N1; ++t) {
+    T c[4];
+    timer_step(G.duty, G.step_freq, G.dt, timer, c);
+    const T ts = T(t) * G.dt;
+    T* r = ref + (size_t)t * 17;
+    r[0] = ts * v[0] + p[0];
+    r[1] = ts * v[1] + p[1];
+    r[2] = height;
+    r[3] = T(1); r[4] = T(0); r[5] = T(0); r[6] = T(0);
+    for (int k = 0; k < 3; ++k) { r[7 + k] = v[k]; r[10 + k] = in[3 + k]; }
+    T row[12], drow[12];
+    for (int i = 0; i < 4; ++i) {
+      if (c[i] == T(0) && cprev[i] != T(0)) {
+        lo[3 * i] = prev[3 * i]; lo[3 * i + 1] = prev[3 * i + 1]; lo[3 * i + 2] = prev[3 * i + 2];
+      }
+      const T s = (timer[i] - G.duty) / (T(1) - G.duty);
+      const bool stance = c[i] > T(0) || early[i] == T(1);
+      for (int d = 0; d < 2; ++d) {
+        const T cur = lo[3 * i + d], h = fh[d][i], w = v0[d];
+        const T a2 = T(3) * (h - cur) - T(2) * w, a3 = w - T(2) * (h - cur);
+        row[3 * i + d] = stance ? prev[3 * i + d] : cur + w * s + a2 * s * s + a3 * s * s * s;
+        drow[3 * i + d] = stance ? T(0) : T(2) * a2 * s + T(3) * a3 * s * s + w;
+      }
+      {
+        const T cur = lo[3 * i + 2], h = cur, top = cur + G.step_height, w = T(0.7);
+        const T a = T(16) * top - T(8) * h - T(8) * cur - T(2) * w;
+        const T bb = T(5) * w + T(14) * h + T(18) * cur - T(32) * top;
+        const T cc = T(16) * top - T(5) * h - T(11) * cur - T(4) * w;
+        row[3 * i + 2] = stance ? prev[3 * i + 2] : (((a * s + bb) * s + cc) * s + w) * s + cur;
+        drow[3 * i + 2] = stance ? T(0) : ((T(4) * a * s + T(3) * bb) * s + T(2) * cc) * s + w;
+      }
+      r[13 + i] = c[i];
+      par[(size_t)t * 16 + i] = c[i];
+      cprev[i] = c[i];
+    }
+    for (int k = 0; k < 12; ++k) {
+      par[(size_t)t * 16 + 4 + k] = row[k];
+      if (fd) fd[(size_t)t * 12 + k] = drow[k];
+      prev[k] = row[k];
+    }
+  }
+  for (int i = 0; i < 12; ++i) G.liftoff[b * 12 + i] = lo[i];
+}
+
+}  // namespace mpxk

@@ -957,6 +957,39 @@ mpxkkt_status mpxkkt_srbd_iterate(void* stream, const mpxkkt_problem* prob, cons
   return srbd_iterate_t<float>(st, prob, model, prm, it, stats, workspace);
 }
 
+mpxkkt_status mpxkkt_srbd_reference(void* stream, int32_t dtype, int32_t batch, int32_t horizon,
+                                    const mpxkkt_srbd_gait* gait, void* t_timer, const void* x, const void* foot,
+                                    const void* input, const void* contact, void* liftoff, void* reference,
+                                    void* parameter, void* foot_ref_dot) {
+  if (dtype != MPXKKT_F32 && dtype != MPXKKT_F64) return fail(MPXKKT_ERR_ARG, "bad dtype");
+  if (batch < 1 || batch > (1 << 24) || horizon < 1 || horizon > 100000)
+    return fail(MPXKKT_ERR_ARG, "batch and horizon must be positive");
+  if (!gait || !t_timer || !x || !foot || !input || !contact || !liftoff || !reference || !parameter)
+    return fail(MPXKKT_ERR_ARG, "null gait or array");
+  if (!(gait->dt > 0.0) || !(gait->mass > 0.0) || !(gait->duty_factor > 0.0 && gait->duty_factor < 1.0) ||
+      !(gait->step_freq > 0.0) || gait->advance_dt < 0.0)
+    return fail(MPXKKT_ERR_ARG, "dt, mass, step_freq must be positive and duty_factor in (0, 1)");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  auto launch = [&](auto zero) -> mpxkkt_status {
+    using T = decltype(zero);
+    SrbdGait<T> G;
+    G.dt = (T)gait->dt; G.mass = (T)gait->mass; G.duty = (T)gait->duty_factor;
+    G.step_freq = (T)gait->step_freq; G.step_height = (T)gait->step_height;
+    G.clearance = (T)gait->clearance_speed; G.advance_dt = (T)gait->advance_dt;
+    for (int k = 0; k < 12; ++k) G.foot0[k] = (T)gait->foot0[k];
+    G.N = horizon; G.B = batch;
+    G.t_timer = static_cast<T*>(t_timer); G.liftoff = static_cast<T*>(liftoff);
+    G.x = static_cast<const T*>(x); G.foot = static_cast<const T*>(foot);
+    G.input = static_cast<const T*>(input); G.contact = static_cast<const T*>(contact);
+    G.ref = static_cast<T*>(reference); G.par = static_cast<T*>(parameter);
+    G.foot_dot = static_cast<T*>(foot_ref_dot);
+    hipLaunchKernelGGL(k_srbd_reference<T>, dim3((batch + 63) / 64), dim3(64), 0, st, G);
+    return launch_check("k_srbd_reference");
+  };
+  if (dtype == MPXKKT_F64) return launch(0.0);
+  return launch(0.0f);
+}
+
 mpxkkt_status mpxkkt_default_solve_opts(mpxkkt_solve_opts* out) {
   if (!out) return fail(MPXKKT_ERR_ARG, "null options");
   out->max_iter = 100;

@@ -12,6 +12,12 @@ line-search trial points and the acceptance all happen on the device.
 Per instance: ``W`` (24, 24) residual weight, ``reference`` (N+1, 17) =
 [p_ref, quat_ref, dp_ref, omega_ref, contact], ``parameter`` (N+1, 16) =
 [contact, p_legs]; the model constants mass, inertia and dt.
+
+:class:`SRBDGait` runs the controller's gait timer and reference generator
+(``timer_run`` and ``reference_generator_srbd``, mpx/utils/mpc_utils.py:7-14,
+:146-268) on the device, writing reference and parameter in that layout, so a
+whole controller tick -- timer, reference, SQP iterations, warm-start shift --
+needs no host step.
 """
 from __future__ import annotations
 
@@ -72,3 +78,45 @@ class SRBDSolver:
                                           C.byref(s.params), C.byref(ci), C.byref(s._stats),
                                           C.c_void_p(s.ws.data_ptr()), s.ws_bytes))
         return s.stats_t
+
+
+class SRBDGait:
+    """Device gait state of B controllers and the reference generator.
+
+    ``t_timer`` (B, 4) gait phases and ``liftoff`` (B, 12) are kept on the
+    device between ticks; :meth:`generate` writes ``reference`` (B, N+1, 17),
+    ``parameter`` (B, N+1, 16) and ``foot_ref_dot`` (B, N+1, 12).
+    """
+
+    def __init__(self, B, N, dt, mass, duty_factor, step_freq, step_height, clearance_speed, foot0, t_timer,
+                 liftoff, dtype=torch.float32, device="cuda"):
+        self.B, self.N, self.dtype = B, N, dtype
+        self.device = torch.device(device)
+        dev = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float64), dtype=dtype,
+                                        device=self.device).clone()
+        self.t_timer = dev(np.broadcast_to(np.asarray(t_timer, np.float64), (B, 4)))
+        self.liftoff = dev(np.broadcast_to(np.asarray(liftoff, np.float64), (B, 12)))
+        z = lambda *shape: torch.zeros(shape, dtype=dtype, device=self.device)
+        self.reference, self.parameter, self.foot_ref_dot = z(B, N + 1, N_REF), z(B, N + 1, N_PAR), z(B, N + 1, 12)
+        g = L.SrbdGait()
+        g.dt, g.mass, g.duty_factor, g.step_freq = float(dt), float(mass), float(duty_factor), float(step_freq)
+        g.step_height, g.clearance_speed, g.advance_dt = float(step_height), float(clearance_speed), 0.0
+        for k, v in enumerate(np.asarray(foot0, np.float64).ravel()):
+            g.foot0[k] = float(v)
+        self.gait = g
+
+    def generate(self, x, foot, command, contact, advance_dt=0.0):
+        """Reference and parameter for states x (B, 13), feet (B, 12), commands
+        (B, 7) = [v, omega, base height] and measured contacts (B, 4); with
+        advance_dt > 0 the gait phases first advance by that step."""
+        dev = lambda a: (a if isinstance(a, torch.Tensor) else torch.as_tensor(np.asarray(a))).to(
+            device=self.device, dtype=self.dtype).contiguous()
+        x, foot, command, contact = dev(x), dev(foot), dev(command), dev(contact)
+        self.gait.advance_dt = float(advance_dt)
+        dt = L.F64 if self.dtype == torch.float64 else L.F32
+        p = lambda t: C.c_void_p(t.data_ptr())
+        L.check(L.load().mpxkkt_srbd_reference(_stream(), dt, self.B, self.N, C.byref(self.gait),
+                                               p(self.t_timer), p(x), p(foot), p(command), p(contact),
+                                               p(self.liftoff), p(self.reference), p(self.parameter),
+                                               p(self.foot_ref_dot)))
+        return self.reference, self.parameter

@@ -351,3 +351,109 @@ def caller_instance(consts, W, reference, parameter, x0, N, p=0, D=None, E=None,
     inst["E"] = np.zeros((N, p, N_CTRL)) if E is None else E
     inst["e"] = np.zeros((N, p)) if e is None else e
     return inst
+
+
+# --------------------------------------------------------------------------
+# gait timer and reference generator (mpx/utils/mpc_utils.py:7-14, :146-268)
+# --------------------------------------------------------------------------
+
+def timer_run(duty_factor, step_freq, leg_time, dt):
+    """(contact, leg_time) after advancing each leg's gait phase by dt step_freq,
+    wrapped back by 1 past 1; a leg is in stance while its phase < duty_factor
+    (mpx/utils/mpc_utils.py:7-14)."""
+    leg_time = leg_time + dt * step_freq
+    leg_time = np.where(leg_time > 1, leg_time - 1, leg_time)
+    return np.where(leg_time < duty_factor, 1.0, 0.0), leg_time
+
+
+def spline_xy(cur, fh, v0, s):
+    """Cubic from cur (slope v0) to fh over s in [0, 1] (mpc_utils.py:203-208)."""
+    a2 = 3 * (fh - cur) - 2 * v0
+    a3 = v0 - 2 * (fh - cur)
+    return cur + v0 * s + a2 * s ** 2 + a3 * s ** 3
+
+
+def spline_xy_dot(cur, fh, v0, s):
+    a2 = 3 * (fh - cur) - 2 * v0
+    a3 = v0 - 2 * (fh - cur)
+    return 2 * a2 * s + 3 * a3 * s ** 2 + v0
+
+
+def spline_z(cur, fh, top, s, v0=0.7):
+    """Quartic through cur (s = 0, slope v0), top (s = 1/2) and fh (s = 1)
+    (mpc_utils.py:210-219)."""
+    a = 16 * top - 8 * fh - 8 * cur - 2 * v0
+    b = 5 * v0 + 14 * fh + 18 * cur - 32 * top
+    c = 16 * top - 5 * fh - 11 * cur - 4 * v0
+    return a * s ** 4 + b * s ** 3 + c * s ** 2 + v0 * s + cur
+
+
+def spline_z_dot(cur, fh, top, s, v0=0.7):
+    a = 16 * top - 8 * fh - 8 * cur - 2 * v0
+    b = 5 * v0 + 14 * fh + 18 * cur - 32 * top
+    c = 16 * top - 5 * fh - 11 * cur - 4 * v0
+    return 4 * a * s ** 3 + 3 * b * s ** 2 + 2 * c * s + v0
+
+
+def reference_generator(N, dt, mass, foot0, t_timer, x, foot, inp, duty_factor, step_freq, step_height,
+                        liftoff, contact, clearance_speed):
+    """reference_generator_srbd with use_terrain_estimator = False
+    (mpx/utils/mpc_utils.py:146-268; the SRBD config's setting,
+    mpx/config/config_srbd.py): returns reference (N+1, 17) = [p_ref, quat_ref,
+    dp_ref, omega_ref, contact], parameter (N+1, 16) = [contact, foot_ref], the
+    updated liftoff (12) and foot_ref_dot (N+1, 12).  ``inp`` = [v (3),
+    omega (3), base height]."""
+    nc = NC
+    p, quat, dp = x[:3], x[3:7], x[7:10]
+    yaw = np.arctan2(2 * (quat[0] * quat[3] + quat[1] * quat[2]), 1 - 2 * (quat[2] ** 2 + quat[3] ** 2))
+    Ryaw = np.array([[np.cos(yaw), -np.sin(yaw), 0.0], [np.sin(yaw), np.cos(yaw), 0.0], [0.0, 0.0, 1.0]])
+    height = inp[6] + np.sum(liftoff[2::3]) / nc
+    p = np.array([p[0], p[1], height])
+    quat_ref = np.array([1.0, 0.0, 0.0, 0.0])
+    # pitch of quat_ref is 0 without the terrain estimator: R_pitch = I
+    v_ref = Ryaw @ inp[:3]
+    w_ref = inp[3:6]
+    ts = np.arange(N + 1) * dt
+    ref = np.zeros((N + 1, 17))
+    ref[:, 0] = ts * v_ref[0] + p[0]
+    ref[:, 1] = ts * v_ref[1] + p[1]
+    ref[:, 2] = height
+    ref[:, 3:7] = quat_ref
+    ref[:, 7:10] = v_ref
+    ref[:, 10:13] = w_ref
+    foot_ref = np.tile(foot, (N + 1, 1))
+    foot_dot = np.zeros((N + 1, 3 * nc))
+    foot0_proj = np.tile(p, nc) + np.concatenate([Ryaw @ foot0[3 * i:3 * i + 3] for i in range(nc)])
+    des, cur_timer = timer_run(duty_factor, step_freq, t_timer, dt)
+    early = ((des == 0) & (contact == 1) & (cur_timer > 0.5 + 0.5 * duty_factor)).astype(float)
+    v0 = -v_ref / (np.linalg.norm(v_ref) + 1e-6) * clearance_speed
+    fh = [0.5 * v_ref[d] * duty_factor / step_freq + np.sqrt(inp[6] / 9.81) * (dp[d] - v_ref[d])
+          + foot0_proj[d::3] for d in (0, 1)]
+    lx, ly, lz = liftoff[0::3].copy(), liftoff[1::3].copy(), liftoff[2::3].copy()
+    cs = np.zeros((N + 1, nc))
+    timer = np.array(t_timer, dtype=float)
+    for t in range(N + 1):
+        prev = foot_ref[t - 1]            # t = 0 reads row N (still the current feet), as JAX indexes -1
+        c_new, timer = timer_run(duty_factor, step_freq, timer, dt)
+        cs[t] = c_new
+        lifting = (cs[t] == 0) & (cs[t - 1] != 0)
+        lx = np.where(lifting, prev[0::3], lx)
+        ly = np.where(lifting, prev[1::3], ly)
+        lz = np.where(lifting, prev[2::3], lz)
+        s = (timer - duty_factor) / (1 - duty_factor)
+        stance = (c_new > 0) | (early == 1)
+        row = np.zeros(3 * nc)
+        row[0::3] = np.where(stance, prev[0::3], spline_xy(lx, fh[0], v0[0], s))
+        row[1::3] = np.where(stance, prev[1::3], spline_xy(ly, fh[1], v0[1], s))
+        row[2::3] = np.where(stance, prev[2::3], spline_z(lz, lz, lz + step_height, s))
+        foot_ref[t] = row
+        drow = np.zeros(3 * nc)
+        drow[0::3] = np.where(stance, 0.0, spline_xy_dot(lx, fh[0], v0[0], s))
+        drow[1::3] = np.where(stance, 0.0, spline_xy_dot(ly, fh[1], v0[1], s))
+        drow[2::3] = np.where(stance, 0.0, spline_z_dot(lz, lz, lz + step_height, s))
+        foot_dot[t] = drow
+    ref[:, 13:17] = cs
+    par = np.concatenate([cs, foot_ref], axis=1)
+    lo = np.zeros(3 * nc)
+    lo[0::3], lo[1::3], lo[2::3] = lx, ly, lz
+    return ref, par, lo, foot_dot

@@ -169,3 +169,63 @@ def test_zero_vector_conventions():
     v = np.array([2e-8, 0.0, 0.0])
     _, Jv = srbd.quat_integrate_jac(q, v, 0.02)
     np.testing.assert_allclose(Jv[1:], 0.01 * np.eye(3), atol=1e-12)
+
+
+# --------------------------------------------------------------------------
+# gait timer and reference generator
+# --------------------------------------------------------------------------
+
+def test_gait_splines_boundary_conditions():
+    """The swing splines start at the liftoff point with the stated slope, end
+    at the foothold, and the height spline passes its apex at mid swing."""
+    cur, fh, v0, top = 0.3, 0.55, -0.4, 0.065
+    assert abs(srbd.spline_xy(cur, fh, v0, 0.0) - cur) < 1e-15
+    assert abs(srbd.spline_xy(cur, fh, v0, 1.0) - fh) < 1e-14
+    assert abs(srbd.spline_xy_dot(cur, fh, v0, 0.0) - v0) < 1e-15
+    h = 1e-6
+    for s in (0.2, 0.7):
+        fd = (srbd.spline_xy(cur, fh, v0, s + h) - srbd.spline_xy(cur, fh, v0, s - h)) / (2 * h)
+        assert abs(fd - srbd.spline_xy_dot(cur, fh, v0, s)) < 1e-8
+        fd = (srbd.spline_z(0.0, 0.0, top, s + h) - srbd.spline_z(0.0, 0.0, top, s - h)) / (2 * h)
+        assert abs(fd - srbd.spline_z_dot(0.0, 0.0, top, s)) < 1e-8
+    assert abs(srbd.spline_z(0.02, 0.02, top, 0.0) - 0.02) < 1e-15
+    assert abs(srbd.spline_z(0.02, 0.02, top, 1.0) - 0.02) < 1e-14
+    assert abs(srbd.spline_z(0.02, 0.02, top, 0.5) - top) < 1e-14
+    assert abs(srbd.spline_z_dot(0.02, 0.02, top, 0.0) - 0.7) < 1e-15
+
+
+def test_timer_and_reference_generator_invariants():
+    """Timer: phases advance by dt f and wrap; stance while phase < duty.
+    Generator: p_ref moves at the yaw-rotated commanded velocity from the
+    measured position at the commanded height, contacts follow the timer
+    stage by stage, stance feet stay put, and the liftoff point is recorded
+    when a foot leaves the ground."""
+    c, t = srbd.timer_run(0.65, 1.35, np.array([0.5, 0.0, 0.99, 0.64]), 0.02)
+    np.testing.assert_allclose(t, [0.527, 0.027, 0.017, 0.667])
+    np.testing.assert_array_equal(c, [1, 1, 1, 0])
+    foot0 = np.array([0.27, 0.19, 0.0, 0.27, -0.19, 0.0, -0.21, 0.19, 0.0, -0.21, -0.19, 0.0])
+    yaw = 0.3
+    x = np.concatenate([[0.1, 0.2, 0.33], [np.cos(yaw / 2), 0, 0, np.sin(yaw / 2)], [0.3, 0.0, 0.0], np.zeros(3)])
+    foot = np.tile([0.1, 0.2, 0.0], 4) + foot0
+    inp = np.array([0.4, 0.1, 0.0, 0.0, 0.0, 0.2, 0.33])
+    N, dt = 30, 0.02
+    t0 = np.array([0.5, 0.0, 0.0, 0.5])
+    ref, par, lo, fd = srbd.reference_generator(N, dt, 24.6, foot0, t0, x, foot, inp, 0.65, 1.35, 0.065, foot,
+                                                np.ones(4), 0.4)
+    v = np.array([np.cos(yaw) * 0.4 - np.sin(yaw) * 0.1, np.sin(yaw) * 0.4 + np.cos(yaw) * 0.1])
+    np.testing.assert_allclose(ref[:, 0], 0.1 + np.arange(N + 1) * dt * v[0], atol=1e-12)
+    np.testing.assert_allclose(ref[:, 1], 0.2 + np.arange(N + 1) * dt * v[1], atol=1e-12)
+    np.testing.assert_allclose(ref[:, 2], 0.33, atol=1e-15)
+    timer = t0.copy()
+    for k in range(N + 1):
+        cs, timer = srbd.timer_run(0.65, 1.35, timer, dt)
+        np.testing.assert_array_equal(ref[k, 13:17], cs)
+        np.testing.assert_array_equal(par[k, 0:4], cs)
+    stance_all = np.all(par[:, 0:4] == 1, axis=0)
+    for i in np.where(stance_all)[0]:
+        np.testing.assert_allclose(par[:, 4 + 3 * i:7 + 3 * i], np.tile(foot[3 * i:3 * i + 3], (N + 1, 1)))
+    lifted = [i for i in range(4) if np.any(par[:, i] == 0)]
+    assert lifted
+    for i in lifted:
+        np.testing.assert_allclose(lo[3 * i:3 * i + 3], foot[3 * i:3 * i + 3], atol=1e-15)
+        assert par[:, 6 + 3 * i].max() > 0.03          # the swing foot lifts
