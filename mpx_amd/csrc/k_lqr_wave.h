@@ -108,56 +108,93 @@ __device__ __forceinline__ float row_bcast(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x150 + J, 0xf, 0xf, false));
 }
 
-// Step J of chol_lower_inv.  Lane (g, c) holds H[4 g + v][c] and Z[4 g + v][c]
-// in register v.  The pivot a = H[J][J] comes from one lane (a scalar read)
-// and is floored at floor |H0[J][J]| (H0: the tile before the factorisation);
-// column J of L = H[:, J] / sqrt(a) below the diagonal comes from lane J of
-// the lane's own row of 16 (a DPP broadcast) and L[c][J] from row J (lane
-// 16 (J / 4) + c; equal to H[c][J] by symmetry).  The trailing matrix takes
-// H[r][c] -= L[r][J] L[c][J] (r, c > J), column J becomes L's column J (zero
-// above the diagonal), and the forward substitution of L Z = I advances by
-// the same column: Z[J][:] /= L[J][J], Z[r][:] -= L[r][J] Z[J][:] (r > J).
+// x from lane 16 (J / 4) + c of the wave (row J of a C-layout tile, for every
+// row of 16 lanes); c4 = 4 c.
 template <int J>
-__device__ __forceinline__ void chol_step(f4& h, f4& z, const f4& h0, float floor, int g, int c) {
-  constexpr int jg = J >> 2, jv = J & 3;
-  const float a = floored_pivot(rlane(h[jv], jg * 16 + J), rlane(h0[jv], jg * 16 + J), floor);
-  const float d = sqrtf(a);
-  const float id = 1.f / d;
-  const float lc = __shfl(h[jv], jg * 16 + c) * id;              // L[c][J] (c > J)
-  const float zj = __shfl(z[jv], jg * 16 + c) * id;              // Z[J][c] after the step
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int r = 4 * g + v;
-    const float lr = row_bcast<J>(h[v]) * id;                     // L[r][J] (r > J)
-    float hv = h[v];
-    if (r > J && c > J) hv -= lr * lc;
-    if (c == J) hv = r > J ? lr : (r == J ? d : 0.f);
-    h[v] = hv;
-    float zv = z[v];
-    if (r == J) zv = zj;
-    else if (r > J) zv -= lr * zj;
-    z[v] = zv;
+__device__ __forceinline__ float tile_row(float x, int c4) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(c4 + 64 * (J >> 2), __float_as_int(x)));
+}
+
+// y + bcast_J(x) s on the rows of 16 lanes set in RM (lane J of each row
+// broadcast to the row); the other rows keep y.
+template <int J, int RM>
+__device__ __forceinline__ float fma_bcast(float x, float s, float y) {
+  if constexpr (RM == 0xf) {
+    return fmaf(row_bcast<J>(x), s, y);
+  } else {
+    const float xb = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x150 + J, RM, 0xf, false));
+    return fmaf(xb, s, y);
   }
 }
 
+// First row of 16 lanes g whose row 4 g + v of a tile lies below row J.
+__host__ __device__ constexpr int chol_gmin(int J, int v) { return J >= v ? (J - v) / 4 + 1 : 0; }
+
+// Z[r][:] += H[r][J] nzj for the rows r = 4 g + v > J of register v.
+template <int J, int V>
+__device__ __forceinline__ float chol_zrow(float h, float z, float nzj) {
+  constexpr int gm = chol_gmin(J, V);
+  if constexpr (gm < 4) return fma_bcast<J, (0xf << gm) & 0xf>(h, nzj, z);
+  return z;
+}
+
+// Step J of chol_lower_inv.  Lane (g, c) holds H[4 g + v][c] and Z[4 g + v][c]
+// in register v.  The pivot a = |H[J][J]| comes from one lane (a scalar read)
+// and is raised to fl = floor |H0[J][J]| (H0: the tile before the
+// factorisation; a NaN pivot stays NaN); id = 1 / L[J][J] = a^{-1/2}.
+// H[r][J] reaches the lanes of row r by a DPP broadcast from lane J of the
+// lane's row of 16, H[J][c] and Z[J][c] from row J (tile_row).  The trailing
+// matrix takes H[r][c] -= H[r][J] H[J][c] / a, i.e. H - L[:, J] L[:, J]'; this
+// update runs over every entry: the entries in a row or a column <= J are
+// never read again.  The forward substitution of L Z = I advances by the same
+// column: Z[J][:] *= id, then Z[r][:] -= L[r][J] Z[J][:] for r > J only (DPP
+// row masks select the rows of 16 lanes whose row 4 g + v lies below J).
 template <int J>
-__device__ __forceinline__ void chol_steps(f4& h, f4& z, const f4& h0, float floor, int g, int c) {
-  if constexpr (J < 16) {
-    chol_step<J>(h, z, h0, floor, g, c);
-    chol_steps<J + 1>(h, z, h0, floor, g, c);
+__device__ __forceinline__ void chol_step(f4& h, f4& z, const float (&fl)[16], int c4) {
+  constexpr int jg = J >> 2, jv = J & 3;
+  const float a = fabsf(rlane(h[jv], jg * 16 + J));
+  const float piv = fl[J] > a ? fl[J] : a;
+  const float id = __builtin_amdgcn_rsqf(piv);
+  const float nid2 = -(id * id);
+  const float hJ = tile_row<J>(h[jv], c4);   // H[J][c]
+  const float zJ = tile_row<J>(z[jv], c4);   // Z[J][c] before the step
+  const float nlc = hJ * nid2;               // -H[J][c] / a
+  const float nzj = zJ * nid2;               // -L[J][J]^{-1} Z[J][c] (after the step)
+  // (Z first: it reads column J of H before the trailing update)
+  z[0] = chol_zrow<J, 0>(h[0], z[0], nzj);
+  z[1] = chol_zrow<J, 1>(h[1], z[1], nzj);
+  z[2] = chol_zrow<J, 2>(h[2], z[2], nzj);
+  z[3] = chol_zrow<J, 3>(h[3], z[3], nzj);
+  // row J of Z: Z[J][c] id into the row of 16 lanes jg of register jv
+  z[jv] = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(z[jv]), __float_as_int(zJ * id), 0xE4,
+                                                     1 << jg, 0xf, false));
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+    if (chol_gmin(J, v) < 4) h[v] = fma_bcast<J, 0xf>(h[v], nlc, h[v]);
+}
+
+template <int J, int NS>
+__device__ __forceinline__ void chol_steps(f4& h, f4& z, const float (&fl)[16], int c4) {
+  if constexpr (J < NS) {
+    chol_step<J>(h, z, fl, c4);
+    chol_steps<J + 1, NS>(h, z, fl, c4);
   }
 }
 
 // Z = L^{-1} for the symmetric tile H (m x m in the top-left corner, the
-// identity elsewhere; H = L L' with floored pivots), in one sweep of 16 steps
-// (straight-line code that the scheduler can place among independent matrix
-// products; past the m x m block every step is the identity).  H is consumed.
+// identity elsewhere; H = L L' with floored pivots), in one sweep of NS >= m
+// steps (straight-line code that the scheduler can place among independent
+// matrix products; past the m x m block every step is the identity, so the
+// sweep may stop at any NS >= m).  H is consumed.
+template <int NS>
 __device__ __forceinline__ f4 chol_lower_inv(f4& h, float floor, int g, int c) {
-  const f4 h0 = h;
+  float fl[16];
+#pragma unroll
+  for (int j = 0; j < NS; ++j) fl[j] = floor * fabsf(rlane(h[j & 3], (j >> 2) * 16 + j));
   f4 z;
 #pragma unroll
   for (int v = 0; v < 4; ++v) z[v] = (4 * g + v == c) ? 1.f : 0.f;
-  chol_steps<0>(h, z, h0, floor, g, c);
+  chol_steps<0, NS>(h, z, fl, 4 * c);
   return z;
 }
 
@@ -379,7 +416,11 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
   WaveStage<NT> cur;
   load_wave_stage<NT, CQ>(cur, P, ws, in, b, N - 1, sr, g, c, L);
   for (int t = N - 1; t >= 0; --t) {
-    const LaneOff<NT> Ls = lane_offsets<NT>(n, m, fresh(g), fresh(c));
+    // lane conditions and offsets from opaque copies of the lane coordinates:
+    // recomputed every stage instead of held (as lane masks in scalar
+    // registers) across the loop
+    const int gf = fresh(g), cf = fresh(c);
+    const LaneOff<NT> Ls = lane_offsets<NT>(n, m, gf, cf);
     // SB = S~ B~ ; H = R + diag(rd) + rho_c E'E + B~' SB
     f4 SB[NT];
 #pragma unroll
@@ -395,8 +436,8 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
     if constexpr (CQ) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int row = 4 * g + v;
-        H[v] = Rc[v] + (row == c ? (row < m ? cur.rdv[v] : 1.f) : 0.f);
+        const int row = 4 * gf + v;
+        H[v] = Rc[v] + (row == cf ? (row < m ? cur.rdv[v] : 1.f) : 0.f);
       }
     } else {
       H = cur.H;
@@ -411,7 +452,7 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
 #pragma unroll
     for (int j = 0; j < NT; ++j)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) G[j][v] = (16 * j + c == n) ? cur.r[v] : 0.f;
+      for (int v = 0; v < 4; ++v) G[j][v] = (16 * j + cf == n) ? cur.r[v] : 0.f;
     if (p > 0) {
 #pragma unroll
       for (int s = 0; s < 4; ++s) H = mma(cur.E[s], cur.E[s], H);
@@ -421,8 +462,8 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
         for (int s = 0; s < 4; ++s) G[j] = mma(cur.E[s], cur.D[j][s], G[j]);
     }
     // Z = L^{-1} (H = L L'); Z' in C layout is the A operand of Z G~
-    const f4 Z = chol_lower_inv(H, prm.hfloor, g, c);
-    const f4 Zt = tile_transpose(Z, tbuf, g, c);
+    const f4 Z = chol_lower_inv<16>(H, prm.hfloor, gf, cf);
+    const f4 Zt = tile_transpose(Z, tbuf, gf, cf);
     f4 Pm[NT][NT];
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
@@ -448,8 +489,8 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
           acc = Qc[i][j];
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            if (j == tn) acc[v] += (c == L.cn) ? cur.qrow[i][v] : 0.f;
-            if (i == tn) acc[v] += (4 * g + v == L.cn) ? cur.qcol[j] : 0.f;
+            if (j == tn) acc[v] += (cf == Ls.cn) ? cur.qrow[i][v] : 0.f;
+            if (i == tn) acc[v] += (4 * gf + v == Ls.cn) ? cur.qcol[j] : 0.f;
           }
         } else {
           acc = cur.Q[i][j];
@@ -492,11 +533,11 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
       }
 #pragma unroll
     for (int i = 0; i < NT; ++i) {
-      const f4 d = tile_transpose(S[i][i], tbuf, g, c);
+      const f4 d = tile_transpose(S[i][i], tbuf, gf, cf);
 #pragma unroll
       for (int v = 0; v < 4; ++v) S[i][i][v] = 0.5f * (S[i][i][v] + d[v]);
 #pragma unroll
-      for (int j = i + 1; j < NT; ++j) S[j][i] = tile_transpose(S[i][j], tbuf, g, c);
+      for (int j = i + 1; j < NT; ++j) S[j][i] = tile_transpose(S[i][j], tbuf, gf, cf);
     }
     {  // the corner of S~ (the constant of the value function) is not carried
       const int tn = n >> 4;
@@ -505,7 +546,7 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
         if (i == tn) {
 #pragma unroll
           for (int v = 0; v < 4; ++v)
-            if (4 * g + v == L.cn && c == L.cn) S[i][i][v] = 0.f;
+            if (4 * gf + v == Ls.cn && cf == Ls.cn) S[i][i][v] = 0.f;
         }
     }
 
@@ -513,7 +554,7 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
     WaveStage<NT> nxt;
     // (the stage index is pinned to a scalar register: selected into a vector
     // register it makes every descriptor built from it look divergent)
-    load_wave_stage<NT, CQ>(nxt, P, ws, in, b, __builtin_amdgcn_readfirstlane(t > 0 ? t - 1 : 0), sr, g, c, Ls);
+    load_wave_stage<NT, CQ>(nxt, P, ws, in, b, __builtin_amdgcn_readfirstlane(t > 0 ? t - 1 : 0), sr, gf, cf, Ls);
 
     // stores: S_t, s_t ; K_t, k_t
     store_value_wave<NT>(S, ws, b, t, N1, n, Ls);
