@@ -102,102 +102,6 @@ __device__ __forceinline__ f4 tile_transpose(const f4& x, float* buf, int g, int
   return y;
 }
 
-// Lane j of each 16-lane row of the wave, broadcast to the row (DPP row_newbcast).
-template <int J>
-__device__ __forceinline__ float row_bcast(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x150 + J, 0xf, 0xf, false));
-}
-
-// x from lane 16 (J / 4) + c of the wave (row J of a C-layout tile, for every
-// row of 16 lanes); c4 = 4 c.
-template <int J>
-__device__ __forceinline__ float tile_row(float x, int c4) {
-  return __int_as_float(__builtin_amdgcn_ds_bpermute(c4 + 64 * (J >> 2), __float_as_int(x)));
-}
-
-// y + bcast_J(x) s on the rows of 16 lanes set in RM (lane J of each row
-// broadcast to the row); the other rows keep y.
-template <int J, int RM>
-__device__ __forceinline__ float fma_bcast(float x, float s, float y) {
-  if constexpr (RM == 0xf) {
-    return fmaf(row_bcast<J>(x), s, y);
-  } else {
-    const float xb = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x150 + J, RM, 0xf, false));
-    return fmaf(xb, s, y);
-  }
-}
-
-// First row of 16 lanes g whose row 4 g + v of a tile lies below row J.
-__host__ __device__ constexpr int chol_gmin(int J, int v) { return J >= v ? (J - v) / 4 + 1 : 0; }
-
-// Z[r][:] += H[r][J] nzj for the rows r = 4 g + v > J of register v.
-template <int J, int V>
-__device__ __forceinline__ float chol_zrow(float h, float z, float nzj) {
-  constexpr int gm = chol_gmin(J, V);
-  if constexpr (gm < 4) return fma_bcast<J, (0xf << gm) & 0xf>(h, nzj, z);
-  return z;
-}
-
-// Step J of chol_lower_inv.  Lane (g, c) holds H[4 g + v][c] and Z[4 g + v][c]
-// in register v.  The pivot a = |H[J][J]| comes from one lane (a scalar read)
-// and is raised to fl = floor |H0[J][J]| (H0: the tile before the
-// factorisation; a NaN pivot stays NaN); id = 1 / L[J][J] = a^{-1/2}.
-// H[r][J] reaches the lanes of row r by a DPP broadcast from lane J of the
-// lane's row of 16, H[J][c] and Z[J][c] from row J (tile_row).  The trailing
-// matrix takes H[r][c] -= H[r][J] H[J][c] / a, i.e. H - L[:, J] L[:, J]'; this
-// update runs over every entry: the entries in a row or a column <= J are
-// never read again.  The forward substitution of L Z = I advances by the same
-// column: Z[J][:] *= id, then Z[r][:] -= L[r][J] Z[J][:] for r > J only (DPP
-// row masks select the rows of 16 lanes whose row 4 g + v lies below J).
-template <int J>
-__device__ __forceinline__ void chol_step(f4& h, f4& z, const float (&fl)[16], int c4) {
-  constexpr int jg = J >> 2, jv = J & 3;
-  const float a = fabsf(rlane(h[jv], jg * 16 + J));
-  const float piv = fl[J] > a ? fl[J] : a;
-  const float id = __builtin_amdgcn_rsqf(piv);
-  const float nid2 = -(id * id);
-  const float hJ = tile_row<J>(h[jv], c4);   // H[J][c]
-  const float zJ = tile_row<J>(z[jv], c4);   // Z[J][c] before the step
-  const float nlc = hJ * nid2;               // -H[J][c] / a
-  const float nzj = zJ * nid2;               // -L[J][J]^{-1} Z[J][c] (after the step)
-  // (Z first: it reads column J of H before the trailing update)
-  z[0] = chol_zrow<J, 0>(h[0], z[0], nzj);
-  z[1] = chol_zrow<J, 1>(h[1], z[1], nzj);
-  z[2] = chol_zrow<J, 2>(h[2], z[2], nzj);
-  z[3] = chol_zrow<J, 3>(h[3], z[3], nzj);
-  // row J of Z: Z[J][c] id into the row of 16 lanes jg of register jv
-  z[jv] = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(z[jv]), __float_as_int(zJ * id), 0xE4,
-                                                     1 << jg, 0xf, false));
-#pragma unroll
-  for (int v = 0; v < 4; ++v)
-    if (chol_gmin(J, v) < 4) h[v] = fma_bcast<J, 0xf>(h[v], nlc, h[v]);
-}
-
-template <int J, int NS>
-__device__ __forceinline__ void chol_steps(f4& h, f4& z, const float (&fl)[16], int c4) {
-  if constexpr (J < NS) {
-    chol_step<J>(h, z, fl, c4);
-    chol_steps<J + 1, NS>(h, z, fl, c4);
-  }
-}
-
-// Z = L^{-1} for the symmetric tile H (m x m in the top-left corner, the
-// identity elsewhere; H = L L' with floored pivots), in one sweep of NS >= m
-// steps (straight-line code that the scheduler can place among independent
-// matrix products; past the m x m block every step is the identity, so the
-// sweep may stop at any NS >= m).  H is consumed.
-template <int NS>
-__device__ __forceinline__ f4 chol_lower_inv(f4& h, float floor, int g, int c) {
-  float fl[16];
-#pragma unroll
-  for (int j = 0; j < NS; ++j) fl[j] = floor * fabsf(rlane(h[j & 3], (j >> 2) * 16 + j));
-  f4 z;
-#pragma unroll
-  for (int v = 0; v < 4; ++v) z[v] = (4 * g + v == c) ? 1.f : 0.f;
-  chol_steps<0, NS>(h, z, fl, 4 * c);
-  return z;
-}
-
 // Loop-invariant lane offsets (bytes) of one instance's shapes.
 template <int NT>
 struct LaneOff {
@@ -227,15 +131,6 @@ __device__ __forceinline__ LaneOff<NT> lane_offsets(int n, int m, int g, int c) 
   return L;
 }
 
-// C-layout tile (it, jt) of a row-major matrix with ld n (descriptor rM).
-template <int NT>
-__device__ __forceinline__ f4 tile_nn(rsrc_t rM, int n, int it, int jt, const LaneOff<NT>& L) {
-  f4 x;
-#pragma unroll
-  for (int v = 0; v < 4; ++v) x[v] = rload(rM, L.nn[jt] + (16 * it + v) * n * 4);
-  return x;
-}
-
 // vector entries v[16 it + 4 g + k] (0 past the descriptor) in tile-row layout
 template <int NT>
 __device__ __forceinline__ f4 rows_vec(rsrc_t r, int it, const LaneOff<NT>& L) {
@@ -245,140 +140,279 @@ __device__ __forceinline__ f4 rows_vec(rsrc_t r, int it, const LaneOff<NT>& L) {
   return x;
 }
 
-// Store tile (it, jt) of an (n+1)-wide homogeneous matrix: the first n columns
-// to rM (ld n), column n to rvec; rows past the descriptors are dropped.
+// ---------------------------------------------------------------------------
+// The backward recursion (k_lqr_wave) holds its tiles in the interleaved
+// order: lane (g, c), register v of tile (it, jt) holds
+//   X[16 it + 4 v + g][16 jt + pc],   pc = 4 (c % 4) + c / 4,
+// i.e. tile entry (p, q) = (4 g + v, c) holds the entry at the offsets
+// (pi(p), pi(q)) within the tile, pi(4 a + b) = 4 b + a (its own inverse).
+// Products and transposes of the permuted matrices are the permuted products
+// and transposes, so the matrix-core code is that of the natural order.  What
+// the order buys: step s of a k tile carries the offsets k = 4 s + g
+// (g = 0..3), so the steps past the end of a dimension (n + 1 within the last
+// state tile, m, p) meet only zero rows and are skipped (276 instead of 352
+// matrix instructions per stage at n = 36, m = 12, p = 8); and register v
+// holds the rows 4 v .. 4 v + 3, so the factorisation stops updating a
+// register once its rows are done.  (On gfx950 the FP32 matrix instructions
+// and the vector ALU do not overlap, within a wave or between the waves of a
+// SIMD: every instruction saved is time saved.)
+
+// lane offsets (bytes) of the interleaved order
 template <int NT>
-__device__ __forceinline__ void store_hom(const f4& x, rsrc_t rM, rsrc_t rvec, int n, int it, int jt,
-                                          const LaneOff<NT>& L) {
+struct IlOff {
+  int nn[NT];   // ld n, column 16 jt + pc < n : (g n + 16 jt + pc) * 4
+  int mm;       // ld m, column pc < m          : (g m + pc) * 4
+  int g4;       // a vector along the rows      : 4 g
+  int pc4;      // a vector along the columns   : 4 pc
+  int hom;      // column n's vector            : 4 g on the lanes pc == n % 16
+};
+
+template <int NT>
+__device__ __forceinline__ IlOff<NT> il_offsets(int n, int m, int g, int c) {
+  IlOff<NT> L;
+  const int pc = 4 * (c & 3) + (c >> 2);
 #pragma unroll
-  for (int v = 0; v < 4; ++v) rstore(rM, L.nn[jt] + (16 * it + v) * n * 4, x[v]);
-  if (jt == (n >> 4)) {   // the tile holding column n (uniform)
+  for (int jt = 0; jt < NT; ++jt) L.nn[jt] = (16 * jt + pc < n) ? (g * n + 16 * jt + pc) * 4 : kOOB;
+  L.mm = (pc < m) ? (g * m + pc) * 4 : kOOB;
+  L.g4 = 4 * g;
+  L.pc4 = 4 * pc;
+  L.hom = (pc == (n & 15)) ? 4 * g : kOOB;
+  return L;
+}
+
+// tile (it, jt) of a row-major matrix with ld n
+template <int NT>
+__device__ __forceinline__ f4 il_tile_n(rsrc_t r, int n, int it, int jt, const IlOff<NT>& L) {
+  f4 x;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) rstore(rvec, L.hom + (16 * it + v) * 4, x[v]);
+  for (int v = 0; v < 4; ++v) x[v] = rload(r, L.nn[jt] + (16 * it + 4 * v) * n * 4);
+  return x;
+}
+// tile (it, 0) of a row-major matrix with ld m (columns < m)
+template <int NT>
+__device__ __forceinline__ f4 il_tile_m(rsrc_t r, int m, int it, const IlOff<NT>& L) {
+  f4 x;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) x[v] = rload(r, L.mm + (16 * it + 4 * v) * m * 4);
+  return x;
+}
+// x[16 it + 4 v + g] in register v, on the lanes whose part off is in range
+__device__ __forceinline__ f4 il_rows(rsrc_t r, int off, int it) {
+  f4 x;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) x[v] = rload(r, off + (16 * it + 4 * v) * 4);
+  return x;
+}
+
+// acc + a b over one k tile (4 steps), or its first ks steps (ks uniform, 1..4)
+__device__ __forceinline__ f4 mma_t(const f4& a, const f4& b, f4 acc) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) acc = mma(a[s], b[s], acc);
+  return acc;
+}
+__device__ __forceinline__ f4 mma_ks(const f4& a, const f4& b, f4 acc, int ks) {
+  acc = mma(a[0], b[0], acc);
+  if (ks > 1) acc = mma(a[1], b[1], acc);
+  if (ks > 2) acc = mma(a[2], b[2], acc);
+  if (ks > 3) acc = mma(a[3], b[3], acc);
+  return acc;
+}
+
+// Lane J of each 16-lane row of the wave, broadcast to the row (DPP row_newbcast).
+template <int J>
+__device__ __forceinline__ float row_bcast(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x150 + J, 0xf, 0xf, true));
+}
+// The same on the rows of 16 lanes set in RM; the other rows get 0.
+template <int J, int RM>
+__device__ __forceinline__ float row_bcast_masked(float v) {
+  if constexpr (RM == 0xf) return row_bcast<J>(v);
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x150 + J, RM, 0xf, false));
+}
+
+// Z[r][:] += H[r][pJ] nzj for the rows r = 4 V + g > J of register V (rows of
+// 16 g >= J - 4 V + 1)
+template <int J, int V>
+__device__ __forceinline__ float chol_zrow(float h, float z, float nzj) {
+  constexpr int pJ = 4 * (J & 3) + (J >> 2);
+  constexpr int gm = J - 4 * V + 1 < 0 ? 0 : J - 4 * V + 1;
+  if constexpr (gm < 4) return fmaf(row_bcast_masked<pJ, (0xf << gm) & 0xf>(h), nzj, z);
+  return z;
+}
+
+// Step J (logical pivot J, 0..15) of the factorisation H = L L' with the
+// forward substitution of Z = L^{-1}, on one tile in the interleaved order:
+// row (and column) J sits at tile position pJ = 4 (J % 4) + J / 4, i.e. in
+// register J / 4 of the lanes of row-of-16 J % 4, column lane pJ.  The pivot
+// a = |H[J][J]| comes from one lane (a scalar read) and is raised to
+// fl = floor |H0[J][J]| (H0: the tile before the factorisation; a NaN pivot
+// stays NaN); id = 1 / L[J][J] = a^{-1/2}.  H[r][J] reaches the lanes of row r
+// by a DPP broadcast from lane pJ of the lane's row of 16, H[J][c] and Z[J][c]
+// from row J (a bpermute).  The trailing matrix takes
+// H[r][c] -= H[r][J] H[J][c] / a (= H - L[:, J] L[:, J]') on every register
+// that still holds a row > J; entries in a row or a column <= J are never read
+// again.  The forward substitution advances by the same column: Z[J][:] *= id,
+// Z[r][:] -= L[r][J] Z[J][:] for the rows r > J only (DPP row masks).
+template <int J>
+__device__ __forceinline__ void chol_step(f4& h, f4& z, const float (&fl)[16], int c4) {
+  constexpr int jg = J & 3, jv = J >> 2, pJ = 4 * (J & 3) + (J >> 2);
+  const float a = fabsf(rlane(h[jv], 16 * jg + pJ));
+  const float piv = fl[J] > a ? fl[J] : a;
+  const float id = __builtin_amdgcn_rsqf(piv);
+  const float nid2 = -(id * id);
+  const float hJ = __int_as_float(__builtin_amdgcn_ds_bpermute(c4 + 64 * jg, __float_as_int(h[jv])));
+  const float zJ = __int_as_float(__builtin_amdgcn_ds_bpermute(c4 + 64 * jg, __float_as_int(z[jv])));
+  const float nlc = hJ * nid2;   // -H[J][c] / a
+  const float nzj = zJ * nid2;   // -id Z[J][c] (Z's row J after the step)
+  // Z first: it reads column J of H before the trailing update
+  z[0] = chol_zrow<J, 0>(h[0], z[0], nzj);
+  z[1] = chol_zrow<J, 1>(h[1], z[1], nzj);
+  z[2] = chol_zrow<J, 2>(h[2], z[2], nzj);
+  z[3] = chol_zrow<J, 3>(h[3], z[3], nzj);
+  z[jv] = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(z[jv]), __float_as_int(zJ * id), 0xE4,
+                                                     1 << jg, 0xf, false));
+  // the registers holding a row > J: 4 v + 3 > J
+  if constexpr (3 > J) h[0] = fmaf(row_bcast<pJ>(h[0]), nlc, h[0]);
+  if constexpr (7 > J) h[1] = fmaf(row_bcast<pJ>(h[1]), nlc, h[1]);
+  if constexpr (11 > J) h[2] = fmaf(row_bcast<pJ>(h[2]), nlc, h[2]);
+  if constexpr (15 > J) h[3] = fmaf(row_bcast<pJ>(h[3]), nlc, h[3]);
+}
+
+// Steps J0 .. 15 while J < m (uniform): past the m x m block (the identity)
+// every step is the identity.
+template <int J>
+__device__ __forceinline__ void chol_steps(f4& h, f4& z, const float (&fl)[16], int c4, int m) {
+  if constexpr (J < 16) {
+    if (J < m) {
+      chol_step<J>(h, z, fl, c4);
+      chol_steps<J + 1>(h, z, fl, c4, m);
+    }
   }
 }
 
-// One stage's data in registers (C-layout tiles).
+// One stage's data in the interleaved order.
 template <int NT>
-struct WaveStage {
-  f4 A[NT][NT];   // A~ = [[A, c], [0, 1]]
-  f4 Q[NT][NT];   // Q~ = [[Q, q], [q', 0]], tiles i <= j
-  f4 B[NT];       // B~ tiles (i, 0)
-  f4 D[NT];       // sqrt(rho_c) [D | 0]
-  f4 E;           // sqrt(rho_c) E
-  f4 H;           // R + diag(rd), identity in the padding
-  f4 r;           // r: lane (g, c) holds r[4 g + v]
-  // with stage-invariant Q, B, R (held by the kernel) only the stage's vectors:
-  f4 qrow[NT];    // q[16 i + 4 g + v]: column n of tile row i
-  float qcol[NT]; // q[16 j + c]: row n of tile column j
-  f4 rdv;         // rd[4 g + v]
+struct IlStage {
+  f4 A[NT][NT];   // A~ = [[A, c], [0, 1]] without the corner 1 (added where used)
+  f4 D[NT];       // [D | 0]
+  f4 E;           // E
+  f4 rdv;         // rd by rows
+  f4 rh;          // r by rows on the lanes of column n (G~'s column n)
+  f4 R;           // without CQ: R
+  f4 Q[NT][NT];   // without CQ: Q~ tiles i <= j
+  f4 B[NT];       // without CQ: B~ tiles (i, 0)
+  f4 qrow[NT];    // with CQ: q by rows on the lanes of column n (Q~'s column n)
+  float qcol[NT]; // with CQ: q[16 j + pc] (Q~'s row n)
 };
 
-// Q~ tiles (i <= j, or all) of Q (ld n) and q
+// The loads below do no arithmetic on what they load: an instruction that
+// uses a loaded register waits for the load, so the values are combined
+// where the stage uses them.  Entries that come from another array (column n
+// of A~ and Q~, row n of Q~) are loaded into the same registers under an
+// execution mask of their lanes (the tile's own load returns 0 there).
+
+// Q~ tiles (i <= j, or all) of Q (ld n) and q: column n holds q by rows, row n
+// q along the columns (the corner stays 0: q has n entries).
 template <int NT>
-__device__ __forceinline__ void load_qbar(f4 (&S)[NT][NT], const float* Q, const float* q, int n, int g, int c,
-                                          bool upper_only, const LaneOff<NT>& L) {
-  const int tn = n >> 4;
+__device__ __forceinline__ void il_load_qbar(f4 (&S)[NT][NT], const float* Q, const float* q, int n,
+                                             bool upper_only, int g, bool homlane, const IlOff<NT>& L) {
   const rsrc_t rQ = make_rsrc(Q, n * n), rq = make_rsrc(q, n);
+  const int cn = n & 15;
 #pragma unroll
   for (int i = 0; i < NT; ++i)
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
       if (upper_only && j < i) continue;
-      f4 x = tile_nn<NT>(rQ, n, i, j, L);
-      if (j == tn) {   // column n: q[16 i + 4 g + v]
-        const f4 qr = rows_vec<NT>(rq, i, L);
+      f4 x = il_tile_n<NT>(rQ, n, i, j, L);
+      if (j == NT - 1 && homlane) x = il_rows(rq, L.g4, i);
+      if (i == NT - 1) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) x[v] += (c == L.cn) ? qr[v] : 0.f;
-      }
-      if (i == tn) {   // row n: q[16 j + c]
-        const float qc = rload(rq, L.c4 + 64 * j);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) x[v] += (4 * g + v == L.cn) ? qc : 0.f;
+        for (int v = 0; v < 4; ++v)
+          if (4 * v + g == cn) x[v] = rload(rq, L.pc4 + 64 * j);
       }
       S[i][j] = x;
     }
 }
 
-// CQ: Q, B and R do not vary over the stages and the kernel holds them; the
-// stage brings only q and rd for them.
+// The stage's vectors and constraint rows E (needed when the stage starts).
 template <int NT, bool CQ>
-__device__ __forceinline__ void load_wave_stage(WaveStage<NT>& d, const Prob<float>& P, const Ws<float>& ws,
-                                                const LqrIn<float>& in, int b, int t, float sr, int g, int c,
-                                                const LaneOff<NT>& L) {
-  const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1, tn = n >> 4;
+__device__ __forceinline__ void il_load_vecs(IlStage<NT>& d, const Prob<float>& P, const Ws<float>& ws, int b, int t,
+                                             const IlOff<NT>& L) {
+  const int m = P.m, p = P.p, N = P.N;
+  const rsrc_t rE = make_rsrc(P.E.at(b, t), p * m);   // p == 0: no records, every load returns 0
+#pragma unroll
+  for (int v = 0; v < 4; ++v) d.E[v] = rload(rE, L.mm + 4 * v * m * 4);
+  d.rdv = il_rows(make_rsrc(ws.rd + ((size_t)b * N + t) * m, m), L.g4, 0);
+  d.rh = il_rows(make_rsrc(ws.rh + ((size_t)b * N + t) * m, m), L.hom, 0);
+  if constexpr (!CQ) d.R = il_tile_m<NT>(make_rsrc(P.R.at(b, t), m * m), m, 0, L);
+}
+
+// The stage's matrices: A~ without its corner (column n = c of the next
+// stage's defect), [D | 0], and q (with CQ) or Q~ and B~.
+template <int NT, bool CQ>
+__device__ __forceinline__ void il_load_mats(IlStage<NT>& d, const Prob<float>& P, const Ws<float>& ws,
+                                             const LqrIn<float>& in, int b, int t, int g, bool homlane,
+                                             const IlOff<NT>& L) {
+  const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
   const rsrc_t rA = make_rsrc(in.A.at(b, t), n * n);
-  const rsrc_t rB = make_rsrc(in.Bm.at(b, t), n * m);
   const rsrc_t rc = make_rsrc(ws.cdef + ((size_t)b * N1 + t + 1) * n, n);
 #pragma unroll
-  for (int i = 0; i < NT; ++i) {
+  for (int i = 0; i < NT; ++i)
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
-      f4 x = tile_nn<NT>(rA, n, i, j, L);
-      if (j == tn) {   // column n: c ; corner 1
-        const f4 cv = rows_vec<NT>(rc, i, L);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          x[v] += (c == L.cn) ? cv[v] : 0.f;
-          if (i == tn) x[v] += (c == L.cn && 4 * g + v == L.cn) ? 1.f : 0.f;
-        }
-      }
+      f4 x = il_tile_n<NT>(rA, n, i, j, L);
+      if (j == NT - 1 && homlane) x = il_rows(rc, L.g4, i);
       d.A[i][j] = x;
     }
-  }
+  const rsrc_t rD = make_rsrc(P.D.at(b, t), p * n);
+#pragma unroll
+  for (int j = 0; j < NT; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) d.D[j][v] = rload(rD, L.nn[j] + 4 * v * n * 4);
   if constexpr (CQ) {
     const rsrc_t rq = make_rsrc(ws.qh + ((size_t)b * N1 + t) * n, n);
 #pragma unroll
     for (int i = 0; i < NT; ++i) {
-      d.qrow[i] = rows_vec<NT>(rq, i, L);
-      d.qcol[i] = rload(rq, L.c4 + 64 * i);
+      d.qrow[i] = il_rows(rq, L.hom, i);
+      d.qcol[i] = rload(rq, L.pc4 + 64 * i);
     }
   } else {
-    load_qbar<NT>(d.Q, P.Q.at(b, t), ws.qh + ((size_t)b * N1 + t) * n, n, g, c, true, L);
+    il_load_qbar<NT>(d.Q, P.Q.at(b, t), ws.qh + ((size_t)b * N1 + t) * n, n, true, g, homlane, L);
+    const rsrc_t rB = make_rsrc(in.Bm.at(b, t), n * m);
 #pragma unroll
-    for (int i = 0; i < NT; ++i)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) d.B[i][v] = rload(rB, L.mm + (16 * i + v) * m * 4);
+    for (int i = 0; i < NT; ++i) d.B[i] = il_tile_m<NT>(rB, m, i, L);
   }
-  if (p > 0) {
-    const rsrc_t rD = make_rsrc(P.D.at(b, t), p * n), rE = make_rsrc(P.E.at(b, t), p * m);
+}
+
+// Store tile (it, jt) of an (n+1)-wide homogeneous matrix: the first n columns
+// to rM (ld n), column n (in the last tile column) to rvec; rows past the
+// descriptors are dropped.
+template <int NT>
+__device__ __forceinline__ void il_store_hom(const f4& x, rsrc_t rM, rsrc_t rvec, int n, int it, int jt,
+                                             const IlOff<NT>& L) {
 #pragma unroll
-    for (int j = 0; j < NT; ++j)
+  for (int v = 0; v < 4; ++v) rstore(rM, L.nn[jt] + (16 * it + 4 * v) * n * 4, x[v]);
+  if (jt == NT - 1) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) d.D[j][v] = sr * rload(rD, L.nn[j] + v * n * 4);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) d.E[v] = sr * rload(rE, L.mm + v * m * 4);
-  } else {
-#pragma unroll
-    for (int j = 0; j < NT; ++j) d.D[j] = zero4();
-    d.E = zero4();
-  }
-  const rsrc_t rR = make_rsrc(P.R.at(b, t), m * m);
-  const rsrc_t rd = make_rsrc(ws.rd + ((size_t)b * N + t) * m, m);
-  const rsrc_t rr = make_rsrc(ws.rh + ((size_t)b * N + t) * m, m);
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int row = 4 * g + v;
-    const float dv = rload(rd, L.g16 + v * 4);
-    if constexpr (CQ) d.rdv[v] = dv;
-    else d.H[v] = rload(rR, L.mm + v * m * 4) + (row == c ? (row < m ? dv : 1.f) : 0.f);
-    d.r[v] = rload(rr, L.g16 + v * 4);
+    for (int v = 0; v < 4; ++v) rstore(rvec, L.hom + (16 * it + 4 * v) * 4, x[v]);
   }
 }
 
 template <int NT>
-__device__ __forceinline__ void store_value_wave(const f4 (&S)[NT][NT], const Ws<float>& ws, int b, int t, int N1,
-                                                 int n, const LaneOff<NT>& L) {
+__device__ __forceinline__ void il_store_value(const f4 (&S)[NT][NT], const Ws<float>& ws, int b, int t, int N1,
+                                               int n, const IlOff<NT>& L) {
   const rsrc_t rS = make_rsrc(ws.S + ((size_t)b * N1 + t) * n * n, n * n);
   const rsrc_t rs = make_rsrc(ws.s + ((size_t)b * N1 + t) * n, n);
 #pragma unroll
   for (int it = 0; it < NT; ++it)
 #pragma unroll
-    for (int jt = 0; jt < NT; ++jt) store_hom<NT>(S[it][jt], rS, rs, n, it, jt, L);
+    for (int jt = 0; jt < NT; ++jt) il_store_hom<NT>(S[it][jt], rS, rs, n, it, jt, L);
 }
 
-// segments == 1, FP32, n + 1 <= 16 NT, m <= 16, p <= 16: backward Riccati of
-// one instance per wave; stores S~_t and K~_t for k_fwd_wave.
+// segments == 1, FP32, n + 1 <= 16 NT (NT minimal: column n lies in the last
+// tile), m <= 16, p <= 16: backward Riccati of one instance per wave; stores
+// S~_t and K~_t for k_fwd_wave.  Each register of a stage's data is reloaded
+// with the next stage's as soon as its last use in the stage has passed.
 // CQ: Q, B, R stage-invariant (views of stage stride 0): held in registers
 // for the whole recursion instead of being re-read every stage.
 template <int NT, bool CQ>
@@ -391,79 +425,104 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
   const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
   __shared__ float tbuf[16 * 17];
   const float sr = sqrtf(prm.rho_c);
-  const LaneOff<NT> L = lane_offsets<NT>(n, m, g, c);
+  // k steps of the last state tile (n + 1 - 16 (NT - 1) offsets), of m and of p
+  const int ksn = (n + 1 - 16 * (NT - 1) + 3) >> 2, ksm = (m + 3) >> 2, ksp = (p + 3) >> 2;
+  const int cn = n & 15, pc = 4 * (c & 3) + (c >> 2);
+  const IlOff<NT> L = il_offsets<NT>(n, m, g, c);
 
   f4 S[NT][NT];
-  load_qbar<NT>(S, P.QN.at(b, 0), ws.qh + ((size_t)b * N1 + N) * n, n, g, c, false, L);
-  store_value_wave<NT>(S, ws, b, N, N1, n, L);
+  il_load_qbar<NT>(S, P.QN.at(b, 0), ws.qh + ((size_t)b * N1 + N) * n, n, false, g, pc == cn, L);
 
-  // stage-invariant Q (tiles i <= j, no q), B~ and R (CQ)
+  // stage-invariant Q (tiles i <= j, no q), B~ and R + the padding (CQ)
   f4 Qc[NT][NT], Bc[NT], Rc;
   if constexpr (CQ) {
     const rsrc_t rQ = make_rsrc(P.Q.at(b, 0), n * n), rB = make_rsrc(in.Bm.at(b, 0), n * m);
-    const rsrc_t rR = make_rsrc(P.R.at(b, 0), m * m);
 #pragma unroll
     for (int i = 0; i < NT; ++i)
 #pragma unroll
-      for (int j = i; j < NT; ++j) Qc[i][j] = tile_nn<NT>(rQ, n, i, j, L);
+      for (int j = i; j < NT; ++j) Qc[i][j] = il_tile_n<NT>(rQ, n, i, j, L);
 #pragma unroll
-    for (int i = 0; i < NT; ++i)
+    for (int i = 0; i < NT; ++i) Bc[i] = il_tile_m<NT>(rB, m, i, L);
+    Rc = il_tile_m<NT>(make_rsrc(P.R.at(b, 0), m * m), m, 0, L);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) Bc[i][v] = rload(rB, L.mm + (16 * i + v) * m * 4);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) Rc[v] = rload(rR, L.mm + v * m * 4);
+    for (int v = 0; v < 4; ++v) Rc[v] += (4 * g + v == c && 4 * v + g >= m) ? 1.f : 0.f;
   }
-  WaveStage<NT> cur;
-  load_wave_stage<NT, CQ>(cur, P, ws, in, b, N - 1, sr, g, c, L);
+  IlStage<NT> cur;
+  il_load_vecs<NT, CQ>(cur, P, ws, b, N - 1, L);
+  il_load_mats<NT, CQ>(cur, P, ws, in, b, N - 1, g, pc == cn, L);
+  // S~_N is stored after the first stage's loads are requested, as every
+  // stage's stores follow the next stage's loads: the waits at the top of the
+  // loop then count the same requests on entry as on the way round
+  il_store_value<NT>(S, ws, b, N, N1, n, L);
   for (int t = N - 1; t >= 0; --t) {
-    // lane conditions and offsets from opaque copies of the lane coordinates:
-    // recomputed every stage instead of held (as lane masks in scalar
-    // registers) across the loop
+    // lane offsets from opaque copies of the lane coordinates: recomputed every
+    // stage instead of held across the loop
     const int gf = fresh(g), cf = fresh(c);
-    const LaneOff<NT> Ls = lane_offsets<NT>(n, m, gf, cf);
+    const IlOff<NT> Ls = il_offsets<NT>(n, m, gf, cf);
+    const bool homlane = 4 * (cf & 3) + (cf >> 2) == cn;
+    // masks (0 / 1), computed where used: the tile diagonal and row n
+    f4 dmask, rown;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      dmask[v] = 4 * gf + v == cf ? 1.f : 0.f;
+      rown[v] = 4 * v + gf == cn ? 1.f : 0.f;
+    }
+    // the next stage (the stage index pinned to a scalar register: selected
+    // into a vector register it makes every descriptor built from it look
+    // divergent)
+    const int tp = __builtin_amdgcn_readfirstlane(t > 0 ? t - 1 : 0);
+    f4 Bt[NT];
+#pragma unroll
+    for (int i = 0; i < NT; ++i) Bt[i] = CQ ? Bc[i] : cur.B[i];
     // SB = S~ B~ ; H = R + diag(rd) + rho_c E'E + B~' SB
     f4 SB[NT];
 #pragma unroll
     for (int i = 0; i < NT; ++i) {
       f4 acc = zero4();
 #pragma unroll
-      for (int k = 0; k < NT; ++k)
-#pragma unroll
-        for (int s = 0; s < 4; ++s) acc = mma(S[k][i][s], CQ ? Bc[k][s] : cur.B[k][s], acc);
-      SB[i] = acc;
+      for (int k = 0; k < NT - 1; ++k) acc = mma_t(S[k][i], Bt[k], acc);
+      SB[i] = mma_ks(S[NT - 1][i], Bt[NT - 1], acc, ksn);
     }
     f4 H;
-    if constexpr (CQ) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = 4 * gf + v;
-        H[v] = Rc[v] + (row == cf ? (row < m ? cur.rdv[v] : 1.f) : 0.f);
-      }
-    } else {
-      H = cur.H;
-    }
+    for (int v = 0; v < 4; ++v)
+      H[v] = fmaf(dmask[v], cur.rdv[v], CQ ? Rc[v] : cur.R[v] + (dmask[v] != 0.f && 4 * v + gf >= m ? 1.f : 0.f));
 #pragma unroll
-    for (int k = 0; k < NT; ++k)
-#pragma unroll
-      for (int s = 0; s < 4; ++s) H = mma(CQ ? Bc[k][s] : cur.B[k][s], SB[k][s], H);
-    // G~ = [0 | r] + rho_c E'[D | 0] + B~' S~ A~ ; Pm = Q~ + A~' S~ A~ (tiles
-    // i <= j), one column tile of S~ A~ at a time
+    for (int k = 0; k < NT - 1; ++k) H = mma_t(Bt[k], SB[k], H);
+    H = mma_ks(Bt[NT - 1], SB[NT - 1], H, ksn);
+    // G~ = [0 | r] + rho_c E'[D | 0] (+ B~' S~ A~ below)
     f4 G[NT];
 #pragma unroll
-    for (int j = 0; j < NT; ++j)
+    for (int j = 0; j < NT - 1; ++j) G[j] = zero4();
+    G[NT - 1] = cur.rh;
+    if (p > 0) {   // E and D scaled by sqrt(rho_c) in place
 #pragma unroll
-      for (int v = 0; v < 4; ++v) G[j][v] = (16 * j + cf == n) ? cur.r[v] : 0.f;
-    if (p > 0) {
-#pragma unroll
-      for (int s = 0; s < 4; ++s) H = mma(cur.E[s], cur.E[s], H);
+      for (int v = 0; v < 4; ++v) cur.E[v] *= sr;
 #pragma unroll
       for (int j = 0; j < NT; ++j)
 #pragma unroll
-        for (int s = 0; s < 4; ++s) G[j] = mma(cur.E[s], cur.D[j][s], G[j]);
+        for (int v = 0; v < 4; ++v) cur.D[j][v] *= sr;
+      H = mma_ks(cur.E, cur.E, H, ksp);
+#pragma unroll
+      for (int j = 0; j < NT; ++j) G[j] = mma_ks(cur.E, cur.D[j], G[j], ksp);
     }
-    // Z = L^{-1} (H = L L'); Z' in C layout is the A operand of Z G~
-    const f4 Z = chol_lower_inv<16>(H, prm.hfloor, gf, cf);
-    const f4 Zt = tile_transpose(Z, tbuf, gf, cf);
+    // E, rd, r (and R) of this stage are dead: the next stage's are requested
+    il_load_vecs<NT, CQ>(cur, P, ws, b, tp, Ls);
+
+    // Z = L^{-1} (H = L L'); Z' is the A operand of Z G~
+    float fl[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) fl[j] = prm.hfloor * fabsf(rlane(H[j >> 2], 16 * (j & 3) + 4 * (j & 3) + (j >> 2)));
+    f4 Z;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Z[v] = dmask[v];
+    chol_steps<0>(H, Z, fl, 4 * fresh(c), m);
+    const f4 Zt = tile_transpose(Z, tbuf, g, c);
+
+    // G~ += B~' S~ A~ ; Pm = Q~ + rho_c [D|0]'[D|0] + A~' S~ A~ (tiles i <= j),
+    // a column tile of S~ A~ at a time
+#pragma unroll
+    for (int v = 0; v < 4; ++v) cur.A[NT - 1][NT - 1][v] += homlane ? rown[v] : 0.f;
     f4 Pm[NT][NT];
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
@@ -472,99 +531,73 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
       for (int i = 0; i < NT; ++i) {
         f4 acc = zero4();
 #pragma unroll
-        for (int k = 0; k < NT; ++k)
-#pragma unroll
-          for (int s = 0; s < 4; ++s) acc = mma(S[k][i][s], cur.A[k][j][s], acc);
-        SAc[i] = acc;
+        for (int k = 0; k < NT - 1; ++k) acc = mma_t(S[k][i], cur.A[k][j], acc);
+        SAc[i] = mma_ks(S[NT - 1][i], cur.A[NT - 1][j], acc, ksn);
       }
 #pragma unroll
-      for (int k = 0; k < NT; ++k)
-#pragma unroll
-        for (int s = 0; s < 4; ++s) G[j] = mma(CQ ? Bc[k][s] : cur.B[k][s], SAc[k][s], G[j]);
+      for (int k = 0; k < NT - 1; ++k) G[j] = mma_t(Bt[k], SAc[k], G[j]);
+      G[j] = mma_ks(Bt[NT - 1], SAc[NT - 1], G[j], ksn);
 #pragma unroll
       for (int i = 0; i <= j; ++i) {
         f4 acc;
         if constexpr (CQ) {   // Q~ = [[Q, q], [q', 0]] from the held Q and this stage's q
-          const int tn = n >> 4;
           acc = Qc[i][j];
+          if (j == NT - 1) {
 #pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            if (j == tn) acc[v] += (cf == Ls.cn) ? cur.qrow[i][v] : 0.f;
-            if (i == tn) acc[v] += (4 * gf + v == Ls.cn) ? cur.qcol[j] : 0.f;
+            for (int v = 0; v < 4; ++v) acc[v] += cur.qrow[i][v];
+          }
+          if (i == NT - 1) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[v] = fmaf(rown[v], cur.qcol[j], acc[v]);
           }
         } else {
           acc = cur.Q[i][j];
         }
+        if (p > 0) acc = mma_ks(cur.D[i], cur.D[j], acc, ksp);
 #pragma unroll
-        for (int k = 0; k < NT; ++k)
-#pragma unroll
-          for (int s = 0; s < 4; ++s) acc = mma(cur.A[k][i][s], SAc[k][s], acc);
-        Pm[i][j] = acc;
+        for (int k = 0; k < NT - 1; ++k) acc = mma_t(cur.A[k][i], SAc[k], acc);
+        Pm[i][j] = mma_ks(cur.A[NT - 1][i], SAc[NT - 1], acc, ksn);
       }
     }
+    // A~, D and q (or Q~ and B~) of this stage are dead
+    il_load_mats<NT, CQ>(cur, P, ws, in, b, tp, gf, homlane, Ls);
 
-    // W~ = Z G~ (in G's registers), K~ = Z' W~
-    f4 K[NT];
+    // W~ = Z G~, K~ = Z' W~ (sums over the m rows of G~ and W~)
+    f4 K[NT], Wn[NT];
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
-      f4 w = zero4();
-#pragma unroll
-      for (int s = 0; s < 4; ++s) w = mma(Zt[s], G[j][s], w);
+      const f4 w = mma_ks(Zt, G[j], zero4(), ksm);
       G[j] = w;
-      f4 acc = zero4();
+      K[j] = mma_ks(Z, w, zero4(), ksm);
 #pragma unroll
-      for (int s = 0; s < 4; ++s) acc = mma(Z[s], w[s], acc);
-      K[j] = acc;
+      for (int v = 0; v < 4; ++v) Wn[j][v] = -w[v];
     }
-
-    // S~ <- Pm + rho_c [D|0]'[D|0] - W~' W~ (tiles i <= j, then mirrored)
+    // S~ <- Pm - W~' W~ (tiles i <= j, then mirrored)
 #pragma unroll
     for (int i = 0; i < NT; ++i)
 #pragma unroll
-      for (int j = i; j < NT; ++j) {
-        f4 acc = Pm[i][j];
-#pragma unroll
-        for (int s = 0; s < 4; ++s) acc = mma(-G[i][s], G[j][s], acc);
-        if (p > 0) {
-#pragma unroll
-          for (int s = 0; s < 4; ++s) acc = mma(cur.D[i][s], cur.D[j][s], acc);
-        }
-        S[i][j] = acc;
-      }
+      for (int j = i; j < NT; ++j) S[i][j] = mma_ks(Wn[i], G[j], Pm[i][j], ksm);
 #pragma unroll
     for (int i = 0; i < NT; ++i) {
-      const f4 d = tile_transpose(S[i][i], tbuf, gf, cf);
+      const f4 d = tile_transpose(S[i][i], tbuf, g, c);
 #pragma unroll
       for (int v = 0; v < 4; ++v) S[i][i][v] = 0.5f * (S[i][i][v] + d[v]);
 #pragma unroll
-      for (int j = i + 1; j < NT; ++j) S[j][i] = tile_transpose(S[i][j], tbuf, gf, cf);
+      for (int j = i + 1; j < NT; ++j) S[j][i] = tile_transpose(S[i][j], tbuf, g, c);
     }
-    {  // the corner of S~ (the constant of the value function) is not carried
-      const int tn = n >> 4;
+    // the corner of S~ (the constant of the value function) is not carried
 #pragma unroll
-      for (int i = 0; i < NT; ++i)
-        if (i == tn) {
-#pragma unroll
-          for (int v = 0; v < 4; ++v)
-            if (4 * gf + v == Ls.cn && cf == Ls.cn) S[i][i][v] = 0.f;
-        }
-    }
-
-    // the next stage's data is requested before this stage's stores
-    WaveStage<NT> nxt;
-    // (the stage index is pinned to a scalar register: selected into a vector
-    // register it makes every descriptor built from it look divergent)
-    load_wave_stage<NT, CQ>(nxt, P, ws, in, b, __builtin_amdgcn_readfirstlane(t > 0 ? t - 1 : 0), sr, gf, cf, Ls);
+    for (int v = 0; v < 4; ++v)
+      if (homlane && 4 * v + gf == cn) S[NT - 1][NT - 1][v] = 0.f;
 
     // stores: S_t, s_t ; K_t, k_t
-    store_value_wave<NT>(S, ws, b, t, N1, n, Ls);
+    il_store_value<NT>(S, ws, b, t, N1, n, Ls);
     {
       const rsrc_t rK = make_rsrc(ws.K + ((size_t)b * N + t) * m * n, m * n);
       const rsrc_t rk = make_rsrc(ws.k + ((size_t)b * N + t) * m, m);
 #pragma unroll
-      for (int j = 0; j < NT; ++j) store_hom<NT>(K[j], rK, rk, n, 0, j, Ls);
+      for (int j = 0; j < NT; ++j) il_store_hom<NT>(K[j], rK, rk, n, 0, j, Ls);
     }
-    cur = nxt;
   }
 }
 
