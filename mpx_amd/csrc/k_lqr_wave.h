@@ -204,18 +204,31 @@ __device__ __forceinline__ f4 il_rows(rsrc_t r, int off, int it) {
   return x;
 }
 
-// acc + a b over one k tile (4 steps), or its first ks steps (ks uniform, 1..4)
-__device__ __forceinline__ f4 mma_t(const f4& a, const f4& b, f4 acc) {
+// acc + a b over the first K steps of one k tile (K = 4: the whole tile)
+template <int K>
+__device__ __forceinline__ f4 mma_k(const f4& a, const f4& b, f4 acc) {
 #pragma unroll
-  for (int s = 0; s < 4; ++s) acc = mma(a[s], b[s], acc);
+  for (int s = 0; s < K; ++s) acc = mma(a[s], b[s], acc);
   return acc;
 }
+// the same with a uniform run-time K (1..4)
 __device__ __forceinline__ f4 mma_ks(const f4& a, const f4& b, f4 acc, int ks) {
   acc = mma(a[0], b[0], acc);
   if (ks > 1) acc = mma(a[1], b[1], acc);
   if (ks > 2) acc = mma(a[2], b[2], acc);
   if (ks > 3) acc = mma(a[3], b[3], acc);
   return acc;
+}
+// f(integral_constant<int, k>) for the uniform k = ks in KMIN..KMAX: one
+// branch per section of code instead of one per step
+template <int KMIN, int KMAX, typename F>
+__device__ __forceinline__ void with_steps(int ks, F&& f) {
+  if constexpr (KMIN == KMAX) {
+    f(std::integral_constant<int, KMAX>{});
+  } else {
+    if (ks <= KMIN) f(std::integral_constant<int, KMIN>{});
+    else with_steps<KMIN + 1, KMAX>(ks, f);
+  }
 }
 
 // Lane J of each 16-lane row of the wave, broadcast to the row (DPP row_newbcast).
@@ -415,7 +428,9 @@ __device__ __forceinline__ void il_store_value(const f4 (&S)[NT][NT], const Ws<f
 // with the next stage's as soon as its last use in the stage has passed.
 // CQ: Q, B, R stage-invariant (views of stage stride 0): held in registers
 // for the whole recursion instead of being re-read every stage.
-template <int NT, bool CQ>
+// KSN: the k steps of the last state tile, (n + 1 - 16 (NT - 1) + 3) / 4 or
+// more (steps past n + 1 add zeros).
+template <int NT, bool CQ, int KSN>
 __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> prm, Iter<float> it_unused,
                                                     Ws<float> ws, LqrIn<float> in) {
   (void)it_unused;
@@ -425,8 +440,8 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
   const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
   __shared__ float tbuf[16 * 17];
   const float sr = sqrtf(prm.rho_c);
-  // k steps of the last state tile (n + 1 - 16 (NT - 1) offsets), of m and of p
-  const int ksn = (n + 1 - 16 * (NT - 1) + 3) >> 2, ksm = (m + 3) >> 2, ksp = (p + 3) >> 2;
+  // k steps over m and over p
+  const int ksm = (m + 3) >> 2, ksp = (p + 3) >> 2;
   const int cn = n & 15, pc = 4 * (c & 3) + (c >> 2);
   const IlOff<NT> L = il_offsets<NT>(n, m, g, c);
 
@@ -480,16 +495,16 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
     for (int i = 0; i < NT; ++i) {
       f4 acc = zero4();
 #pragma unroll
-      for (int k = 0; k < NT - 1; ++k) acc = mma_t(S[k][i], Bt[k], acc);
-      SB[i] = mma_ks(S[NT - 1][i], Bt[NT - 1], acc, ksn);
+      for (int k = 0; k < NT - 1; ++k) acc = mma_k<4>(S[k][i], Bt[k], acc);
+      SB[i] = mma_k<KSN>(S[NT - 1][i], Bt[NT - 1], acc);
     }
     f4 H;
 #pragma unroll
     for (int v = 0; v < 4; ++v)
       H[v] = fmaf(dmask[v], cur.rdv[v], CQ ? Rc[v] : cur.R[v] + (dmask[v] != 0.f && 4 * v + gf >= m ? 1.f : 0.f));
 #pragma unroll
-    for (int k = 0; k < NT - 1; ++k) H = mma_t(Bt[k], SB[k], H);
-    H = mma_ks(Bt[NT - 1], SB[NT - 1], H, ksn);
+    for (int k = 0; k < NT - 1; ++k) H = mma_k<4>(Bt[k], SB[k], H);
+    H = mma_k<KSN>(Bt[NT - 1], SB[NT - 1], H);
     // G~ = [0 | r] + rho_c E'[D | 0] (+ B~' S~ A~ below)
     f4 G[NT];
 #pragma unroll
@@ -502,9 +517,12 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
       for (int j = 0; j < NT; ++j)
 #pragma unroll
         for (int v = 0; v < 4; ++v) cur.D[j][v] *= sr;
-      H = mma_ks(cur.E, cur.E, H, ksp);
+      with_steps<1, 4>(ksp, [&](auto KP) {
+        constexpr int kp = decltype(KP)::value;
+        H = mma_k<kp>(cur.E, cur.E, H);
 #pragma unroll
-      for (int j = 0; j < NT; ++j) G[j] = mma_ks(cur.E, cur.D[j], G[j], ksp);
+        for (int j = 0; j < NT; ++j) G[j] = mma_k<kp>(cur.E, cur.D[j], G[j]);
+      });
     }
     // E, rd, r (and R) of this stage are dead: the next stage's are requested
     il_load_vecs<NT, CQ>(cur, P, ws, b, tp, Ls);
@@ -531,12 +549,12 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
       for (int i = 0; i < NT; ++i) {
         f4 acc = zero4();
 #pragma unroll
-        for (int k = 0; k < NT - 1; ++k) acc = mma_t(S[k][i], cur.A[k][j], acc);
-        SAc[i] = mma_ks(S[NT - 1][i], cur.A[NT - 1][j], acc, ksn);
+        for (int k = 0; k < NT - 1; ++k) acc = mma_k<4>(S[k][i], cur.A[k][j], acc);
+        SAc[i] = mma_k<KSN>(S[NT - 1][i], cur.A[NT - 1][j], acc);
       }
 #pragma unroll
-      for (int k = 0; k < NT - 1; ++k) G[j] = mma_t(Bt[k], SAc[k], G[j]);
-      G[j] = mma_ks(Bt[NT - 1], SAc[NT - 1], G[j], ksn);
+      for (int k = 0; k < NT - 1; ++k) G[j] = mma_k<4>(Bt[k], SAc[k], G[j]);
+      G[j] = mma_k<KSN>(Bt[NT - 1], SAc[NT - 1], G[j]);
 #pragma unroll
       for (int i = 0; i <= j; ++i) {
         f4 acc;
@@ -555,28 +573,32 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
         }
         if (p > 0) acc = mma_ks(cur.D[i], cur.D[j], acc, ksp);
 #pragma unroll
-        for (int k = 0; k < NT - 1; ++k) acc = mma_t(cur.A[k][i], SAc[k], acc);
-        Pm[i][j] = mma_ks(cur.A[NT - 1][i], SAc[NT - 1], acc, ksn);
+        for (int k = 0; k < NT - 1; ++k) acc = mma_k<4>(cur.A[k][i], SAc[k], acc);
+        Pm[i][j] = mma_k<KSN>(cur.A[NT - 1][i], SAc[NT - 1], acc);
       }
     }
     // A~, D and q (or Q~ and B~) of this stage are dead
     il_load_mats<NT, CQ>(cur, P, ws, in, b, tp, gf, homlane, Ls);
 
-    // W~ = Z G~, K~ = Z' W~ (sums over the m rows of G~ and W~)
-    f4 K[NT], Wn[NT];
-#pragma unroll
-    for (int j = 0; j < NT; ++j) {
-      const f4 w = mma_ks(Zt, G[j], zero4(), ksm);
-      G[j] = w;
-      K[j] = mma_ks(Z, w, zero4(), ksm);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) Wn[j][v] = -w[v];
-    }
+    // W~ = Z G~, K~ = Z' W~ (sums over the m rows of G~ and W~);
     // S~ <- Pm - W~' W~ (tiles i <= j, then mirrored)
+    f4 K[NT];
+    with_steps<1, 4>(ksm, [&](auto KM) {
+      constexpr int km = decltype(KM)::value;
+      f4 Wn[NT];
 #pragma unroll
-    for (int i = 0; i < NT; ++i)
+      for (int j = 0; j < NT; ++j) {
+        const f4 w = mma_k<km>(Zt, G[j], zero4());
+        G[j] = w;
+        K[j] = mma_k<km>(Z, w, zero4());
 #pragma unroll
-      for (int j = i; j < NT; ++j) S[i][j] = mma_ks(Wn[i], G[j], Pm[i][j], ksm);
+        for (int v = 0; v < 4; ++v) Wn[j][v] = -w[v];
+      }
+#pragma unroll
+      for (int i = 0; i < NT; ++i)
+#pragma unroll
+        for (int j = i; j < NT; ++j) S[i][j] = mma_k<km>(Wn[i], G[j], Pm[i][j]);
+    });
 #pragma unroll
     for (int i = 0; i < NT; ++i) {
       const f4 d = tile_transpose(S[i][i], tbuf, g, c);

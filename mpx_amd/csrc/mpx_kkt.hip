@@ -439,16 +439,31 @@ mpxkkt_status run_lqr_wave(hipStream_t st, const mpxkkt_problem* pr, const Prob<
                            const Iter<T>& I, const Ws<T>& W, const LqrIn<T>& in) {
   if constexpr (IsF32<T>::value) {
     const int NT = (pr->n + 1 + 15) / 16;
+    // k steps of the last state tile: the n + 1 - 16 (NT - 1) offsets it holds, by 4
+    const int ksn = (pr->n + 1 - 16 * (NT - 1) + 3) / 4;
     const bool cq = P.Q.st == 0 && P.R.st == 0 && in.Bm.st == 0;   // Q, B, R held for the whole recursion
     const dim3 gb(pr->batch);
+    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, gb, dim3(64), 0, st, P, R, I, W, in); };
     if (cq) {
-      if (NT == 1) hipLaunchKernelGGL((k_lqr_wave<1, true>), gb, dim3(64), 0, st, P, R, I, W, in);
-      else if (NT == 2) hipLaunchKernelGGL((k_lqr_wave<2, true>), gb, dim3(64), 0, st, P, R, I, W, in);
-      else hipLaunchKernelGGL((k_lqr_wave<3, true>), gb, dim3(64), 0, st, P, R, I, W, in);
-    } else {
-      if (NT == 1) hipLaunchKernelGGL((k_lqr_wave<1, false>), gb, dim3(64), 0, st, P, R, I, W, in);
-      else if (NT == 2) hipLaunchKernelGGL((k_lqr_wave<2, false>), gb, dim3(64), 0, st, P, R, I, W, in);
-      else hipLaunchKernelGGL((k_lqr_wave<3, false>), gb, dim3(64), 0, st, P, R, I, W, in);
+      // one instantiation per step count: the steps are unrolled matrix instructions
+      switch (NT * 4 + ksn) {
+        case 5: go(k_lqr_wave<1, true, 1>); break;
+        case 6: go(k_lqr_wave<1, true, 2>); break;
+        case 7: go(k_lqr_wave<1, true, 3>); break;
+        case 8: go(k_lqr_wave<1, true, 4>); break;
+        case 9: go(k_lqr_wave<2, true, 1>); break;
+        case 10: go(k_lqr_wave<2, true, 2>); break;
+        case 11: go(k_lqr_wave<2, true, 3>); break;
+        case 12: go(k_lqr_wave<2, true, 4>); break;
+        case 13: go(k_lqr_wave<3, true, 1>); break;
+        case 14: go(k_lqr_wave<3, true, 2>); break;
+        case 15: go(k_lqr_wave<3, true, 3>); break;
+        default: go(k_lqr_wave<3, true, 4>); break;
+      }
+    } else {   // stage-varying data: the generic step count (the steps past n + 1 add zeros)
+      if (NT == 1) go(k_lqr_wave<1, false, 4>);
+      else if (NT == 2) go(k_lqr_wave<2, false, 4>);
+      else go(k_lqr_wave<3, false, 4>);
     }
     return launch_check("k_lqr_wave");
   } else {

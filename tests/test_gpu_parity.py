@@ -137,6 +137,7 @@ def test_shift_and_nan_fallback():
 
 @pytest.mark.parametrize("wl,shape,iters", [
     (1, dict(n=12, m=12, N=20), 1),            # LTV, one tile (n + 1 <= 16)
+    (1, dict(n=40, m=10, N=8), 1),             # LTV, three tiles: stage-varying Q, B, R
     (2, dict(n=15, m=16, N=14), 3),            # box, n + 1 = 16 exactly, m = 16
     (3, dict(n=16, m=3, N=11, p=16), 2),       # two tiles, p = 16
     (3, dict(n=31, m=7, N=9, p=5), 2),         # n + 1 = 32
@@ -149,6 +150,33 @@ def test_fp32_register_wave_kernel(wl, shape, iters):
     insts, starts, refs, out, _, _ = _run(wl, 3, iters, torch.float32, shape=shape)
     _, _, _, out_blk, _, _ = _run(wl, 3, iters, torch.float32, shape=shape, flags=2)
     for k in range(iters):
+        g, st = out[k]
+        gb, _ = out_blk[k]
+        for i in range(3):
+            r = refs[i][k]
+            assert st["status"][i] == 0
+            for key, tol in (("X", TOL32_X), ("U", 10 * TOL32_X), ("V", 10 * TOL32_X)):
+                assert _rel(g[key][i], r[key]) < tol, (k, i, key, _rel(g[key][i], r[key]))
+                assert _rel(g[key][i], gb[key][i]) < tol, (k, i, key, _rel(g[key][i], gb[key][i]))
+
+
+@pytest.mark.parametrize("shape", [
+    dict(n=3, m=1, N=7, p=0),      # one tile: 1 matrix step in the last state tile, 1 over m, none over p
+    dict(n=6, m=5, N=7, p=3),      # 2 / 2 / 1 steps
+    dict(n=10, m=9, N=7, p=6),     # 3 / 3 / 2
+    dict(n=20, m=13, N=6, p=11),   # two tiles: 2 / 4 / 3
+    dict(n=26, m=16, N=6, p=16),   # 3 / 4 / 4
+    dict(n=32, m=2, N=5, p=1),     # three tiles: 1 / 1 / 1
+    dict(n=41, m=11, N=5, p=9),    # 3 / 3 / 3
+])
+def test_fp32_wave_kernel_step_counts(shape):
+    """The wave LQR kernel skips the matrix steps past n + 1 in the last state
+    tile (one instantiation per step count) and past m and p (a switch per
+    section): shapes that reach every step count, against the FP64 oracle and
+    the LDS block kernel."""
+    insts, starts, refs, out, _, _ = _run(3, 3, 2, torch.float32, shape=shape)
+    _, _, _, out_blk, _, _ = _run(3, 3, 2, torch.float32, shape=shape, flags=2)
+    for k in range(2):
         g, st = out[k]
         gb, _ = out_blk[k]
         for i in range(3):
