@@ -231,76 +231,100 @@ __device__ __forceinline__ void with_steps(int ks, F&& f) {
   }
 }
 
+// f(integral_constant<int, i>) for i = B .. E - 1
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
 // Lane J of each 16-lane row of the wave, broadcast to the row (DPP row_newbcast).
 template <int J>
 __device__ __forceinline__ float row_bcast(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x150 + J, 0xf, 0xf, true));
 }
-// The same on the rows of 16 lanes set in RM; the other rows get 0.
+// y + x[lane J of the row of 16] s on the rows of 16 lanes set in RM (a DPP
+// broadcast fused into the multiply-add; the other rows keep y).  The DPP
+// source must not have been written in the last two instructions: the nop
+// covers that, whatever the compiler places before the block.
 template <int J, int RM>
-__device__ __forceinline__ float row_bcast_masked(float v) {
-  if constexpr (RM == 0xf) return row_bcast<J>(v);
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x150 + J, RM, 0xf, false));
+__device__ __forceinline__ float fmac_bcast(float y, float x, float s) {
+  asm("s_nop 1\n\tv_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:%4 bank_mask:0xf"
+      : "+v"(y)
+      : "v"(x), "v"(s), "n"(J), "n"(RM));
+  return y;
 }
 
-// Z[r][:] += H[r][pJ] nzj for the rows r = 4 V + g > J of register V (rows of
-// 16 g >= J - 4 V + 1)
-template <int J, int V>
-__device__ __forceinline__ float chol_zrow(float h, float z, float nzj) {
-  constexpr int pJ = 4 * (J & 3) + (J >> 2);
-  constexpr int gm = J - 4 * V + 1 < 0 ? 0 : J - 4 * V + 1;
-  if constexpr (gm < 4) return fmaf(row_bcast_masked<pJ, (0xf << gm) & 0xf>(h), nzj, z);
-  return z;
+// Lane l gets x from lane 16 JG + l % 16: the row of 16 lanes JG broadcast to
+// all four.  v_permlane16_swap exchanges the odd rows of its first operand with
+// the even rows of its second, v_permlane32_swap the upper half of the first
+// with the lower half of the second; with x as both operands the first gives
+// the rows (r0, r0, r2, r2) and (r1, r1, r3, r3), the second of those
+// (y0, y1, y0, y1) and (y2, y3, y2, y3).
+template <int JG>
+__device__ __forceinline__ float row16_bcast(float x) {
+  const unsigned u = __float_as_uint(x);
+  const auto p = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+  const unsigned y = (JG & 1) ? p[1] : p[0];
+  const auto q = __builtin_amdgcn_permlane32_swap(y, y, false, false);
+  return __uint_as_float((JG & 2) ? q[1] : q[0]);
 }
 
-// Step J (logical pivot J, 0..15) of the factorisation H = L L' with the
-// forward substitution of Z = L^{-1}, on one tile in the interleaved order:
-// row (and column) J sits at tile position pJ = 4 (J % 4) + J / 4, i.e. in
-// register J / 4 of the lanes of row-of-16 J % 4, column lane pJ.  The pivot
-// a = |H[J][J]| comes from one lane (a scalar read) and is raised to
-// fl = floor |H0[J][J]| (H0: the tile before the factorisation; a NaN pivot
-// stays NaN); id = 1 / L[J][J] = a^{-1/2}.  H[r][J] reaches the lanes of row r
-// by a DPP broadcast from lane pJ of the lane's row of 16, H[J][c] and Z[J][c]
-// from row J (a bpermute).  The trailing matrix takes
-// H[r][c] -= H[r][J] H[J][c] / a (= H - L[:, J] L[:, J]') on every register
-// that still holds a row > J; entries in a row or a column <= J are never read
-// again.  The forward substitution advances by the same column: Z[J][:] *= id,
-// Z[r][:] -= L[r][J] Z[J][:] for the rows r > J only (DPP row masks).
-template <int J>
-__device__ __forceinline__ void chol_step(f4& h, f4& z, const float (&fl)[16], int c4) {
+// Step J (logical pivot J) of the factorisation H = L L' with the forward
+// substitution of Z = L^{-1}, on one tile in the interleaved order, for
+// m <= 4 KM: the rows >= 4 KM (registers >= KM) are the identity padding and
+// are never touched.  Row (and column) J sits at tile position
+// pJ = 4 (J % 4) + J / 4, i.e. in register J / 4 of the lanes of row-of-16
+// J % 4, column lane pJ.  Row J of H and of Z reaches every lane (row16_bcast:
+// lane (g, c) gets H[J][pc]); the pivot a = |H[J][J]| is lane pJ of it and is
+// raised to fl = floor |H0[J][J]| (H0: the tile before the factorisation; a NaN
+// pivot stays NaN); id = 1 / L[J][J] = a^{-1/2}.  H[r][J] reaches the lanes of
+// row r by a DPP broadcast from lane pJ of the lane's row of 16.  The trailing
+// matrix takes H[r][c] -= H[r][J] H[J][c] / a (= H - L[:, J] L[:, J]') on every
+// register that still holds a row > J; entries in a row or a column <= J are
+// never read again.  The forward substitution advances by the same column:
+// Z[J][:] *= id, Z[r][:] -= L[r][J] Z[J][:] for the rows r > J only (DPP row
+// masks).  Steps past m (pivot 1, zero column) change nothing that is read.
+template <int J, int KM>
+__device__ __forceinline__ void chol_step(f4& h, f4& z, const float (&fl)[16]) {
   constexpr int jg = J & 3, jv = J >> 2, pJ = 4 * (J & 3) + (J >> 2);
-  const float a = fabsf(rlane(h[jv], 16 * jg + pJ));
+  const float hJ = row16_bcast<jg>(h[jv]);
+  const float zJ = row16_bcast<jg>(z[jv]);
+  const float a = fabsf(row_bcast<pJ>(hJ));
   const float piv = fl[J] > a ? fl[J] : a;
   const float id = __builtin_amdgcn_rsqf(piv);
   const float nid2 = -(id * id);
-  const float hJ = __int_as_float(__builtin_amdgcn_ds_bpermute(c4 + 64 * jg, __float_as_int(h[jv])));
-  const float zJ = __int_as_float(__builtin_amdgcn_ds_bpermute(c4 + 64 * jg, __float_as_int(z[jv])));
   const float nlc = hJ * nid2;   // -H[J][c] / a
   const float nzj = zJ * nid2;   // -id Z[J][c] (Z's row J after the step)
-  // Z first: it reads column J of H before the trailing update
-  z[0] = chol_zrow<J, 0>(h[0], z[0], nzj);
-  z[1] = chol_zrow<J, 1>(h[1], z[1], nzj);
-  z[2] = chol_zrow<J, 2>(h[2], z[2], nzj);
-  z[3] = chol_zrow<J, 3>(h[3], z[3], nzj);
+  // Z first: it reads column J of H before the trailing update.  Register v
+  // holds the rows 4 v + g: those > J are the rows of 16 g >= J - 4 v + 1.
+  static_for<0, KM>([&](auto V) {
+    constexpr int v = decltype(V)::value;
+    constexpr int gm = J - 4 * v + 1 < 0 ? 0 : J - 4 * v + 1;
+    if constexpr (gm < 4) z[v] = fmac_bcast<pJ, (0xf << gm) & 0xf>(z[v], h[v], nzj);
+  });
   z[jv] = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(z[jv]), __float_as_int(zJ * id), 0xE4,
                                                      1 << jg, 0xf, false));
   // the registers holding a row > J: 4 v + 3 > J
-  if constexpr (3 > J) h[0] = fmaf(row_bcast<pJ>(h[0]), nlc, h[0]);
-  if constexpr (7 > J) h[1] = fmaf(row_bcast<pJ>(h[1]), nlc, h[1]);
-  if constexpr (11 > J) h[2] = fmaf(row_bcast<pJ>(h[2]), nlc, h[2]);
-  if constexpr (15 > J) h[3] = fmaf(row_bcast<pJ>(h[3]), nlc, h[3]);
+  static_for<0, KM>([&](auto V) {
+    constexpr int v = decltype(V)::value;
+    if constexpr (4 * v + 3 > J) h[v] = fmac_bcast<pJ, 0xf>(h[v], h[v], nlc);
+  });
 }
 
-// Steps J0 .. 15 while J < m (uniform): past the m x m block (the identity)
-// every step is the identity.
-template <int J>
-__device__ __forceinline__ void chol_steps(f4& h, f4& z, const float (&fl)[16], int c4, int m) {
-  if constexpr (J < 16) {
-    if (J < m) {
-      chol_step<J>(h, z, fl, c4);
-      chol_steps<J + 1>(h, z, fl, c4, m);
-    }
-  }
+// Z = L^{-1} for the tile H (m x m in the top-left corner, m <= 4 KM, the
+// identity elsewhere; H = L L' with floored pivots); z enters as the identity.
+// H is consumed.
+template <int KM>
+__device__ __forceinline__ void chol_lower_inv(f4& h, f4& z, float floor) {
+  float fl[16];
+  static_for<0, 4 * KM>([&](auto JJ) {
+    constexpr int J = decltype(JJ)::value;
+    fl[J] = floor * fabsf(rlane(h[J >> 2], 16 * (J & 3) + 4 * (J & 3) + (J >> 2)));
+  });
+  static_for<0, 4 * KM>([&](auto JJ) { chol_step<decltype(JJ)::value, KM>(h, z, fl); });
 }
 
 // One stage's data in the interleaved order.
@@ -528,13 +552,10 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
     il_load_vecs<NT, CQ>(cur, P, ws, b, tp, Ls);
 
     // Z = L^{-1} (H = L L'); Z' is the A operand of Z G~
-    float fl[16];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) fl[j] = prm.hfloor * fabsf(rlane(H[j >> 2], 16 * (j & 3) + 4 * (j & 3) + (j >> 2)));
     f4 Z;
 #pragma unroll
     for (int v = 0; v < 4; ++v) Z[v] = dmask[v];
-    chol_steps<0>(H, Z, fl, 4 * fresh(c), m);
+    with_steps<1, 4>(ksm, [&](auto KM) { chol_lower_inv<decltype(KM)::value>(H, Z, prm.hfloor); });
     const f4 Zt = tile_transpose(Z, tbuf, g, c);
 
     // G~ += B~' S~ A~ ; Pm = Q~ + rho_c [D|0]'[D|0] + A~' S~ A~ (tiles i <= j),
