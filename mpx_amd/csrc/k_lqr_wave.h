@@ -678,22 +678,70 @@ __device__ __forceinline__ void load_aop(f4 (&out)[RT][NT], const float* M, cons
   }
 }
 
-// One forward step's A-operand tiles: A~_t, K~_t and (unless the kernel holds
-// a stage-invariant B, CB) B~_t.
-template <int NT, bool CB>
-__device__ __forceinline__ void load_chain_stage(f4 (&A)[NT][NT], f4 (&K)[1][NT], f4 (&BT)[NT], const Ws<float>& ws,
-                                                 const LqrIn<float>& in, int b, int t, int N, int n, int m, int g,
-                                                 int c, const int (&off)[NT][4], const LaneOff<NT>& L) {
-  const int N1 = N + 1;
-  load_aop<NT, NT>(A, in.A.at(b, t), ws.cdef + ((size_t)b * N1 + t + 1) * n, n, n, true, g, c, off, L);
-  load_aop<NT, 1>(K, ws.K + ((size_t)b * N + t) * m * n, ws.k + ((size_t)b * N + t) * m, n, m, false, g, c, off,
-                  L);
-  if constexpr (CB) return;
-  const rsrc_t rB = make_rsrc(in.Bm.at(b, t), n * m);
+// A-operand tiles of a row-major block M (ld n, nrows rows) without any
+// column n: lane (g, c) of tile (i, k) holds M[16 i + c][16 k + 4 g + s] in
+// register s, 0 past column n - 1 or row nrows - 1 (no arithmetic on the
+// loaded values: a use of a loaded register waits for its load).  WIDE
+// (n % 4 == 0): a lane's four columns 16 k + 4 g .. + 3 of a row are
+// contiguous, 16-byte aligned, and either all below n or all past it: one
+// 128-bit load.  (A compile-time choice: after a run-time branch between the
+// two forms the wait counts of the loads in flight are not known.)
+template <int NT, int RT, bool WIDE>
+__device__ __forceinline__ void load_aop_raw(f4 (&out)[RT][NT], const float* M, int n, int nrows,
+                                             const int (&off)[NT][4]) {
+  const rsrc_t rM = make_rsrc(M, nrows * n);
+  if constexpr (WIDE) {
 #pragma unroll
-  for (int i = 0; i < NT; ++i)
+    for (int i = 0; i < RT; ++i)
 #pragma unroll
-    for (int s = 0; s < 4; ++s) BT[i][s] = rload(rB, L.bt[s] + 16 * i * m * 4);   // B[16 i + c][4 g + s]
+      for (int k = 0; k < NT; ++k) out[i][k] = rload4(rM, off[k][0] + 16 * i * n * 4);
+  } else {
+#pragma unroll
+    for (int i = 0; i < RT; ++i)
+#pragma unroll
+      for (int k = 0; k < NT; ++k)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) out[i][k][s] = rload(rM, off[k][s] + 16 * i * n * 4);
+  }
+}
+
+// One forward step's data: the A-operand tiles of A_t and K_t (column n of
+// A~ and K~ left out: their loads return 0 there), and the columns n as
+// vectors: lane column i < NT of colc holds c_{t+1}[16 i + 4 g + v] in
+// register v (the other lanes 0), every lane of colk holds k_t[4 g + v].  A
+// product's accumulator starts from its lane column broadcast along the row of
+// 16 (DPP), or from colk: every column of the result then carries the term,
+// column 0 (dx's) included; the other columns are never read.  (No branches
+// and no arithmetic on loaded values: nothing waits for these loads before the
+// next stage uses them.)
+template <int NT>
+struct FwdA {
+  f4 A[NT][NT];
+  f4 colc;
+};
+template <int NT>
+struct FwdK {
+  f4 K[1][NT];
+  f4 colk;
+};
+
+template <int NT, bool WIDE>
+__device__ __forceinline__ void fwd_load_a(FwdA<NT>& d, const Ws<float>& ws, const LqrIn<float>& in, int b, int t,
+                                           int N, int n, int g, int c, const int (&off)[NT][4]) {
+  load_aop_raw<NT, NT, WIDE>(d.A, in.A.at(b, t), n, n, off);
+  const rsrc_t rc = make_rsrc(ws.cdef + ((size_t)b * (N + 1) + t + 1) * n, n);
+  const int oc = c < NT ? (16 * c + 4 * g) * 4 : kOOB;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) d.colc[v] = rload(rc, oc + v * 4);
+}
+
+template <int NT, bool WIDE>
+__device__ __forceinline__ void fwd_load_k(FwdK<NT>& d, const Ws<float>& ws, int b, int t, int N, int n, int m,
+                                           int g, const int (&off)[NT][4]) {
+  load_aop_raw<NT, 1, WIDE>(d.K, ws.K + ((size_t)b * N + t) * m * n, n, m, off);
+  const rsrc_t rk = make_rsrc(ws.k + ((size_t)b * N + t) * m, m);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) d.colk[v] = rload(rk, (4 * g + v) * 4);
 }
 
 // lane's byte offsets of columns 16 k + 4 g + s within a row of ld n (kOOB
@@ -710,13 +758,16 @@ __device__ __forceinline__ void aop_offsets(int (&off)[NT][4], int n, int g, int
 }
 
 // Forward pass of the LQR solve, one wave per instance (the gains K~_t of
-// k_lqr_wave): with x~_t = [dx_t; 1] as column 0 of B-operand tiles,
-//   du_t = -K~_t x~_t,  x~_{t+1} = A~_t x~_t + B~_t du_t
-// on the matrix cores; writes dx and du.  The next stage's tiles are requested
-// before this stage's stores.
-// CB: B~ stage-invariant (stride 0), loaded once.
-template <int NT, bool CB>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 3))) k_fwd_wave(Prob<float> P, Ws<float> ws, LqrIn<float> in) {
+// k_lqr_wave): with dx_t as column 0 of B-operand tiles,
+//   du_t = -(K_t dx_t + k_t),  dx_{t+1} = A_t dx_t + c_{t+1} + B_t du_t
+// on the matrix cores (the affine terms start the accumulators, so the
+// homogeneous coordinate of the backward kernel is not needed); writes dx and
+// du.  Stage t + 1's data is requested before stage t's products, into the
+// other of two buffers (the loop is unrolled by two, so no buffer is copied).
+// CB: B~ stage-invariant (stride 0), loaded once; else loaded every stage.
+// WIDE: n % 4 == 0 (128-bit loads of the A-operand tiles).
+template <int NT, bool CB, bool WIDE>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) k_fwd_wave(Prob<float> P, Ws<float> ws, LqrIn<float> in) {
   const int b = blockIdx.x;
   if (skip_instance(ws, b)) return;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
@@ -724,51 +775,66 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 3)))
   const LaneOff<NT> L = lane_offsets<NT>(n, m, g, c);
   int off[NT][4];
   aop_offsets<NT>(off, n, g, c);
-  f4 x[NT];
+  f4 x[NT];   // dx_0 = c_init
   {
     const rsrc_t r0 = make_rsrc(ws.cdef + (size_t)b * N1 * n, n);
 #pragma unroll
     for (int k = 0; k < NT; ++k)
 #pragma unroll
-      for (int v = 0; v < 4; ++v)
-        x[k][v] = rload(r0, L.col0 + (16 * k + v) * 4) + ((c == 0 && 16 * k + 4 * g + v == n) ? 1.f : 0.f);
+      for (int v = 0; v < 4; ++v) x[k][v] = rload(r0, L.col0 + (16 * k + v) * 4);
   }
-  f4 cA[NT][NT], cK[1][NT], cB[NT];
-  if constexpr (CB) {
-    const rsrc_t rB = make_rsrc(in.Bm.at(b, 0), n * m);
+  // -B~ (stage-invariant)
+  f4 nB[NT];
+  auto load_nb = [&](int t) {
+    const rsrc_t rB = make_rsrc(in.Bm.at(b, t), n * m);
 #pragma unroll
     for (int i = 0; i < NT; ++i)
 #pragma unroll
-      for (int s = 0; s < 4; ++s) cB[i][s] = rload(rB, L.bt[s] + 16 * i * m * 4);
-  }
-  load_chain_stage<NT, CB>(cA, cK, cB, ws, in, b, 0, N, n, m, g, c, off, L);
-  for (int t = 0; t < N; ++t) {
-    // K~ x~ (tile 0, column 0), then x~_{t+1} = A~ x~ - B~ (K~ x~)
-    f4 kx = zero4();
+      for (int s = 0; s < 4; ++s) nB[i][s] = -rload(rB, L.bt[s] + 16 * i * m * 4);   // B[16 i + c][4 g + s]
+  };
+  if constexpr (CB) load_nb(0);
+  FwdA<NT> s0, s1;
+  FwdK<NT> kk;
+  fwd_load_a<NT, WIDE>(s0, ws, in, b, 0, N, n, g, c, off);
+  fwd_load_k<NT, WIDE>(kk, ws, b, 0, N, n, m, g, off);
+  auto step = [&](FwdA<NT>& cur, FwdA<NT>& nxt, int t) {
+    const int tn = __builtin_amdgcn_readfirstlane(t + 1 < N ? t + 1 : t);   // the next stage
+    const int gf = fresh(g), cf = fresh(c);
+    int offs[NT][4];
+    aop_offsets<NT>(offs, n, gf, cf);
+    fwd_load_a<NT, WIDE>(nxt, ws, in, b, tn, N, n, gf, cf, offs);
+    if constexpr (!CB) load_nb(t);
+    // the loads stay here, a stage ahead of their use (under register pressure
+    // the scheduler would sink them towards their uses in the next step)
+    __builtin_amdgcn_sched_barrier(0);
+    // K dx + k (tile 0, column 0), then dx_{t+1} = A dx + c - B (K dx + k)
+    f4 kx = kk.colk;
 #pragma unroll
     for (int k = 0; k < NT; ++k)
 #pragma unroll
-      for (int s = 0; s < 4; ++s) kx = mma(cK[0][k][s], x[k][s], kx);
+      for (int s = 0; s < 4; ++s) kx = mma(kk.K[0][k][s], x[k][s], kx);
+    // K and k of this stage are dead: the next stage's, in place
+    fwd_load_k<NT, WIDE>(kk, ws, b, tn, N, n, m, gf, offs);
+    __builtin_amdgcn_sched_barrier(0);
     f4 y[NT];
 #pragma unroll
     for (int i = 0; i < NT; ++i) {
-      f4 acc = zero4();
+      f4 acc;
+      auto bc = [&](auto I) {
+        constexpr int ii = decltype(I)::value;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[v] = row_bcast<ii>(cur.colc[v]);
+      };
+      if (i == 0) bc(std::integral_constant<int, 0>{});
+      else if (i == 1) bc(std::integral_constant<int, 1>{});
+      else bc(std::integral_constant<int, 2>{});
 #pragma unroll
       for (int k = 0; k < NT; ++k)
 #pragma unroll
-        for (int s = 0; s < 4; ++s) acc = mma(cA[i][k][s], x[k][s], acc);
+        for (int s = 0; s < 4; ++s) acc = mma(cur.A[i][k][s], x[k][s], acc);
 #pragma unroll
-      for (int s = 0; s < 4; ++s) acc = mma(-cB[i][s], kx[s], acc);
+      for (int s = 0; s < 4; ++s) acc = mma(nB[i][s], kx[s], acc);
       y[i] = acc;
-    }
-    f4 nA[NT][NT], nK[1][NT], nB[NT];
-    {
-      const int gf = fresh(g), cf = fresh(c);
-      const LaneOff<NT> Ls = lane_offsets<NT>(n, m, gf, cf);
-      int offs[NT][4];
-      aop_offsets<NT>(offs, n, gf, cf);
-      load_chain_stage<NT, CB>(nA, nK, nB, ws, in, b, __builtin_amdgcn_readfirstlane(t + 1 < N ? t + 1 : t), N, n, m,
-                               gf, cf, offs, Ls);
     }
     {
       const rsrc_t rdx = make_rsrc(ws.dx + ((size_t)b * N1 + t) * n, n);
@@ -781,14 +847,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 3)))
       for (int v = 0; v < 4; ++v) rstore(rdu, L.col0 + v * 4, -kx[v]);
     }
 #pragma unroll
-    for (int k = 0; k < NT; ++k) {
-      x[k] = y[k];
-      cK[0][k] = nK[0][k];
-      if constexpr (!CB) cB[k] = nB[k];
-#pragma unroll
-      for (int i = 0; i < NT; ++i) cA[i][k] = nA[i][k];
-    }
+    for (int k = 0; k < NT; ++k) x[k] = y[k];
+  };
+  int t = 0;
+  for (; t + 1 < N; t += 2) {
+    step(s0, s1, t);
+    step(s1, s0, t + 1);
   }
+  if (t < N) step(s0, s1, t);
   {
     const rsrc_t rdx = make_rsrc(ws.dx + ((size_t)b * N1 + N) * n, n);
 #pragma unroll

@@ -352,6 +352,17 @@ mpxkkt_status run_scan(hipStream_t st, const mpxkkt_problem* pr, int nt, const P
 template <typename T>
 bool use_wave_lqr(const mpxkkt_problem* pr, const Prm<T>& R);
 
+// k_fwd_wave for a stage-invariant B (cb) or not, with 128-bit tile loads when
+// n % 4 == 0 (wide)
+template <int NT>
+void fwd_wave(hipStream_t st, dim3 gb, bool cb, bool wide, const Prob<float>& P, const Ws<float>& W,
+              const LqrIn<float>& in) {
+  if (cb && wide) hipLaunchKernelGGL((k_fwd_wave<NT, true, true>), gb, dim3(64), 0, st, P, W, in);
+  else if (cb) hipLaunchKernelGGL((k_fwd_wave<NT, true, false>), gb, dim3(64), 0, st, P, W, in);
+  else if (wide) hipLaunchKernelGGL((k_fwd_wave<NT, false, true>), gb, dim3(64), 0, st, P, W, in);
+  else hipLaunchKernelGGL((k_fwd_wave<NT, false, false>), gb, dim3(64), 0, st, P, W, in);
+}
+
 // costates, control steps and slope partials: on the wave path the forward
 // pass of the LQR solve with them (k_fwd_wave), else parallel over stages
 // after the dx chain (k_fwd_stage)
@@ -365,16 +376,13 @@ mpxkkt_status run_fwd_stage(hipStream_t st, const mpxkkt_problem* pr, const Prob
       const dim3 gb(pr->batch);
       const bool cb = in.Bm.st == 0;   // B~ held for the whole chain
       if (NT == 1) {
-        if (cb) hipLaunchKernelGGL((k_fwd_wave<1, true>), gb, dim3(64), 0, st, P, W, in);
-        else hipLaunchKernelGGL((k_fwd_wave<1, false>), gb, dim3(64), 0, st, P, W, in);
+        fwd_wave<1>(st, gb, cb, (pr->n & 3) == 0, P, W, in);
         hipLaunchKernelGGL(k_costate_wave<1>, gc, dim3(64), 0, st, P, I, W);
       } else if (NT == 2) {
-        if (cb) hipLaunchKernelGGL((k_fwd_wave<2, true>), gb, dim3(64), 0, st, P, W, in);
-        else hipLaunchKernelGGL((k_fwd_wave<2, false>), gb, dim3(64), 0, st, P, W, in);
+        fwd_wave<2>(st, gb, cb, (pr->n & 3) == 0, P, W, in);
         hipLaunchKernelGGL(k_costate_wave<2>, gc, dim3(64), 0, st, P, I, W);
       } else {
-        if (cb) hipLaunchKernelGGL((k_fwd_wave<3, true>), gb, dim3(64), 0, st, P, W, in);
-        else hipLaunchKernelGGL((k_fwd_wave<3, false>), gb, dim3(64), 0, st, P, W, in);
+        fwd_wave<3>(st, gb, cb, (pr->n & 3) == 0, P, W, in);
         hipLaunchKernelGGL(k_costate_wave<3>, gc, dim3(64), 0, st, P, I, W);
       }
       return launch_check("k_fwd_wave / k_costate_wave");
