@@ -128,7 +128,11 @@ typedef struct {
                              bit 2: single-kernel line search (one block per
                              instance) instead of the stage-parallel one;
                              bit 3: linearisation parallel over stages (lower
-                             latency for small batches; n, m, n_eq <= 64) */
+                             latency for small batches; n, m, n_eq <= 64);
+                             bit 4: line-search trials with one row per lane
+                             instead of the matrix-core form (FP32, tanh model
+                             with stage-invariant A, W, B, Q, R, n <= 48,
+                             m <= 16, n_eq <= 16) */
   double hess_floor;      /* pivot floor of the m x m Cholesky factorisations
                              (H_t = R + B'S B, ...): pivot j is raised to
                              hess_floor * |H_jj| when it falls below; keeps

@@ -191,13 +191,14 @@ __device__ int ls_first_accept(const Prm<T>& prm, const Ws<T>& ws, int b, int N1
   return -1;
 }
 
-// Trials j0 <= j < j1 at stage t = blockIdx.x * waves + wave of instance
+// Trials j0 <= j < j1 at stage t = t0 + blockIdx.x * waves + wave of instance
 // blockIdx.y.  RPL: state (and equality) rows per lane, ceil(max(n, p) / 64).
 template <typename T, int RPL, bool V4, bool SH>
-__global__ void __launch_bounds__(512) k_trials(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, int j0, int j1) {
+__global__ void __launch_bounds__(512) k_trials(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, int j0, int j1,
+                                                int t0) {
   static_assert(!V4 || IsF32<T>::value, "float4 rows are FP32 only");
   const int wpb = blockDim.x >> 6, wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int b = blockIdx.y, t = blockIdx.x * wpb + wv;
+  const int b = blockIdx.y, t = t0 + blockIdx.x * wpb + wv;
   if (skip_instance(ws, b)) return;
   const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
   // an earlier pass accepted: the same answer in every wave of the block (one instance)

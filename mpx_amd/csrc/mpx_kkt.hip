@@ -20,6 +20,7 @@
 #include "k_scan.h"
 #include "k_linesearch.h"
 #include "k_trials.h"
+#include "k_trials_mma.h"
 #include "k_misc.h"
 #include "k_caller.h"
 #include "k_srbd.h"
@@ -527,13 +528,15 @@ bool trial_rows16(const mpxkkt_problem* pr) {
 // in LDS, else 4 waves each read their rows from memory.
 struct TrialShape {
   bool sh, v4;
+  int mma;   // > 0: k_trials_mma<mma> for the stages < N (its state tiles)
+  bool mma_wide;   // the problem's rows of k_trials_mma 16-byte aligned (n, m multiples of 4)
   int waves;
   size_t lds;
   dim3 grid;
 };
 
 template <typename T>
-TrialShape trial_shape(const mpxkkt_problem* pr) {
+TrialShape trial_shape(const mpxkkt_problem* pr, int flags) {
   TrialShape ts;
   const int N1 = pr->horizon + 1;
   ts.v4 = trial_rows16<T>(pr);
@@ -542,6 +545,15 @@ TrialShape trial_shape(const mpxkkt_problem* pr) {
   ts.sh = pr->model == MPXKKT_MODEL_TANH && pr->Q.stage_stride == 0 && pr->R.stage_stride == 0 &&
           lds_limit(&lim) == MPXKKT_OK && shared * sizeof(T) <= (size_t)lim;
   ts.waves = ts.sh ? 8 : kTrialWaves;
+  // the matrix-core form: FP32, the shared matrices stage-invariant, at most
+  // three state tiles, one control and one equality-row tile, and the flag
+  // bit 4 clear (bit 4 keeps the row-per-lane form, for comparison)
+  ts.mma = 0;
+  if (IsF32<T>::value && ts.sh && pr->W.stage_stride == 0 && pr->A.stage_stride == 0 &&
+      pr->B.stage_stride == 0 && pr->n <= 48 && pr->m <= 16 && pr->n_eq <= 16 && (flags & 16) == 0)
+    ts.mma = (pr->n + 15) / 16;
+  ts.mma_wide = ts.mma && ts.v4 && rows16(pr->xref, pr->n) && rows16(pr->uref, pr->m) &&
+                (!pr->has_box || (rows16(pr->ulo, pr->m) && rows16(pr->uhi, pr->m)));
   ts.lds = (ts.sh ? shared : (size_t)kTrialWaves * trial_wave_elems(pr->n, pr->m)) * sizeof(T);
   ts.grid = dim3((N1 + ts.waves - 1) / ts.waves, pr->batch);
   return ts;
@@ -551,8 +563,34 @@ template <typename T, int RPL, bool V4, bool SH>
 mpxkkt_status launch_trials_k(hipStream_t st, const TrialShape& ts, const Prob<T>& P, const Prm<T>& R,
                               const Iter<T>& I, const Ws<T>& W, int j0, int j1) {
   mpxkkt_status s;
+  if constexpr (IsF32<T>::value) {
+    if (ts.mma) {
+      // stages 0 .. N - 1 on the matrix cores, the terminal stage by one wave of k_trials
+      const int N = P.N;
+      const dim3 grid((N + 15) / 16, P.B);
+      // 128-bit loads when the iterate and the step are 16-byte aligned too
+      auto al16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
+      const bool wide = ts.mma_wide && al16(I.X) && al16(I.U) && al16(I.V) && al16(W.dx) && al16(W.du) &&
+                        al16(W.dv);
+      auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(64), 0, st, P, R, I, W, j0, j1); };
+      if (wide) {
+        if (ts.mma == 1) go(k_trials_mma<1, true>);
+        else if (ts.mma == 2) go(k_trials_mma<2, true>);
+        else go(k_trials_mma<3, true>);
+      } else {
+        if (ts.mma == 1) go(k_trials_mma<1, false>);
+        else if (ts.mma == 2) go(k_trials_mma<2, false>);
+        else go(k_trials_mma<3, false>);
+      }
+      if ((s = launch_check("k_trials_mma")) != MPXKKT_OK) return s;
+      const size_t lds = (size_t)trial_wave_elems(P.n, P.m) * sizeof(T);
+      if ((s = prep_lds(k_trials<T, RPL, V4, false>, lds)) != MPXKKT_OK) return s;
+      hipLaunchKernelGGL((k_trials<T, RPL, V4, false>), dim3(1, P.B), dim3(64), lds, st, P, R, I, W, j0, j1, N);
+      return launch_check("k_trials");
+    }
+  }
   if ((s = prep_lds(k_trials<T, RPL, V4, SH>, ts.lds)) != MPXKKT_OK) return s;
-  hipLaunchKernelGGL((k_trials<T, RPL, V4, SH>), ts.grid, dim3(64 * ts.waves), ts.lds, st, P, R, I, W, j0, j1);
+  hipLaunchKernelGGL((k_trials<T, RPL, V4, SH>), ts.grid, dim3(64 * ts.waves), ts.lds, st, P, R, I, W, j0, j1, 0);
   return launch_check("k_trials");
 }
 
@@ -588,7 +626,7 @@ mpxkkt_status run_linesearch(hipStream_t st, const mpxkkt_problem* pr, const Pro
     const size_t setup_lds = (size_t)6 * 256 * sizeof(T);
     hipLaunchKernelGGL(k_ls_setup<T>, dim3(B), dim3(256), setup_lds, st, P, R, I, W);
     if ((s = launch_check("k_ls_setup")) != MPXKKT_OK) return s;
-    const TrialShape ts = trial_shape<T>(pr);
+    const TrialShape ts = trial_shape<T>(pr, R.flags);
     if ((s = run_trials<T>(st, pr, ts, P, R, I, W, 0, R.group)) != MPXKKT_OK) return s;
     if (R.group < R.max_trials && (s = run_trials<T>(st, pr, ts, P, R, I, W, R.group, R.max_trials)) != MPXKKT_OK)
       return s;

@@ -226,6 +226,35 @@ def test_stage_parallel_line_search(wl, shape, iters, start, group, dtype):
                     assert _rel(g[key][i], r[key]) < 10 * tol, (k, i, key, _rel(g[key][i], r[key]))
 
 
+@pytest.mark.parametrize("group", [1, 0])
+@pytest.mark.parametrize("wl,shape,iters,start", [
+    (2, dict(n=8, m=4, N=12), 4, "zero"),              # box, one tile, cold start: several trials
+    (2, dict(n=20, m=13, N=37), 3, "zero"),            # box, two tiles, 3 waves of 16 stages (the last partial)
+    (3, dict(n=36, m=12, N=50, p=8), 2, None),         # the benchmark shape, equality rows
+    (3, dict(n=47, m=16, N=17, p=16), 2, "zero"),      # three full tiles, 16 rows, N = 16 + 1
+])
+def test_trials_matrix_core_form(wl, shape, iters, start, group):
+    """FP32 trials on the matrix cores (k_trials_mma, the default for these
+    shapes: 16 stages per wave, products of the stage-invariant matrices with
+    the stages' vectors as columns) against the row-per-lane k_trials (flags
+    bit 4) and the oracle: the same accepted step and trial count, iterates
+    equal to rounding."""
+    kw = dict(shape=shape, start=start, trial_group=group)
+    insts, starts, refs, out, _, _ = _run(wl, 3, iters, torch.float32, **kw)
+    _, _, _, rows, _, _ = _run(wl, 3, iters, torch.float32, flags=16, **kw)
+    for k in range(iters):
+        (g, st), (gr, sr) = out[k], rows[k]
+        for i in range(3):
+            assert st["status"][i] == sr["status"][i], (k, i)
+            assert st["trials"][i] == sr["trials"][i], (k, i, st["trials"][i], sr["trials"][i])
+            assert st["alpha"][i] == pytest.approx(sr["alpha"][i], rel=1e-5), (k, i)
+            for key in ("X", "U", "V", "Yeq", "Ybox"):
+                if g[key].size == 0:
+                    continue
+                assert _rel(g[key][i], gr[key][i]) < 10 * TOL32_X, (k, i, key, _rel(g[key][i], gr[key][i]))
+            assert _rel(g["X"][i], refs[i][k]["X"]) < 10 * TOL32_X, (k, i)
+
+
 @pytest.mark.parametrize("wl,shape,iters,start", [
     (2, dict(n=8, m=4, N=12), 4, "zero"),
     (3, dict(n=36, m=12, N=10, p=8), 2, None),
