@@ -21,6 +21,7 @@
 #include "k_linesearch.h"
 #include "k_trials.h"
 #include "k_trials_mma.h"
+#include "k_lin_mma.h"
 #include "k_misc.h"
 #include "k_caller.h"
 #include "k_srbd.h"
@@ -397,11 +398,54 @@ mpxkkt_status run_fwd_stage(hipStream_t st, const mpxkkt_problem* pr, const Prob
   return launch_check("k_fwd_stage");
 }
 
+// Rows of a view of row length L start 16-byte aligned (null views pass).
+bool rows16(const mpxkkt_view& v, long long L) {
+  return v.ptr == nullptr || ((reinterpret_cast<uintptr_t>(v.ptr) & 15) == 0 && v.batch_stride % 4 == 0 &&
+                              v.stage_stride % 4 == 0 && L % 4 == 0);
+}
+
+template <typename T>
+bool trial_rows16(const mpxkkt_problem* pr) {
+  if constexpr (!IsF32<T>::value) {
+    return false;
+  } else {
+    const long long n = pr->n, m = pr->m;
+    return rows16(pr->A, n) && rows16(pr->W, n) && rows16(pr->B, m) && rows16(pr->Q, n) && rows16(pr->R, m) &&
+           rows16(pr->QN, n) && (pr->n_eq == 0 || (rows16(pr->D, n) && rows16(pr->E, m)));
+  }
+}
+
+// The matrix-core forms of the linearisation and of the trials
+// (k_lin_mma.h, k_trials_mma.h): FP32, the tanh model with stage-invariant A,
+// W, B, Q, R, at most three state tiles, one control tile and one tile of
+// equality rows, and flags bit 4 clear (bit 4 keeps the row-per-lane forms,
+// for comparison).  Returns the number of state tiles, 0 when they do not apply.
+template <typename T>
+int mma_tiles(const mpxkkt_problem* pr, int flags) {
+  if (!IsF32<T>::value || pr->model != MPXKKT_MODEL_TANH || (flags & 16) != 0) return 0;
+  if (pr->A.stage_stride != 0 || pr->W.stage_stride != 0 || pr->B.stage_stride != 0 || pr->Q.stage_stride != 0 ||
+      pr->R.stage_stride != 0)
+    return 0;
+  if (pr->n > 48 || pr->m > 16 || pr->n_eq > 16) return 0;
+  return (pr->n + 15) / 16;
+}
+
+// 128-bit loads and stores in the matrix-core forms: every row of the
+// problem's views they read 16-byte aligned (n, m multiples of 4)
+template <typename T>
+bool mma_rows16(const mpxkkt_problem* pr) {
+  return trial_rows16<T>(pr) && rows16(pr->xref, pr->n) && rows16(pr->uref, pr->m) &&
+         (!pr->has_box || (rows16(pr->ulo, pr->m) && rows16(pr->uhi, pr->m)));
+}
+
+inline bool al16(const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; }
+
 // K1: with flags bit 3 parallel over stages with shared constants
 // (k_linearize_stage: lower latency for small batches) when n, m, n_eq <= 64
-// and its LDS plan fits; else one block per instance (k_linearize_inst, higher
-// throughput for large batches) under the same limits; the general block
-// kernel otherwise.
+// and its LDS plan fits; else the matrix-core form (k_linearize_mma, 16 stages
+// per wave, and k_linearize_term) where mma_tiles allows it; else one block
+// per instance (k_linearize_inst, higher throughput for large batches) under
+// the limits of k_linearize_stage; the general block kernel otherwise.
 template <typename T>
 mpxkkt_status run_linearize(hipStream_t st, const mpxkkt_problem* pr, const Prob<T>& P, const Prm<T>& R,
                             const Iter<T>& I, const Ws<T>& W) {
@@ -418,6 +462,30 @@ mpxkkt_status run_linearize(hipStream_t st, const mpxkkt_problem* pr, const Prob
       const dim3 grid((pr->horizon + kLinStageWaves) / kLinStageWaves, pr->batch);
       hipLaunchKernelGGL(k_linearize_stage<T>, grid, dim3(64 * kLinStageWaves), lds, st, P, R, I, W);
       return launch_check("k_linearize_stage");
+    }
+  }
+  if constexpr (IsF32<T>::value) {
+    const int nt = caller ? 0 : mma_tiles<T>(pr, R.flags);
+    if (nt) {
+      // 128-bit loads and stores when the iterate and the outputs are 16-byte aligned too
+      const bool wide = mma_rows16<T>(pr) && al16(I.X) && al16(I.U) && al16(I.V) &&
+                        (!pr->has_box || al16(I.Ybox)) && al16(W.qh) && al16(W.cdef) && al16(W.rh) &&
+                        al16(W.rd) && al16(W.Alin);
+      const dim3 grid((pr->horizon + 15) / 16, pr->batch);
+      auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(64), 0, st, P, R, I, W); };
+      if (wide) {
+        if (nt == 1) go(k_linearize_mma<1, true>);
+        else if (nt == 2) go(k_linearize_mma<2, true>);
+        else go(k_linearize_mma<3, true>);
+      } else {
+        if (nt == 1) go(k_linearize_mma<1, false>);
+        else if (nt == 2) go(k_linearize_mma<2, false>);
+        else go(k_linearize_mma<3, false>);
+      }
+      mpxkkt_status s = launch_check("k_linearize_mma");
+      if (s != MPXKKT_OK) return s;
+      hipLaunchKernelGGL(k_linearize_term<T>, dim3(pr->batch), dim3(64), 0, st, P, I, W);
+      return launch_check("k_linearize_term");
     }
   }
   if (!caller && use_inst<T>(pr, false)) {
@@ -506,23 +574,6 @@ mpxkkt_status run_lqr(hipStream_t st, const mpxkkt_problem* pr, const mpxkkt_par
   return run_lqr_m<T, 64>(st, pr, nt, P, R, I, W, in);
 }
 
-// Rows of a view of row length L start 16-byte aligned (null views pass).
-bool rows16(const mpxkkt_view& v, long long L) {
-  return v.ptr == nullptr || ((reinterpret_cast<uintptr_t>(v.ptr) & 15) == 0 && v.batch_stride % 4 == 0 &&
-                              v.stage_stride % 4 == 0 && L % 4 == 0);
-}
-
-template <typename T>
-bool trial_rows16(const mpxkkt_problem* pr) {
-  if constexpr (!IsF32<T>::value) {
-    return false;
-  } else {
-    const long long n = pr->n, m = pr->m;
-    return rows16(pr->A, n) && rows16(pr->W, n) && rows16(pr->B, m) && rows16(pr->Q, n) && rows16(pr->R, m) &&
-           rows16(pr->QN, n) && (pr->n_eq == 0 || (rows16(pr->D, n) && rows16(pr->E, m)));
-  }
-}
-
 // Launch shape of k_trials: with `sh` (tanh model, stage-invariant Q and R,
 // the shared constants fit) 8 waves per block share the instance's matrices
 // in LDS, else 4 waves each read their rows from memory.
@@ -545,15 +596,8 @@ TrialShape trial_shape(const mpxkkt_problem* pr, int flags) {
   ts.sh = pr->model == MPXKKT_MODEL_TANH && pr->Q.stage_stride == 0 && pr->R.stage_stride == 0 &&
           lds_limit(&lim) == MPXKKT_OK && shared * sizeof(T) <= (size_t)lim;
   ts.waves = ts.sh ? 8 : kTrialWaves;
-  // the matrix-core form: FP32, the shared matrices stage-invariant, at most
-  // three state tiles, one control and one equality-row tile, and the flag
-  // bit 4 clear (bit 4 keeps the row-per-lane form, for comparison)
-  ts.mma = 0;
-  if (IsF32<T>::value && ts.sh && pr->W.stage_stride == 0 && pr->A.stage_stride == 0 &&
-      pr->B.stage_stride == 0 && pr->n <= 48 && pr->m <= 16 && pr->n_eq <= 16 && (flags & 16) == 0)
-    ts.mma = (pr->n + 15) / 16;
-  ts.mma_wide = ts.mma && ts.v4 && rows16(pr->xref, pr->n) && rows16(pr->uref, pr->m) &&
-                (!pr->has_box || (rows16(pr->ulo, pr->m) && rows16(pr->uhi, pr->m)));
+  ts.mma = ts.sh ? mma_tiles<T>(pr, flags) : 0;
+  ts.mma_wide = ts.mma && mma_rows16<T>(pr);
   ts.lds = (ts.sh ? shared : (size_t)kTrialWaves * trial_wave_elems(pr->n, pr->m)) * sizeof(T);
   ts.grid = dim3((N1 + ts.waves - 1) / ts.waves, pr->batch);
   return ts;
@@ -569,7 +613,6 @@ mpxkkt_status launch_trials_k(hipStream_t st, const TrialShape& ts, const Prob<T
       const int N = P.N;
       const dim3 grid((N + 15) / 16, P.B);
       // 128-bit loads when the iterate and the step are 16-byte aligned too
-      auto al16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
       const bool wide = ts.mma_wide && al16(I.X) && al16(I.U) && al16(I.V) && al16(W.dx) && al16(W.du) &&
                         al16(W.dv);
       auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(64), 0, st, P, R, I, W, j0, j1); };
