@@ -49,6 +49,7 @@ template <typename T>
 struct Ws {
   T* Alin;   // [B][N][n][n]   Jacobians f_x of the tanh model
   T* Blin;   // [B][n][m]      h*B of the tanh model
+  T* sdh;    // [B][N][n]      sech^2(W x_t) of the tanh model (the matrix-core linearisation)
   T* qh;     // [B][N1][n]     augmented state gradients (row N: terminal)
   T* rh;     // [B][N][m]      augmented control gradients
   T* rd;     // [B][N][m]      box Hessian diagonal nu/s

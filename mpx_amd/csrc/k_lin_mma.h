@@ -10,7 +10,8 @@
 // diagonal, the defect c_t = f(x_t, u_t) - x_{t+1}, the merit parts and the
 // box multipliers that are not positive replaced -- and the projection of u_t
 // into the box interior, from the products A x, W x, B u, Q (x - x_ref) and
-// R (u - u_ref) on the matrix cores.  The Jacobian
+// R (u - u_ref) on the matrix cores; and sech^2(W x_t) (k_fwd_tanh's form of
+// the Jacobian).  The Jacobian
 //   f_x = I + h (A + kappa diag(sech^2(W x)) W)
 // is written last by the whole wave, stage by stage: each lane keeps the
 // entries e = 256 k + 4 lane .. + 3 of I + h A and h kappa W (the same for
@@ -252,6 +253,9 @@ k_linearize_mma(Prob<float> P, Prm<float> prm, Iter<float> it, Ws<float> ws) {
       vstore4<WIDE>(rqh, t * n + 16 * i + 4 * g, 16 * i + 4 * g, n, live, qa[i]);
       vstore4<WIDE>(rcd, (t + 1) * n + 16 * i + 4 * g, 16 * i + 4 * g, n, live, C[i]);
     }
+    const rsrc_t rsd = make_rsrc(ws.sdh + (size_t)b * N * n, N * n);
+#pragma unroll
+    for (int i = 0; i < NT; ++i) vstore4<WIDE>(rsd, t * n + 16 * i + 4 * g, 16 * i + 4 * g, n, live, SDv[i]);
     vstore4<WIDE>(rrh, t * m + 4 * g, 4 * g, m, live, ra);
     vstore4<WIDE>(rrd, t * m + 4 * g, 4 * g, m, live, rd);
     cost = col_sum4(cost);

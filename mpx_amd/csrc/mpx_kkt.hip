@@ -22,6 +22,7 @@
 #include "k_trials.h"
 #include "k_trials_mma.h"
 #include "k_lin_mma.h"
+#include "k_fwd_tanh.h"
 #include "k_misc.h"
 #include "k_caller.h"
 #include "k_srbd.h"
@@ -100,6 +101,7 @@ size_t carve(const mpxkkt_problem* pr, int P, bool wave, Ws<T>* ws, char* base) 
   const bool tanh_model = pr->model == MPXKKT_MODEL_TANH;
   ws->Alin = take(tanh_model ? B * N * n * n : 0);
   ws->Blin = take(tanh_model ? B * n * m : 0);
+  ws->sdh = take(tanh_model ? B * N * n : 0);
   ws->qh = take(B * N1 * n);
   ws->rh = take(B * N * m);
   ws->rd = take(B * N * m);
@@ -365,39 +367,6 @@ void fwd_wave(hipStream_t st, dim3 gb, bool cb, bool wide, const Prob<float>& P,
   else hipLaunchKernelGGL((k_fwd_wave<NT, false, false>), gb, dim3(64), 0, st, P, W, in);
 }
 
-// costates, control steps and slope partials: on the wave path the forward
-// pass of the LQR solve with them (k_fwd_wave), else parallel over stages
-// after the dx chain (k_fwd_stage)
-template <typename T>
-mpxkkt_status run_fwd_stage(hipStream_t st, const mpxkkt_problem* pr, const Prob<T>& P, const Prm<T>& R,
-                            const Iter<T>& I, const Ws<T>& W, const LqrIn<T>& in) {
-  if constexpr (IsF32<T>::value) {
-    if (use_wave_lqr<T>(pr, R)) {
-      const int NT = (pr->n + 1 + 15) / 16;
-      const dim3 gc(pr->horizon + 1, pr->batch);
-      const dim3 gb(pr->batch);
-      const bool cb = in.Bm.st == 0;   // B~ held for the whole chain
-      if (NT == 1) {
-        fwd_wave<1>(st, gb, cb, (pr->n & 3) == 0, P, W, in);
-        hipLaunchKernelGGL(k_costate_wave<1>, gc, dim3(64), 0, st, P, I, W);
-      } else if (NT == 2) {
-        fwd_wave<2>(st, gb, cb, (pr->n & 3) == 0, P, W, in);
-        hipLaunchKernelGGL(k_costate_wave<2>, gc, dim3(64), 0, st, P, I, W);
-      } else {
-        fwd_wave<3>(st, gb, cb, (pr->n & 3) == 0, P, W, in);
-        hipLaunchKernelGGL(k_costate_wave<3>, gc, dim3(64), 0, st, P, I, W);
-      }
-      return launch_check("k_fwd_wave / k_costate_wave");
-    }
-  }
-  const int nt = 64;
-  const size_t lds = stage_smem_elems<T>(pr->n, nt) * sizeof(T);
-  mpxkkt_status s = prep_lds(k_fwd_stage<T>, lds);
-  if (s != MPXKKT_OK) return s;
-  hipLaunchKernelGGL(k_fwd_stage<T>, dim3(pr->horizon + 1, pr->batch), dim3(nt), lds, st, P, I, W);
-  return launch_check("k_fwd_stage");
-}
-
 // Rows of a view of row length L start 16-byte aligned (null views pass).
 bool rows16(const mpxkkt_view& v, long long L) {
   return v.ptr == nullptr || ((reinterpret_cast<uintptr_t>(v.ptr) & 15) == 0 && v.batch_stride % 4 == 0 &&
@@ -440,6 +409,62 @@ bool mma_rows16(const mpxkkt_problem* pr) {
 
 inline bool al16(const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; }
 
+// costates, control steps and slope partials: on the wave path the forward
+// pass of the LQR solve (k_fwd_wave, or k_fwd_tanh for the tanh model after
+// k_linearize_mma) and then the costates (k_costate_wave), else parallel over
+// stages after the dx chain (k_fwd_stage)
+template <typename T>
+mpxkkt_status run_fwd_stage(hipStream_t st, const mpxkkt_problem* pr, const Prob<T>& P, const Prm<T>& R,
+                            const Iter<T>& I, const Ws<T>& W, const LqrIn<T>& in) {
+  if constexpr (IsF32<T>::value) {
+    if (use_wave_lqr<T>(pr, R)) {
+      const int NT = (pr->n + 1 + 15) / 16;
+      const dim3 gc(pr->horizon + 1, pr->batch);
+      const dim3 gb(pr->batch);
+      const bool cb = in.Bm.st == 0;   // B~ held for the whole chain
+      // the tanh model after k_linearize_mma (flags bit 3 clear: it ran and wrote
+      // sd_t): the forward chain on the vector ALU without A_t
+      const int nf = (R.flags & 8) == 0 ? mma_tiles<T>(pr, R.flags) : 0;
+      if (nf) {
+        const bool wide = mma_rows16<T>(pr) && al16(W.K) && al16(W.cdef) && al16(W.dx);
+        auto go = [&](auto kern) { hipLaunchKernelGGL(kern, gb, dim3(64), 0, st, P, W); };
+        if (wide) {
+          if (nf == 1) go(k_fwd_tanh<1, true, 4>);
+          else if (nf == 2) go(k_fwd_tanh<2, true, 4>);
+          else go(k_fwd_tanh<3, true, 4>);
+        } else {
+          if (nf == 1) go(k_fwd_tanh<1, false, 4>);
+          else if (nf == 2) go(k_fwd_tanh<2, false, 4>);
+          else go(k_fwd_tanh<3, false, 4>);
+        }
+        mpxkkt_status s = launch_check("k_fwd_tanh");
+        if (s != MPXKKT_OK) return s;
+        if (NT == 1) hipLaunchKernelGGL(k_costate_wave<1>, gc, dim3(64), 0, st, P, I, W);
+        else if (NT == 2) hipLaunchKernelGGL(k_costate_wave<2>, gc, dim3(64), 0, st, P, I, W);
+        else hipLaunchKernelGGL(k_costate_wave<3>, gc, dim3(64), 0, st, P, I, W);
+        return launch_check("k_costate_wave");
+      }
+      if (NT == 1) {
+        fwd_wave<1>(st, gb, cb, (pr->n & 3) == 0, P, W, in);
+        hipLaunchKernelGGL(k_costate_wave<1>, gc, dim3(64), 0, st, P, I, W);
+      } else if (NT == 2) {
+        fwd_wave<2>(st, gb, cb, (pr->n & 3) == 0, P, W, in);
+        hipLaunchKernelGGL(k_costate_wave<2>, gc, dim3(64), 0, st, P, I, W);
+      } else {
+        fwd_wave<3>(st, gb, cb, (pr->n & 3) == 0, P, W, in);
+        hipLaunchKernelGGL(k_costate_wave<3>, gc, dim3(64), 0, st, P, I, W);
+      }
+      return launch_check("k_fwd_wave / k_costate_wave");
+    }
+  }
+  const int nt = 64;
+  const size_t lds = stage_smem_elems<T>(pr->n, nt) * sizeof(T);
+  mpxkkt_status s = prep_lds(k_fwd_stage<T>, lds);
+  if (s != MPXKKT_OK) return s;
+  hipLaunchKernelGGL(k_fwd_stage<T>, dim3(pr->horizon + 1, pr->batch), dim3(nt), lds, st, P, I, W);
+  return launch_check("k_fwd_stage");
+}
+
 // K1: with flags bit 3 parallel over stages with shared constants
 // (k_linearize_stage: lower latency for small batches) when n, m, n_eq <= 64
 // and its LDS plan fits; else the matrix-core form (k_linearize_mma, 16 stages
@@ -470,7 +495,7 @@ mpxkkt_status run_linearize(hipStream_t st, const mpxkkt_problem* pr, const Prob
       // 128-bit loads and stores when the iterate and the outputs are 16-byte aligned too
       const bool wide = mma_rows16<T>(pr) && al16(I.X) && al16(I.U) && al16(I.V) &&
                         (!pr->has_box || al16(I.Ybox)) && al16(W.qh) && al16(W.cdef) && al16(W.rh) &&
-                        al16(W.rd) && al16(W.Alin);
+                        al16(W.rd) && al16(W.Alin) && al16(W.sdh);
       const dim3 grid((pr->horizon + 15) / 16, pr->batch);
       auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(64), 0, st, P, R, I, W); };
       if (wide) {
