@@ -13,7 +13,7 @@
 // R (u - u_ref) on the matrix cores; and sech^2(W x_t) (k_fwd_tanh's form of
 // the Jacobian).  The Jacobian
 //   f_x = I + h (A + kappa diag(sech^2(W x)) W)
-// is written last by the whole wave, stage by stage: each lane keeps the
+// is written last (wfx != 0) by the whole wave, stage by stage: each lane keeps the
 // entries e = 256 k + 4 lane .. + 3 of I + h A and h kappa W (the same for
 // every stage) and reads the stage's sech^2 of row e / n from LDS, so every
 // store is 1 KiB of consecutive floats.  The terminal stage, the initial-state
@@ -42,7 +42,7 @@ __device__ __forceinline__ void vstore4(rsrc_t r, int o, int e0, int lim, bool o
 
 template <int NT, bool WIDE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NT == 1 ? 3 : 2)))
-k_linearize_mma(Prob<float> P, Prm<float> prm, Iter<float> it, Ws<float> ws) {
+k_linearize_mma(Prob<float> P, Prm<float> prm, Iter<float> it, Ws<float> ws, int wfx) {
   constexpr int SD = 16 * NT + 1;   // one stage's sech^2(W x) in LDS
   __shared__ float sdl[16 * SD];
   const int b = blockIdx.y, lane = threadIdx.x, g = lane >> 4, c = lane & 15;
@@ -269,7 +269,8 @@ k_linearize_mma(Prob<float> P, Prm<float> prm, Iter<float> it, Ws<float> ws) {
     }
   }
 
-  // the Jacobians of the wave's stages
+  // the Jacobians of the wave's stages (wfx = 0: the LQR kernels form them)
+  if (!wfx) return;
   __syncthreads();   // (one wave per block) the sech^2 rows in LDS
   const int nn = n * n;
   constexpr int KC = NT * NT;   // ceil((16 NT)^2 / 256) chunks of 256 entries

@@ -340,6 +340,7 @@ struct IlStage {
   f4 B[NT];       // without CQ: B~ tiles (i, 0)
   f4 qrow[NT];    // with CQ: q by rows on the lanes of column n (Q~'s column n)
   float qcol[NT]; // with CQ: q[16 j + pc] (Q~'s row n)
+  f4 sdr[NT];     // with FA: sech^2(W x_t) by rows (A~ is formed where it is used)
 };
 
 // The loads below do no arithmetic on what they load: an instruction that
@@ -386,21 +387,33 @@ __device__ __forceinline__ void il_load_vecs(IlStage<NT>& d, const Prob<float>& 
 
 // The stage's matrices: A~ without its corner (column n = c of the next
 // stage's defect), [D | 0], and q (with CQ) or Q~ and B~.
-template <int NT, bool CQ>
+// FA (the tanh model): sech^2(W x_t) by rows instead of A_t, and column n
+// (c) alone into the last tile column of A~.
+template <int NT, bool CQ, bool FA>
 __device__ __forceinline__ void il_load_mats(IlStage<NT>& d, const Prob<float>& P, const Ws<float>& ws,
                                              const LqrIn<float>& in, int b, int t, int g, bool homlane,
                                              const IlOff<NT>& L) {
   const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
-  const rsrc_t rA = make_rsrc(in.A.at(b, t), n * n);
   const rsrc_t rc = make_rsrc(ws.cdef + ((size_t)b * N1 + t + 1) * n, n);
+  if constexpr (FA) {
+    const rsrc_t rs = make_rsrc(ws.sdh + ((size_t)b * N + t) * n, n);
 #pragma unroll
-  for (int i = 0; i < NT; ++i)
+    for (int i = 0; i < NT; ++i) d.sdr[i] = il_rows(rs, L.g4, i);
+    if (homlane) {
 #pragma unroll
-    for (int j = 0; j < NT; ++j) {
-      f4 x = il_tile_n<NT>(rA, n, i, j, L);
-      if (j == NT - 1 && homlane) x = il_rows(rc, L.g4, i);
-      d.A[i][j] = x;
+      for (int i = 0; i < NT; ++i) d.A[i][NT - 1] = il_rows(rc, L.g4, i);
     }
+  } else {
+    const rsrc_t rA = make_rsrc(in.A.at(b, t), n * n);
+#pragma unroll
+    for (int i = 0; i < NT; ++i)
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        f4 x = il_tile_n<NT>(rA, n, i, j, L);
+        if (j == NT - 1 && homlane) x = il_rows(rc, L.g4, i);
+        d.A[i][j] = x;
+      }
+  }
   const rsrc_t rD = make_rsrc(P.D.at(b, t), p * n);
 #pragma unroll
   for (int j = 0; j < NT; ++j)
@@ -454,20 +467,42 @@ __device__ __forceinline__ void il_store_value(const f4 (&S)[NT][NT], const Ws<f
 // for the whole recursion instead of being re-read every stage.
 // KSN: the k steps of the last state tile, (n + 1 - 16 (NT - 1) + 3) / 4 or
 // more (steps past n + 1 add zeros).
-template <int NT, bool CQ, int KSN>
+template <int NT, bool CQ, int KSN, bool FA>
 __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> prm, Iter<float> it_unused,
                                                     Ws<float> ws, LqrIn<float> in) {
+  static_assert(!FA || CQ, "A~ is formed from stage-invariant A, W only with stage-invariant Q, B, R");
   (void)it_unused;
   const int b = blockIdx.x;
   if (skip_instance(ws, b)) return;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
   const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
   __shared__ float tbuf[16 * 17];
+  // FA: the lane's entries of the tiles of I + h A and h kappa W (rows and
+  // columns past n - 1 are 0), each lane reading back only its own
+  __shared__ f4 fab[FA ? 2 * NT * NT * 64 : 1];
   const float sr = sqrtf(prm.rho_c);
   // k steps over m and over p
   const int ksm = (m + 3) >> 2, ksp = (p + 3) >> 2;
   const int cn = n & 15, pc = 4 * (c & 3) + (c >> 2);
   const IlOff<NT> L = il_offsets<NT>(n, m, g, c);
+  if constexpr (FA) {
+    const rsrc_t rA = make_rsrc(P.A.at(b, 0), n * n), rW = make_rsrc(P.W.at(b, 0), n * n);
+    const float hs = P.h, hk = P.h * P.kappa;
+#pragma unroll
+    for (int i = 0; i < NT; ++i)
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        f4 a = il_tile_n<NT>(rA, n, i, j, L), w = il_tile_n<NT>(rW, n, i, j, L);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = 16 * i + 4 * v + g, col = 16 * j + pc;
+          a[v] = (row == col && row < n ? 1.f : 0.f) + hs * a[v];
+          w[v] = hk * w[v];
+        }
+        fab[(i * NT + j) * 64 + lane] = a;
+        fab[(NT * NT + i * NT + j) * 64 + lane] = w;
+      }
+  }
 
   f4 S[NT][NT];
   il_load_qbar<NT>(S, P.QN.at(b, 0), ws.qh + ((size_t)b * N1 + N) * n, n, false, g, pc == cn, L);
@@ -488,7 +523,7 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
   }
   IlStage<NT> cur;
   il_load_vecs<NT, CQ>(cur, P, ws, b, N - 1, L);
-  il_load_mats<NT, CQ>(cur, P, ws, in, b, N - 1, g, pc == cn, L);
+  il_load_mats<NT, CQ, FA>(cur, P, ws, in, b, N - 1, g, pc == cn, L);
   // S~_N is stored after the first stage's loads are requested, as every
   // stage's stores follow the next stage's loads: the waits at the top of the
   // loop then count the same requests on entry as on the way round
@@ -560,6 +595,19 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
 
     // G~ += B~' S~ A~ ; Pm = Q~ + rho_c [D|0]'[D|0] + A~' S~ A~ (tiles i <= j),
     // a column tile of S~ A~ at a time
+    if constexpr (FA) {   // A~ = I + h A + diag(sd) h kappa W, column n = c on its lanes
+#pragma unroll
+      for (int i = 0; i < NT; ++i)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+          const f4 a = fab[(i * NT + j) * 64 + lane], w = fab[(NT * NT + i * NT + j) * 64 + lane];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const float x = fmaf(cur.sdr[i][v], w[v], a[v]);
+            cur.A[i][j][v] = (j == NT - 1 && homlane) ? cur.A[i][j][v] : x;
+          }
+        }
+    }
 #pragma unroll
     for (int v = 0; v < 4; ++v) cur.A[NT - 1][NT - 1][v] += homlane ? rown[v] : 0.f;
     f4 Pm[NT][NT];
@@ -599,7 +647,7 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
       }
     }
     // A~, D and q (or Q~ and B~) of this stage are dead
-    il_load_mats<NT, CQ>(cur, P, ws, in, b, tp, gf, homlane, Ls);
+    il_load_mats<NT, CQ, FA>(cur, P, ws, in, b, tp, gf, homlane, Ls);
 
     // W~ = Z G~, K~ = Z' W~ (sums over the m rows of G~ and W~);
     // S~ <- Pm - W~' W~ (tiles i <= j, then mirrored)

@@ -409,6 +409,14 @@ bool mma_rows16(const mpxkkt_problem* pr) {
 
 inline bool al16(const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; }
 
+// The tanh model on the wave path after k_linearize_mma (flags bit 3 clear:
+// it ran and wrote sd_t): the LQR kernels form f_x = I + h (A + kappa
+// diag(sd_t) W) where they use it and the linearisation does not write it.
+template <typename T>
+bool fx_in_lqr(const mpxkkt_problem* pr, const Prm<T>& R) {
+  return use_wave_lqr<T>(pr, R) && (R.flags & 8) == 0 && mma_tiles<T>(pr, R.flags) > 0;
+}
+
 // costates, control steps and slope partials: on the wave path the forward
 // pass of the LQR solve (k_fwd_wave, or k_fwd_tanh for the tanh model after
 // k_linearize_mma) and then the costates (k_costate_wave), else parallel over
@@ -424,7 +432,7 @@ mpxkkt_status run_fwd_stage(hipStream_t st, const mpxkkt_problem* pr, const Prob
       const bool cb = in.Bm.st == 0;   // B~ held for the whole chain
       // the tanh model after k_linearize_mma (flags bit 3 clear: it ran and wrote
       // sd_t): the forward chain on the vector ALU without A_t
-      const int nf = (R.flags & 8) == 0 ? mma_tiles<T>(pr, R.flags) : 0;
+      const int nf = fx_in_lqr<T>(pr, R) ? mma_tiles<T>(pr, R.flags) : 0;
       if (nf) {
         const bool wide = mma_rows16<T>(pr) && al16(W.K) && al16(W.cdef) && al16(W.dx);
         auto go = [&](auto kern) { hipLaunchKernelGGL(kern, gb, dim3(64), 0, st, P, W); };
@@ -497,7 +505,9 @@ mpxkkt_status run_linearize(hipStream_t st, const mpxkkt_problem* pr, const Prob
                         (!pr->has_box || al16(I.Ybox)) && al16(W.qh) && al16(W.cdef) && al16(W.rh) &&
                         al16(W.rd) && al16(W.Alin) && al16(W.sdh);
       const dim3 grid((pr->horizon + 15) / 16, pr->batch);
-      auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(64), 0, st, P, R, I, W); };
+      // f_x only for the LQR kernels that read it
+      const int wfx = fx_in_lqr<T>(pr, R) ? 0 : 1;
+      auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(64), 0, st, P, R, I, W, wfx); };
       if (wide) {
         if (nt == 1) go(k_linearize_mma<1, true>);
         else if (nt == 2) go(k_linearize_mma<2, true>);
@@ -548,24 +558,30 @@ mpxkkt_status run_lqr_wave(hipStream_t st, const mpxkkt_problem* pr, const Prob<
     auto go = [&](auto kern) { hipLaunchKernelGGL(kern, gb, dim3(64), 0, st, P, R, I, W, in); };
     if (cq) {
       // one instantiation per step count: the steps are unrolled matrix instructions
-      switch (NT * 4 + ksn) {
-        case 5: go(k_lqr_wave<1, true, 1>); break;
-        case 6: go(k_lqr_wave<1, true, 2>); break;
-        case 7: go(k_lqr_wave<1, true, 3>); break;
-        case 8: go(k_lqr_wave<1, true, 4>); break;
-        case 9: go(k_lqr_wave<2, true, 1>); break;
-        case 10: go(k_lqr_wave<2, true, 2>); break;
-        case 11: go(k_lqr_wave<2, true, 3>); break;
-        case 12: go(k_lqr_wave<2, true, 4>); break;
-        case 13: go(k_lqr_wave<3, true, 1>); break;
-        case 14: go(k_lqr_wave<3, true, 2>); break;
-        case 15: go(k_lqr_wave<3, true, 3>); break;
-        default: go(k_lqr_wave<3, true, 4>); break;
-      }
+      auto cases = [&](auto FA) {
+        constexpr bool fa = decltype(FA)::value;
+        switch (NT * 4 + ksn) {
+          case 5: go(k_lqr_wave<1, true, 1, fa>); break;
+          case 6: go(k_lqr_wave<1, true, 2, fa>); break;
+          case 7: go(k_lqr_wave<1, true, 3, fa>); break;
+          case 8: go(k_lqr_wave<1, true, 4, fa>); break;
+          case 9: go(k_lqr_wave<2, true, 1, fa>); break;
+          case 10: go(k_lqr_wave<2, true, 2, fa>); break;
+          case 11: go(k_lqr_wave<2, true, 3, fa>); break;
+          case 12: go(k_lqr_wave<2, true, 4, fa>); break;
+          case 13: go(k_lqr_wave<3, true, 1, fa>); break;
+          case 14: go(k_lqr_wave<3, true, 2, fa>); break;
+          case 15: go(k_lqr_wave<3, true, 3, fa>); break;
+          default: go(k_lqr_wave<3, true, 4, fa>); break;
+        }
+      };
+      // the tanh model after k_linearize_mma: A~ formed from A, W and sd_t
+      if (fx_in_lqr<T>(pr, R)) cases(std::true_type{});
+      else cases(std::false_type{});
     } else {   // stage-varying data: the generic step count (the steps past n + 1 add zeros)
-      if (NT == 1) go(k_lqr_wave<1, false, 4>);
-      else if (NT == 2) go(k_lqr_wave<2, false, 4>);
-      else go(k_lqr_wave<3, false, 4>);
+      if (NT == 1) go(k_lqr_wave<1, false, 4, false>);
+      else if (NT == 2) go(k_lqr_wave<2, false, 4, false>);
+      else go(k_lqr_wave<3, false, 4, false>);
     }
     return launch_check("k_lqr_wave");
   } else {
