@@ -8,7 +8,7 @@
 // line search.  Here the caller does that between mpxkkt_sqp_direction and
 // mpxkkt_sqp_accept; the merit terms that are the library's own (V'c,
 // |c|^2, the equality rows' augmented-Lagrangian terms, the box barrier) are
-// added on the device, stage by stage, in the layout k_ls_accept reduces.
+// added on the device, stage by stage, in the layout k_ls_decide reduces.
 #pragma once
 #include "common.h"
 #include "k_trials.h"
@@ -20,7 +20,7 @@ namespace mpxk {
 //          c(a) = trial_f[b][j][t] - (X_{t+1} + a dX_{t+1});
 //   t = N: trial_g[b][j][N] + V_0(a)'c_init(a), c_init(a) = x0 - (X_0 + a dX_0);
 // and |c(a)|^2, into tpart; the equality rows at the iterate and their slopes
-// into hq (k_ls_accept's multiplier update reads them).
+// into hq (k_ls_update's multiplier update reads them).
 template <typename T>
 __global__ void __launch_bounds__(256) k_trials_caller(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, int L,
                                                        const T* __restrict__ trial_g,

@@ -33,8 +33,10 @@
 // (k_fwd_wave) runs on the matrix cores too, with x~ as column 0 of a
 // B-operand tile: every product's result tile is again in that layout, so
 // x~_{t+1} = A~ x~ + B~ du and du = -K~ x~ chain from registers and
-// F = A - B K is never formed; the costates lam = S~ x~ follow in parallel over
-// the stages (k_costate_wave).
+// F = A - B K is never formed (for the tanh model k_fwd_tanh, k_fwd_tanh.h,
+// runs the chain on the vector ALU); the costates lam = S x + s follow in
+// parallel over the stages from the stored upper tiles of S (k_costate_rows,
+// k_costate.h).
 //
 // Memory goes through raw buffer descriptors, one per matrix and stage, whose
 // record count ends at the matrix's last row.  The lane's byte offset of
@@ -448,6 +450,8 @@ __device__ __forceinline__ void il_store_hom(const f4& x, rsrc_t rM, rsrc_t rvec
   }
 }
 
+// S~_t: the tiles (it, jt), it <= jt, of the symmetric S_t (the lower ones
+// are read as their transposes) and s_t (column n, in the last tile column)
 template <int NT>
 __device__ __forceinline__ void il_store_value(const f4 (&S)[NT][NT], const Ws<float>& ws, int b, int t, int N1,
                                                int n, const IlOff<NT>& L) {
@@ -456,7 +460,7 @@ __device__ __forceinline__ void il_store_value(const f4 (&S)[NT][NT], const Ws<f
 #pragma unroll
   for (int it = 0; it < NT; ++it)
 #pragma unroll
-    for (int jt = 0; jt < NT; ++jt) il_store_hom<NT>(S[it][jt], rS, rs, n, it, jt, L);
+    for (int jt = it; jt < NT; ++jt) il_store_hom<NT>(S[it][jt], rS, rs, n, it, jt, L);
 }
 
 // segments == 1, FP32, n + 1 <= 16 NT (NT minimal: column n lies in the last
@@ -909,82 +913,6 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3)))
     for (int k = 0; k < NT; ++k)
 #pragma unroll
       for (int v = 0; v < 4; ++v) rstore(rdx, L.col0 + (16 * k + v) * 4, x[k][v]);
-  }
-}
-
-// Costates and slope partials, one wave per (stage t, instance b):
-//   lam_t = (S~_t x~_t)[:n] on the matrix cores (x~_t = [dx_t; 1] as column 0 of
-//   B-operand tiles), dv_t = lam_t - V_t, and the partials q_t'dx_t + r_t'du_t,
-//   (dv_t - V_t)'c_t, lam_t'c_t of k_fwd_stage (c_t is the constraint row paired
-//   with V_t; c_0 = x0 - X_0).
-template <int NT>
-__global__ void __launch_bounds__(64) k_costate_wave(Prob<float> P, Iter<float> it, Ws<float> ws) {
-  const int t = blockIdx.x, b = blockIdx.y;
-  if (skip_instance(ws, b)) return;
-  const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
-  const int n = P.n, m = P.m, N = P.N, N1 = N + 1;
-  const LaneOff<NT> L = lane_offsets<NT>(n, m, g, c);
-  int off[NT][4];
-  aop_offsets<NT>(off, n, g, c);
-  f4 Sa[NT][NT];
-  load_aop<NT, NT>(Sa, ws.S + ((size_t)b * N1 + t) * n * n, ws.s + ((size_t)b * N1 + t) * n, n, n, false, g, c,
-                   off, L);
-  f4 x[NT], qv[NT], vv[NT], cv[NT];
-  const rsrc_t rx = make_rsrc(ws.dx + ((size_t)b * N1 + t) * n, n);
-  const rsrc_t rq = make_rsrc(ws.qh + ((size_t)b * N1 + t) * n, n);
-  const rsrc_t rV = make_rsrc(it.V + ((size_t)b * N1 + t) * n, n);
-  const rsrc_t rc = make_rsrc(ws.cdef + ((size_t)b * N1 + t) * n, n);
-#pragma unroll
-  for (int k = 0; k < NT; ++k) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-      x[k][v] = rload(rx, L.col0 + (16 * k + v) * 4) + ((c == 0 && 16 * k + 4 * g + v == n) ? 1.f : 0.f);
-    qv[k] = rows_vec<NT>(rq, k, L);
-    vv[k] = rows_vec<NT>(rV, k, L);
-    cv[k] = rows_vec<NT>(rc, k, L);
-  }
-  float df = 0.f, dc = 0.f, lc = 0.f;
-  const bool col0 = c == 0;
-  if (t < N) {   // r_t'du_t
-    const rsrc_t rr = make_rsrc(ws.rh + ((size_t)b * N + t) * m, m);
-    const rsrc_t ru = make_rsrc(ws.du + ((size_t)b * N + t) * m, m);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const float rv = rload(rr, (4 * g + v) * 4), uv = rload(ru, (4 * g + v) * 4);
-      df += col0 ? rv * uv : 0.f;
-    }
-  }
-  f4 lam[NT];
-#pragma unroll
-  for (int i = 0; i < NT; ++i) {
-    f4 acc = zero4();
-#pragma unroll
-    for (int k = 0; k < NT; ++k)
-#pragma unroll
-      for (int s = 0; s < 4; ++s) acc = mma(Sa[i][k][s], x[k][s], acc);
-    lam[i] = acc;
-  }
-  const rsrc_t rdv = make_rsrc(ws.dv + ((size_t)b * N1 + t) * n, n);
-#pragma unroll
-  for (int k = 0; k < NT; ++k)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const bool row_ok = col0 && 16 * k + 4 * g + v < n;
-      const float dv = lam[k][v] - vv[k][v];
-      df += row_ok ? qv[k][v] * x[k][v] : 0.f;
-      dc += row_ok ? (dv - vv[k][v]) * cv[k][v] : 0.f;
-      lc += row_ok ? lam[k][v] * cv[k][v] : 0.f;
-      rstore(rdv, L.col0 + (16 * k + v) * 4, dv);
-    }
-  df = wave_sum(df);
-  dc = wave_sum(dc);
-  lc = wave_sum(lc);
-  if (lane == 0) {
-    float* fp = ws.fpart + ((size_t)b * N1 + t) * NFP;
-    fp[FP_DF] = df;
-    fp[FP_DC] = dc;
-    fp[FP_LC] = lc;
-    fp[3] = 0.f;
   }
 }
 

@@ -6,9 +6,10 @@
 //                boundary steps of the box slacks and multipliers;
 //   k_trials     one wave per stage: that stage's part of phi(a_j) for the
 //                trials a_j = a_max beta^j, j0 <= j < j1;
-//   k_ls_accept  one wave per stage: the first a_j that passes the Armijo
-//                test (the same choice in every wave), then the multiplier
-//                and iterate update of that stage.
+//   k_ls_decide  one wave per instance: the first a_j that passes the Armijo
+//                test (the stage parts summed in a fixed order);
+//   k_ls_update  one thread per entry: the multiplier and iterate update at
+//                the accepted step.
 //
 // At a trial point x = X + a dX, u = U + a dU every matrix product of the
 // merit is affine in a: A x = A X + a A dX, W x, B u, Q (x - xr), R (u - ur),
@@ -33,7 +34,7 @@
 
 namespace mpxk {
 
-enum { LS_RHO = 0, LS_PHI0 = 1, LS_DPHI = 2, LS_AMAX = 3, LS_ADUAL = 4, LS_SLACK = 5, NLS = 8 };
+enum { LS_RHO = 0, LS_PHI0 = 1, LS_DPHI = 2, LS_AMAX = 3, LS_ADUAL = 4, LS_SLACK = 5, LS_ALPHA = 6, NLS = 8 };
 constexpr int kTrialWaves = 4;
 
 // Elements of one wave's LDS vectors in k_trials: x0 = X_t, x1 = dX_t,
@@ -413,16 +414,16 @@ __global__ void __launch_bounds__(512) k_trials(Prob<T> P, Prm<T> prm, Iter<T> i
   }
 }
 
-// Accepted step, multiplier and iterate update of stage t; block 0 reports.
+// The accepted step of instance blockIdx.x (one wave): the first trial that
+// passes the Armijo test, else the last one (status 2); the trial count
+// reported is that of backtracking in groups of prm.group.  Writes the step
+// to lsp[LS_ALPHA] for k_ls_update and the instance's report.
 template <typename T>
-__global__ void __launch_bounds__(256) k_ls_accept(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, T* o_alpha,
-                                                   T* o_rho, T* o_phi0, T* o_dphi, int* o_status, int* o_trials) {
-  const int wpb = blockDim.x >> 6, wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int b = blockIdx.y, t = blockIdx.x * wpb + wv;
+__global__ void __launch_bounds__(64) k_ls_decide(Prm<T> prm, Ws<T> ws, int N1, T* o_alpha, T* o_rho, T* o_phi0,
+                                                  T* o_dphi, int* o_status, int* o_trials) {
+  const int b = blockIdx.x, lane = threadIdx.x;
   if (skip_instance(ws, b)) return;
-  const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
-  if (t > N) return;
-  const T* ls = ws.lsp + (size_t)b * NLS;
+  T* ls = ws.lsp + (size_t)b * NLS;
   const int MT = prm.max_trials, g = prm.group;
   const int js = ls_first_accept(prm, ws, b, N1, MT);
   T alpha = ls[LS_AMAX];
@@ -437,33 +438,8 @@ __global__ void __launch_bounds__(256) k_ls_accept(Prob<T> P, Prm<T> prm, Iter<T
   }
   const T rho = ls[LS_RHO], phi0 = ls[LS_PHI0], dphi = ls[LS_DPHI];
   if (!isfinite(phi0) || !isfinite(dphi)) status = 1;
-  if (t < N) {
-    const size_t ou = ((size_t)b * N + t) * m;
-    for (int i = lane; i < m; i += 64) {
-      const T u = it.U[ou + i], du = ws.du[ou + i];
-      if (P.box) {
-        const T adual = ls[LS_ADUAL];
-        const T lo = P.ulo.at(b, t)[i], hi = P.uhi.at(b, t)[i];
-        const T shi = hi - u, slo = u - lo;
-        T* nu = it.Ybox + ((size_t)b * N + t) * 2 * m;
-        const T nhi = nu[i], nlo = nu[m + i];
-        nu[i] = nhi + adual * (prm.mu_b / shi - nhi + nhi * du / shi);
-        nu[m + i] = nlo + adual * (prm.mu_b / slo - nlo - nlo * du / slo);
-      }
-      it.U[ou + i] = u + alpha * du;
-    }
-    if (p > 0) {
-      const T* hq = ws.hq + ((size_t)b * N + t) * 2 * p;
-      T* y = it.Yeq + ((size_t)b * N + t) * p;
-      for (int k = lane; k < p; k += 64) y[k] += prm.rho_c * (hq[k] + alpha * hq[p + k]);
-    }
-  }
-  const size_t ox = ((size_t)b * N1 + t) * n;
-  for (int i = lane; i < n; i += 64) {
-    it.X[ox + i] += alpha * ws.dx[ox + i];
-    it.V[ox + i] += alpha * ws.dv[ox + i];
-  }
-  if (t == 0 && lane == 0) {
+  if (lane == 0) {
+    ls[LS_ALPHA] = alpha;
     if (o_alpha) o_alpha[b] = alpha;
     if (o_rho) o_rho[b] = rho;
     if (o_phi0) o_phi0[b] = phi0;
@@ -471,6 +447,59 @@ __global__ void __launch_bounds__(256) k_ls_accept(Prob<T> P, Prm<T> prm, Iter<T
     if (o_status) o_status[b] = status;
     if (o_trials) o_trials[b] = trials;
   }
+}
+
+// Iterate and multiplier update at the accepted step a = lsp[LS_ALPHA], one
+// thread per entry: blocks [0, nbx) take X += a dX and V += a dV, the next
+// nbu blocks U += a dU with the box multipliers' update (at u before the
+// step), the rest Y_eq += rho_c (h + a dh) from the equality rows and slopes
+// the trials stored.
+template <typename T>
+__global__ void __launch_bounds__(256) k_ls_update(Prob<T> P, Prm<T> prm, Iter<T> it, Ws<T> ws, int nbx, int nbu) {
+  const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
+  const size_t B = (size_t)P.B;
+  int blk = blockIdx.x;
+  if (blk < nbx) {
+    const size_t e = (size_t)blk * 256 + threadIdx.x;
+    if (e >= B * N1 * n) return;
+    const int b = (int)(e / ((size_t)N1 * n));
+    if (skip_instance(ws, b)) return;
+    const T a = ws.lsp[(size_t)b * NLS + LS_ALPHA];
+    it.X[e] += a * ws.dx[e];
+    it.V[e] += a * ws.dv[e];
+    return;
+  }
+  blk -= nbx;
+  if (blk < nbu) {
+    const size_t e = (size_t)blk * 256 + threadIdx.x;
+    if (e >= B * N * m) return;
+    const int b = (int)(e / ((size_t)N * m));
+    if (skip_instance(ws, b)) return;
+    const int r = (int)(e - (size_t)b * N * m), t = r / m, i = r - t * m;
+    const T* ls = ws.lsp + (size_t)b * NLS;
+    const T a = ls[LS_ALPHA];
+    const T u = it.U[e], du = ws.du[e];
+    if (P.box) {
+      const T adual = ls[LS_ADUAL];
+      const T lo = P.ulo.at(b, t)[i], hi = P.uhi.at(b, t)[i];
+      const T shi = hi - u, slo = u - lo;
+      T* nu = it.Ybox + ((size_t)b * N + t) * 2 * m;
+      const T nhi = nu[i], nlo = nu[m + i];
+      nu[i] = nhi + adual * (prm.mu_b / shi - nhi + nhi * du / shi);
+      nu[m + i] = nlo + adual * (prm.mu_b / slo - nlo - nlo * du / slo);
+    }
+    it.U[e] = u + a * du;
+    return;
+  }
+  blk -= nbu;
+  const size_t e = (size_t)blk * 256 + threadIdx.x;
+  if (p == 0 || e >= B * N * p) return;
+  const int b = (int)(e / ((size_t)N * p));
+  if (skip_instance(ws, b)) return;
+  const int r = (int)(e - (size_t)b * N * p), t = r / p, k = r - t * p;
+  const T a = ws.lsp[(size_t)b * NLS + LS_ALPHA];
+  const T* hq = ws.hq + ((size_t)b * N + t) * 2 * p;
+  it.Yeq[e] += prm.rho_c * (hq[k] + a * hq[p + k]);
 }
 
 }  // namespace mpxk

@@ -23,6 +23,7 @@
 #include "k_trials_mma.h"
 #include "k_lin_mma.h"
 #include "k_fwd_tanh.h"
+#include "k_costate.h"
 #include "k_misc.h"
 #include "k_caller.h"
 #include "k_srbd.h"
@@ -419,7 +420,7 @@ bool fx_in_lqr(const mpxkkt_problem* pr, const Prm<T>& R) {
 
 // costates, control steps and slope partials: on the wave path the forward
 // pass of the LQR solve (k_fwd_wave, or k_fwd_tanh for the tanh model after
-// k_linearize_mma) and then the costates (k_costate_wave), else parallel over
+// k_linearize_mma) and then the costates (k_costate_rows), else parallel over
 // stages after the dx chain (k_fwd_stage)
 template <typename T>
 mpxkkt_status run_fwd_stage(hipStream_t st, const mpxkkt_problem* pr, const Prob<T>& P, const Prm<T>& R,
@@ -447,22 +448,27 @@ mpxkkt_status run_fwd_stage(hipStream_t st, const mpxkkt_problem* pr, const Prob
         }
         mpxkkt_status s = launch_check("k_fwd_tanh");
         if (s != MPXKKT_OK) return s;
-        if (NT == 1) hipLaunchKernelGGL(k_costate_wave<1>, gc, dim3(64), 0, st, P, I, W);
-        else if (NT == 2) hipLaunchKernelGGL(k_costate_wave<2>, gc, dim3(64), 0, st, P, I, W);
-        else hipLaunchKernelGGL(k_costate_wave<3>, gc, dim3(64), 0, st, P, I, W);
-        return launch_check("k_costate_wave");
-      }
-      if (NT == 1) {
-        fwd_wave<1>(st, gb, cb, (pr->n & 3) == 0, P, W, in);
-        hipLaunchKernelGGL(k_costate_wave<1>, gc, dim3(64), 0, st, P, I, W);
-      } else if (NT == 2) {
-        fwd_wave<2>(st, gb, cb, (pr->n & 3) == 0, P, W, in);
-        hipLaunchKernelGGL(k_costate_wave<2>, gc, dim3(64), 0, st, P, I, W);
       } else {
-        fwd_wave<3>(st, gb, cb, (pr->n & 3) == 0, P, W, in);
-        hipLaunchKernelGGL(k_costate_wave<3>, gc, dim3(64), 0, st, P, I, W);
+        if (NT == 1) fwd_wave<1>(st, gb, cb, (pr->n & 3) == 0, P, W, in);
+        else if (NT == 2) fwd_wave<2>(st, gb, cb, (pr->n & 3) == 0, P, W, in);
+        else fwd_wave<3>(st, gb, cb, (pr->n & 3) == 0, P, W, in);
+        mpxkkt_status s = launch_check("k_fwd_wave");
+        if (s != MPXKKT_OK) return s;
       }
-      return launch_check("k_fwd_wave / k_costate_wave");
+      // the costates: the stored tiles of S_t without the homogeneous coordinate
+      const int nc = (pr->n + 15) / 16;
+      const bool cw = (pr->n & 3) == 0 && al16(W.S) && al16(W.dx) && al16(W.qh);
+      auto gocs = [&](auto kern) { hipLaunchKernelGGL(kern, gc, dim3(64), 0, st, P, I, W); };
+      if (cw) {
+        if (nc == 1) gocs(k_costate_rows<1, true>);
+        else if (nc == 2) gocs(k_costate_rows<2, true>);
+        else gocs(k_costate_rows<3, true>);
+      } else {
+        if (nc == 1) gocs(k_costate_rows<1, false>);
+        else if (nc == 2) gocs(k_costate_rows<2, false>);
+        else gocs(k_costate_rows<3, false>);
+      }
+      return launch_check("k_costate_rows");
     }
   }
   const int nt = 64;
@@ -698,8 +704,29 @@ mpxkkt_status run_trials(hipStream_t st, const mpxkkt_problem* pr, const TrialSh
   return launch_trials<T, 8>(st, ts, P, R, I, W, j0, j1);
 }
 
+// The acceptance: k_ls_decide (the step, once per instance), then k_ls_update
+// (the iterate and multipliers, one thread per entry).
+template <typename T>
+mpxkkt_status ls_accept(hipStream_t st, const mpxkkt_problem* pr, const Prob<T>& P, const Prm<T>& R,
+                        const Iter<T>& I, const Ws<T>& W, const mpxkkt_stats* so) {
+  const size_t B = pr->batch, n = pr->n, m = pr->m, p = pr->n_eq, N = pr->horizon, N1 = N + 1;
+  T* a = so ? static_cast<T*>(so->alpha) : nullptr;
+  T* r = so ? static_cast<T*>(so->rho) : nullptr;
+  T* f0 = so ? static_cast<T*>(so->merit0) : nullptr;
+  T* df = so ? static_cast<T*>(so->dmerit) : nullptr;
+  int* stt = so ? so->status : nullptr;
+  int* tr = so ? so->trials : nullptr;
+  hipLaunchKernelGGL(k_ls_decide<T>, dim3((unsigned)B), dim3(64), 0, st, R, W, (int)N1, a, r, f0, df, stt, tr);
+  mpxkkt_status s = launch_check("k_ls_decide");
+  if (s != MPXKKT_OK) return s;
+  const int nbx = (int)((B * N1 * n + 255) / 256), nbu = (int)((B * N * m + 255) / 256);
+  const int nbp = (int)((B * N * p + 255) / 256);
+  hipLaunchKernelGGL(k_ls_update<T>, dim3(nbx + nbu + nbp), dim3(256), 0, st, P, R, I, W, nbx, nbu);
+  return launch_check("k_ls_update");
+}
+
 // K4: k_ls_setup, k_trials over [0, group) and then [group, max_trials) for the
-// instances that accepted none of the first group, k_ls_accept.  flags bit 2
+// instances that accepted none of the first group, k_ls_decide and k_ls_update.  flags bit 2
 // selects the single-kernel line search of k_inst.h / k_linesearch.h instead.
 template <typename T>
 mpxkkt_status run_linesearch(hipStream_t st, const mpxkkt_problem* pr, const Prob<T>& P, const Prm<T>& R,
@@ -714,15 +741,7 @@ mpxkkt_status run_linesearch(hipStream_t st, const mpxkkt_problem* pr, const Pro
     if ((s = run_trials<T>(st, pr, ts, P, R, I, W, 0, R.group)) != MPXKKT_OK) return s;
     if (R.group < R.max_trials && (s = run_trials<T>(st, pr, ts, P, R, I, W, R.group, R.max_trials)) != MPXKKT_OK)
       return s;
-    const dim3 grid((N1 + kTrialWaves - 1) / kTrialWaves, B);
-    T* a = so ? static_cast<T*>(so->alpha) : nullptr;
-    T* r = so ? static_cast<T*>(so->rho) : nullptr;
-    T* f0 = so ? static_cast<T*>(so->merit0) : nullptr;
-    T* df = so ? static_cast<T*>(so->dmerit) : nullptr;
-    int* stt = so ? so->status : nullptr;
-    int* tr = so ? so->trials : nullptr;
-    hipLaunchKernelGGL(k_ls_accept<T>, grid, dim3(64 * kTrialWaves), 0, st, P, R, I, W, a, r, f0, df, stt, tr);
-    return launch_check("k_ls_accept");
+    return ls_accept<T>(st, pr, P, R, I, W, so);
   }
   if (use_inst<T>(pr, true)) {
     const InstLds L = inst_lds(pr->n, pr->m, pr->horizon, pr->model == MPXKKT_MODEL_TANH, pr->Q.stage_stride == 0,
@@ -873,14 +892,7 @@ mpxkkt_status accept_t(hipStream_t st, const mpxkkt_problem* pr, const mpxkkt_pa
                      static_cast<const T*>(tg), static_cast<const T*>(tf));
   mpxkkt_status s = launch_check("k_trials_caller");
   if (s != MPXKKT_OK) return s;
-  T* a = so ? static_cast<T*>(so->alpha) : nullptr;
-  T* r = so ? static_cast<T*>(so->rho) : nullptr;
-  T* f0 = so ? static_cast<T*>(so->merit0) : nullptr;
-  T* df = so ? static_cast<T*>(so->dmerit) : nullptr;
-  int* stt = so ? so->status : nullptr;
-  int* tr = so ? so->trials : nullptr;
-  hipLaunchKernelGGL(k_ls_accept<T>, grid, dim3(64 * kTrialWaves), 0, st, P, R, I, W, a, r, f0, df, stt, tr);
-  return launch_check("k_ls_accept");
+  return ls_accept<T>(st, pr, P, R, I, W, so);
 }
 
 template <typename T>

@@ -196,7 +196,7 @@ def test_fp32_wave_kernel_step_counts(shape):
     (4, dict(n=76, m=32, N=5, p=12), 3, "zero"),     # two state rows per lane (FP32: FP64 LQR LDS too large)
 ])
 def test_stage_parallel_line_search(wl, shape, iters, start, group, dtype):
-    """The stage-parallel line search (k_ls_setup / k_trials / k_ls_accept,
+    """The stage-parallel line search (k_ls_setup / k_trials / k_ls_decide / k_ls_update,
     the default) against the single-kernel line search (flags bit 2) on the
     same inputs, and both against the oracle: the same accepted step and trial
     count, iterates equal to rounding."""
