@@ -531,7 +531,7 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
   // S~_N is stored after the first stage's loads are requested, as every
   // stage's stores follow the next stage's loads: the waits at the top of the
   // loop then count the same requests on entry as on the way round
-  il_store_value<NT>(S, ws, b, N, N1, n, L);
+  if constexpr (!FA) il_store_value<NT>(S, ws, b, N, N1, n, L);   // (FA: the costates come from k_adjoint_tanh)
   for (int t = N - 1; t >= 0; --t) {
     // lane offsets from opaque copies of the lane coordinates: recomputed every
     // stage instead of held across the loop
@@ -685,8 +685,8 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
     for (int v = 0; v < 4; ++v)
       if (homlane && 4 * v + gf == cn) S[NT - 1][NT - 1][v] = 0.f;
 
-    // stores: S_t, s_t ; K_t, k_t
-    il_store_value<NT>(S, ws, b, t, N1, n, Ls);
+    // stores: S_t, s_t (not with FA) ; K_t, k_t
+    if constexpr (!FA) il_store_value<NT>(S, ws, b, t, N1, n, Ls);
     {
       const rsrc_t rK = make_rsrc(ws.K + ((size_t)b * N + t) * m * n, m * n);
       const rsrc_t rk = make_rsrc(ws.k + ((size_t)b * N + t) * m, m);

@@ -24,6 +24,7 @@
 #include "k_lin_mma.h"
 #include "k_fwd_tanh.h"
 #include "k_costate.h"
+#include "k_adjoint.h"
 #include "k_misc.h"
 #include "k_caller.h"
 #include "k_srbd.h"
@@ -419,16 +420,16 @@ bool fx_in_lqr(const mpxkkt_problem* pr, const Prm<T>& R) {
 }
 
 // costates, control steps and slope partials: on the wave path the forward
-// pass of the LQR solve (k_fwd_wave, or k_fwd_tanh for the tanh model after
-// k_linearize_mma) and then the costates (k_costate_rows), else parallel over
-// stages after the dx chain (k_fwd_stage)
+// pass of the LQR solve and the costates -- for the tanh model after
+// k_linearize_mma k_fwd_tanh and the adjoint sweep k_adjoint_tanh, else
+// k_fwd_wave and k_costate_rows (lam = S x + s from the stored S) --, else
+// parallel over stages after the dx chain (k_fwd_stage)
 template <typename T>
 mpxkkt_status run_fwd_stage(hipStream_t st, const mpxkkt_problem* pr, const Prob<T>& P, const Prm<T>& R,
                             const Iter<T>& I, const Ws<T>& W, const LqrIn<T>& in) {
   if constexpr (IsF32<T>::value) {
     if (use_wave_lqr<T>(pr, R)) {
       const int NT = (pr->n + 1 + 15) / 16;
-      const dim3 gc(pr->horizon + 1, pr->batch);
       const dim3 gb(pr->batch);
       const bool cb = in.Bm.st == 0;   // B~ held for the whole chain
       // the tanh model after k_linearize_mma (flags bit 3 clear: it ran and wrote
@@ -448,6 +449,18 @@ mpxkkt_status run_fwd_stage(hipStream_t st, const mpxkkt_problem* pr, const Prob
         }
         mpxkkt_status s = launch_check("k_fwd_tanh");
         if (s != MPXKKT_OK) return s;
+        // the costates by the adjoint recursion (k_lqr_wave stored no S_t)
+        auto goa = [&](auto kern) { hipLaunchKernelGGL(kern, gb, dim3(64), 0, st, P, R, I, W); };
+        if (wide) {
+          if (nf == 1) goa(k_adjoint_tanh<1, true>);
+          else if (nf == 2) goa(k_adjoint_tanh<2, true>);
+          else goa(k_adjoint_tanh<3, true>);
+        } else {
+          if (nf == 1) goa(k_adjoint_tanh<1, false>);
+          else if (nf == 2) goa(k_adjoint_tanh<2, false>);
+          else goa(k_adjoint_tanh<3, false>);
+        }
+        return launch_check("k_adjoint_tanh");
       } else {
         if (NT == 1) fwd_wave<1>(st, gb, cb, (pr->n & 3) == 0, P, W, in);
         else if (NT == 2) fwd_wave<2>(st, gb, cb, (pr->n & 3) == 0, P, W, in);
@@ -458,15 +471,17 @@ mpxkkt_status run_fwd_stage(hipStream_t st, const mpxkkt_problem* pr, const Prob
       // the costates: the stored tiles of S_t without the homogeneous coordinate
       const int nc = (pr->n + 15) / 16;
       const bool cw = (pr->n & 3) == 0 && al16(W.S) && al16(W.dx) && al16(W.qh);
-      auto gocs = [&](auto kern) { hipLaunchKernelGGL(kern, gc, dim3(64), 0, st, P, I, W); };
+      constexpr int spw = kCostateStages;
+      const dim3 gcs((pr->horizon + 1 + spw - 1) / spw, pr->batch);
+      auto gocs = [&](auto kern) { hipLaunchKernelGGL(kern, gcs, dim3(64), 0, st, P, I, W); };
       if (cw) {
-        if (nc == 1) gocs(k_costate_rows<1, true>);
-        else if (nc == 2) gocs(k_costate_rows<2, true>);
-        else gocs(k_costate_rows<3, true>);
+        if (nc == 1) gocs(k_costate_rows<1, true, spw>);
+        else if (nc == 2) gocs(k_costate_rows<2, true, spw>);
+        else gocs(k_costate_rows<3, true, spw>);
       } else {
-        if (nc == 1) gocs(k_costate_rows<1, false>);
-        else if (nc == 2) gocs(k_costate_rows<2, false>);
-        else gocs(k_costate_rows<3, false>);
+        if (nc == 1) gocs(k_costate_rows<1, false, spw>);
+        else if (nc == 2) gocs(k_costate_rows<2, false, spw>);
+        else gocs(k_costate_rows<3, false, spw>);
       }
       return launch_check("k_costate_rows");
     }
