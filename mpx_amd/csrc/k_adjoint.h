@@ -11,13 +11,14 @@
 // (c_t the constraint row paired with V_t; c_0 = x0 - X_0).
 //
 // Lane (g, c) holds rows 16 i + c, columns 16 k + 4 g + s of Q (registers),
-// (I + h A)' and (h kappa W)' (LDS) for the whole sweep, lam_{t+1}, sd_t * lam_{t+1} and dx_t by
-// the entries 16 k + 4 g + s, and D_t' (rows 16 i + c, columns 4 g + s < p).
+// (I + h A)' and (h kappa W)' (LDS) for the whole sweep, lam_{t+1},
+// sd_t * lam_{t+1} and dx_t by the entries 16 k + 4 g + s (gathered from the
+// rows by ds_bpermute), and D_t' (rows 16 i + c, columns 4 g + s < p).
 // D dx + E du is a row sum over the 16 lanes (every lane of row g then holds
 // the entries 4 g + s); each lam_t row is 4 NT^2 products per matrix per lane
 // and a sum over the four lanes of a column; lam_t's rows are gathered back
-// into the column layout by ds_bpermute.  Stage t - 1's data is requested
-// before stage t's arithmetic (two register buffers).
+// into the column layout by ds_bpermute.  Each stage's data is requested two
+// stages ahead (three register buffers).
 #pragma once
 #include "k_fwd_tanh.h"
 
@@ -32,35 +33,30 @@ __device__ __forceinline__ float row_sum16(float v) {
   return v;
 }
 
-// One stage's data, lane (g, c): dx_t and sd_t by the entries 16 k + 4 g + s;
-// dx_t, q_t, V_t, c_t rows 16 i + c; du_t[c], r_t[c]; D_t[4 g + s][16 i + c]
-// (0 past p - 1 and n - 1) and E_t[4 g + s][c] (0 past p - 1 and m - 1).
+// One stage's data, lane (g, c): dx_t, sd_t, q_t, V_t, c_t rows 16 i + c;
+// du_t[c], r_t[c]; D_t[4 g + s][16 i + c] (0 past p - 1 and n - 1) and
+// E_t[4 g + s][c] (0 past p - 1 and m - 1).
 template <int NT>
 struct AdjT {
-  f4 xc[NT], sdc[NT];
-  float xr[NT], qr[NT], vr[NT], cr[NT];
+  float xr[NT], sr[NT], qr[NT], vr[NT], cr[NT];
   float du, rh;
   f4 Dt[NT], Et;
 };
 
-template <int NT, bool WIDE>
+template <int NT>
 __device__ __forceinline__ void adj_load(AdjT<NT>& d, const Prob<float>& P, const Iter<float>& it,
                                          const Ws<float>& ws, int b, int t, int g, int c) {
   const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
   const size_t st = (size_t)b * N1 + t;
   const rsrc_t rx = make_rsrc(ws.dx + st * n, n);
   const rsrc_t rsd = make_rsrc(ws.sdh + ((size_t)b * N + t) * n, n);   // t < N
-#pragma unroll
-  for (int k = 0; k < NT; ++k) {
-    d.xc[k] = vload4<WIDE>(rx, 16 * k + 4 * g, 16 * k + 4 * g, n);
-    d.sdc[k] = vload4<WIDE>(rsd, 16 * k + 4 * g, 16 * k + 4 * g, n);
-  }
   const rsrc_t rq = make_rsrc(ws.qh + st * n, n), rV = make_rsrc(it.V + st * n, n);
   const rsrc_t rc = make_rsrc(ws.cdef + st * n, n);
 #pragma unroll
   for (int i = 0; i < NT; ++i) {
     const int o = (16 * i + c) * 4;
     d.xr[i] = rload(rx, o);
+    d.sr[i] = rload(rsd, o);
     d.qr[i] = rload(rq, o);
     d.vr[i] = rload(rV, o);
     d.cr[i] = rload(rc, o);
@@ -83,7 +79,7 @@ __device__ __forceinline__ void adj_load(AdjT<NT>& d, const Prob<float>& P, cons
   }
 }
 
-// lam's rows 16 i + c (every g) into the column layout (entries 16 k + 4 g + s)
+// a vector's rows 16 i + c (every g) into the column layout (entries 16 k + 4 g + s)
 template <int NT>
 __device__ __forceinline__ void rows_to_cols(const float (&yr)[NT], f4 (&yc)[NT], int g) {
 #pragma unroll
@@ -92,32 +88,24 @@ __device__ __forceinline__ void rows_to_cols(const float (&yr)[NT], f4 (&yc)[NT]
     for (int s = 0; s < 4; ++s) yc[k][s] = lane_read(yr[k], 20 * g + s);
 }
 
-// dv_t = lam_t - V_t (stored), the slope partials of stage t
+// dv_t = lam_t - V_t (stored) and the lane's parts of the slope partials of
+// stage t, added to acc (q'dx + r'du, (dv - V)'c, lam'c)
 template <int NT>
 __device__ __forceinline__ void adj_out(const float (&lam)[NT], const float (&xr)[NT], const float (&qr)[NT],
                                         const float (&vr)[NT], const float (&cr)[NT], float du, float rh,
-                                        const Ws<float>& ws, int b, int t, int n, int N, int g, int c, int lane) {
+                                        const Ws<float>& ws, int b, int t, int n, int N, int g, int c,
+                                        float (&acc)[3]) {
   const size_t st = (size_t)b * (N + 1) + t;
   const rsrc_t rdv = make_rsrc(ws.dv + st * n, n);
-  float df = g == 0 ? rh * du : 0.f, dc = 0.f, lc = 0.f;
+  acc[0] += g == 0 ? rh * du : 0.f;
 #pragma unroll
   for (int i = 0; i < NT; ++i) {
     const float dv = lam[i] - vr[i];
     const bool row_ok = g == 0 && 16 * i + c < n;
-    df += row_ok ? qr[i] * xr[i] : 0.f;
-    dc += row_ok ? (dv - vr[i]) * cr[i] : 0.f;
-    lc += row_ok ? lam[i] * cr[i] : 0.f;
+    acc[0] += row_ok ? qr[i] * xr[i] : 0.f;
+    acc[1] += row_ok ? (dv - vr[i]) * cr[i] : 0.f;
+    acc[2] += row_ok ? lam[i] * cr[i] : 0.f;
     rstore(rdv, row_ok ? (16 * i + c) * 4 : kOOB, dv);
-  }
-  df = wave_sum(df);
-  dc = wave_sum(dc);
-  lc = wave_sum(lc);
-  if (lane == 0) {
-    float* fp = ws.fpart + st * NFP;
-    fp[FP_DF] = df;
-    fp[FP_DC] = dc;
-    fp[FP_LC] = lc;
-    fp[3] = 0.f;
   }
 }
 
@@ -130,6 +118,9 @@ __global__ void __launch_bounds__(64) k_adjoint_tanh(Prob<float> P, Prm<float> p
   const float hs = P.h, hk = P.h * P.kappa, rc = prm.rho_c;
   int off[NT][4];
   aop_offsets<NT>(off, n, g, c);
+  // the slope partials are summed over the stages by the line search: the
+  // instance's totals go to stage N's entry, the other stages' are 0
+  float acc[3] = {0.f, 0.f, 0.f};
 
   // the terminal stage: lam_N = Q_N dx_N + q_N
   float lamr[NT];
@@ -160,7 +151,7 @@ __global__ void __launch_bounds__(64) k_adjoint_tanh(Prob<float> P, Prm<float> p
         for (int s = 0; s < 4; ++s) pa += QN[i][k][s] * xc[k][s];
       lamr[i] = col_sum4(pa) + qr[i];
     }
-    adj_out<NT>(lamr, xr, qr, vr, cr, 0.f, 0.f, ws, b, N, n, N, g, c, lane);
+    adj_out<NT>(lamr, xr, qr, vr, cr, 0.f, 0.f, ws, b, N, n, N, g, c, acc);
     rows_to_cols<NT>(lamr, lamc, g);
   }
 
@@ -198,11 +189,16 @@ __global__ void __launch_bounds__(64) k_adjoint_tanh(Prob<float> P, Prm<float> p
         atw[(NT * NT + i * NT + k) * 64 + lane] = Wt[i][k];
       }
   }
-  AdjT<NT> d0, d1;
-  adj_load<NT, WIDE>(d0, P, it, ws, b, N - 1, g, c);
-
   // stage t from lam_{t+1} (in lamc)
   auto step = [&](const AdjT<NT>& d, int t) {
+    // dx_t and sd_t * lam_{t+1} by the entries 16 k + 4 g + s
+    f4 xc[NT], zc[NT];
+    rows_to_cols<NT>(d.xr, xc, g);
+    rows_to_cols<NT>(d.sr, zc, g);
+#pragma unroll
+    for (int k = 0; k < NT; ++k)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) zc[k][s] *= lamc[k][s];
     // h = D dx + E du: the lane's rows 4 g + s, summed over the row of lanes
     f4 w;
 #pragma unroll
@@ -221,26 +217,47 @@ __global__ void __launch_bounds__(64) k_adjoint_tanh(Prob<float> P, Prm<float> p
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
           pa += a[s] * lamc[k][s];
-          pa += wt[s] * (d.sdc[k][s] * lamc[k][s]);
-          pa += Qt[i][k][s] * d.xc[k][s];
+          pa += wt[s] * zc[k][s];
+          pa += Qt[i][k][s] * xc[k][s];
         }
       }
 #pragma unroll
       for (int s = 0; s < 4; ++s) pa += d.Dt[i][s] * w[s];
       lamr[i] = col_sum4(pa) + d.qr[i];
     }
-    adj_out<NT>(lamr, d.xr, d.qr, d.vr, d.cr, d.du, d.rh, ws, b, t, n, N, g, c, lane);
+    adj_out<NT>(lamr, d.xr, d.qr, d.vr, d.cr, d.du, d.rh, ws, b, t, n, N, g, c, acc);
     rows_to_cols<NT>(lamr, lamc, g);
   };
 
-  for (int t = N - 1; t >= 0; t -= 2) {
-    adj_load<NT, WIDE>(d1, P, it, ws, b, t > 0 ? t - 1 : 0, g, c);
+  // the stages N - 1 .. 0, each one's data requested two stages ahead
+  AdjT<NT> d0, d1, d2;
+  adj_load<NT>(d0, P, it, ws, b, N - 1, g, c);
+  adj_load<NT>(d1, P, it, ws, b, N > 1 ? N - 2 : 0, g, c);
+  for (int t = N - 1; t >= 0; t -= 3) {
+    adj_load<NT>(d2, P, it, ws, b, t > 1 ? t - 2 : 0, g, c);
     __builtin_amdgcn_sched_barrier(0);
     step(d0, t);
     if (t == 0) break;
-    adj_load<NT, WIDE>(d0, P, it, ws, b, t > 1 ? t - 2 : 0, g, c);
+    adj_load<NT>(d0, P, it, ws, b, t > 2 ? t - 3 : 0, g, c);
     __builtin_amdgcn_sched_barrier(0);
     step(d1, t - 1);
+    if (t == 1) break;
+    adj_load<NT>(d1, P, it, ws, b, t > 3 ? t - 4 : 0, g, c);
+    __builtin_amdgcn_sched_barrier(0);
+    step(d2, t - 2);
+  }
+  const float df = wave_sum(acc[0]), dc = wave_sum(acc[1]), lc = wave_sum(acc[2]);
+  // stage N's entry carries the totals, the others 0
+  const rsrc_t rfp = make_rsrc(ws.fpart + (size_t)b * N1 * NFP, N1 * NFP);
+  for (int t = lane; t < N; t += 64) {
+#pragma unroll
+    for (int q = 0; q < NFP; ++q) rstore(rfp, (t * NFP + q) * 4, 0.f);
+  }
+  if (lane == 0) {
+    rstore(rfp, (N * NFP + FP_DF) * 4, df);
+    rstore(rfp, (N * NFP + FP_DC) * 4, dc);
+    rstore(rfp, (N * NFP + FP_LC) * 4, lc);
+    rstore(rfp, (N * NFP + 3) * 4, 0.f);
   }
 }
 
