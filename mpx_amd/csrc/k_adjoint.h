@@ -8,7 +8,8 @@
 // multipliers of the dense KKT system, the same as S_t dx_t + s_t of the
 // Riccati solution, which k_lqr_wave then never stores); then dv_t = lam_t -
 // V_t and the slope partials q_t'dx_t + r_t'du_t, (dv_t - V_t)'c_t, lam_t'c_t
-// (c_t the constraint row paired with V_t; c_0 = x0 - X_0).
+// (c_t the constraint row paired with V_t; c_0 = x0 - X_0); the equality
+// rows' slope D_t dx_t + E_t du_t goes to hq for the trials.
 //
 // Lane (g, c) holds rows 16 i + c, columns 16 k + 4 g + s of Q (registers),
 // (I + h A)' and (h kappa W)' (LDS) for the whole sweep, lam_{t+1},
@@ -43,39 +44,57 @@ struct AdjT {
   f4 Dt[NT], Et;
 };
 
-template <int NT>
-__device__ __forceinline__ void adj_load(AdjT<NT>& d, const Prob<float>& P, const Iter<float>& it,
-                                         const Ws<float>& ws, int b, int t, int g, int c) {
+// The instance's arrays (one descriptor each for all of its stages).
+struct AdjR {
+  rsrc_t x, sd, q, V, c, du, rh, D, E, dv, hq;
+  long long dst, est;   // stage strides of D and E (elements)
+};
+
+__device__ __forceinline__ AdjR adj_rsrc(const Prob<float>& P, const Iter<float>& it, const Ws<float>& ws, int b) {
   const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
-  const size_t st = (size_t)b * N1 + t;
-  const rsrc_t rx = make_rsrc(ws.dx + st * n, n);
-  const rsrc_t rsd = make_rsrc(ws.sdh + ((size_t)b * N + t) * n, n);   // t < N
-  const rsrc_t rq = make_rsrc(ws.qh + st * n, n), rV = make_rsrc(it.V + st * n, n);
-  const rsrc_t rc = make_rsrc(ws.cdef + st * n, n);
+  AdjR r;
+  r.x = make_rsrc(ws.dx + (size_t)b * N1 * n, N1 * n);
+  r.sd = make_rsrc(ws.sdh + (size_t)b * N * n, N * n);
+  r.q = make_rsrc(ws.qh + (size_t)b * N1 * n, N1 * n);
+  r.V = make_rsrc(it.V + (size_t)b * N1 * n, N1 * n);
+  r.c = make_rsrc(ws.cdef + (size_t)b * N1 * n, N1 * n);
+  r.du = make_rsrc(ws.du + (size_t)b * N * m, N * m);
+  r.rh = make_rsrc(ws.rh + (size_t)b * N * m, N * m);
+  r.D = view_rsrc(P.D, b, N, p * n);   // p = 0: no records
+  r.E = view_rsrc(P.E, b, N, p * m);
+  r.dv = make_rsrc(ws.dv + (size_t)b * N1 * n, N1 * n);
+  r.hq = make_rsrc(ws.hq + (size_t)b * N * 2 * p, N * 2 * p);
+  r.dst = P.D.st;
+  r.est = P.E.st;
+  return r;
+}
+
+template <int NT>
+__device__ __forceinline__ void adj_load(AdjT<NT>& d, const AdjR& R, int n, int m, int p, int t, int g, int c) {
 #pragma unroll
   for (int i = 0; i < NT; ++i) {
-    const int o = (16 * i + c) * 4;
-    d.xr[i] = rload(rx, o);
-    d.sr[i] = rload(rsd, o);
-    d.qr[i] = rload(rq, o);
-    d.vr[i] = rload(rV, o);
-    d.cr[i] = rload(rc, o);
+    const int row = 16 * i + c, o = (t * n + row) * 4;
+    const bool in = row < n;
+    d.xr[i] = rload(R.x, in ? o : kOOB);
+    d.sr[i] = rload(R.sd, in ? o : kOOB);   // t < N
+    d.qr[i] = rload(R.q, in ? o : kOOB);
+    d.vr[i] = rload(R.V, in ? o : kOOB);
+    d.cr[i] = rload(R.c, in ? o : kOOB);
   }
-  const size_t su = (size_t)b * N + t;
-  d.du = rload(make_rsrc(ws.du + su * m, m), c * 4);
-  d.rh = rload(make_rsrc(ws.rh + su * m, m), c * 4);
-  const rsrc_t rD = make_rsrc(P.D.at(b, t), p * n), rE = make_rsrc(P.E.at(b, t), p * m);   // p = 0: no records
+  d.du = rload(R.du, c < m ? (t * m + c) * 4 : kOOB);
+  d.rh = rload(R.rh, c < m ? (t * m + c) * 4 : kOOB);
+  const int od = (int)(t * R.dst), oe = (int)(t * R.est);
 #pragma unroll
   for (int i = 0; i < NT; ++i)
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const int r = 4 * g + s, col = 16 * i + c;
-      d.Dt[i][s] = rload(rD, r < p && col < n ? (r * n + col) * 4 : kOOB);
+      d.Dt[i][s] = rload(R.D, r < p && col < n ? (od + r * n + col) * 4 : kOOB);
     }
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     const int r = 4 * g + s;
-    d.Et[s] = rload(rE, r < p && c < m ? (r * m + c) * 4 : kOOB);
+    d.Et[s] = rload(R.E, r < p && c < m ? (oe + r * m + c) * 4 : kOOB);
   }
 }
 
@@ -93,10 +112,7 @@ __device__ __forceinline__ void rows_to_cols(const float (&yr)[NT], f4 (&yc)[NT]
 template <int NT>
 __device__ __forceinline__ void adj_out(const float (&lam)[NT], const float (&xr)[NT], const float (&qr)[NT],
                                         const float (&vr)[NT], const float (&cr)[NT], float du, float rh,
-                                        const Ws<float>& ws, int b, int t, int n, int N, int g, int c,
-                                        float (&acc)[3]) {
-  const size_t st = (size_t)b * (N + 1) + t;
-  const rsrc_t rdv = make_rsrc(ws.dv + st * n, n);
+                                        rsrc_t rdv, int t, int n, int g, int c, float (&acc)[3]) {
   acc[0] += g == 0 ? rh * du : 0.f;
 #pragma unroll
   for (int i = 0; i < NT; ++i) {
@@ -105,7 +121,7 @@ __device__ __forceinline__ void adj_out(const float (&lam)[NT], const float (&xr
     acc[0] += row_ok ? qr[i] * xr[i] : 0.f;
     acc[1] += row_ok ? (dv - vr[i]) * cr[i] : 0.f;
     acc[2] += row_ok ? lam[i] * cr[i] : 0.f;
-    rstore(rdv, row_ok ? (16 * i + c) * 4 : kOOB, dv);
+    rstore(rdv, row_ok ? (t * n + 16 * i + c) * 4 : kOOB, dv);
   }
 }
 
@@ -114,8 +130,9 @@ __global__ void __launch_bounds__(64) k_adjoint_tanh(Prob<float> P, Prm<float> p
   const int b = blockIdx.x;
   if (skip_instance(ws, b)) return;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
-  const int n = P.n, N = P.N, N1 = N + 1;
+  const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
   const float hs = P.h, hk = P.h * P.kappa, rc = prm.rho_c;
+  const AdjR R = adj_rsrc(P, it, ws, b);
   int off[NT][4];
   aop_offsets<NT>(off, n, g, c);
   // the slope partials are summed over the stages by the line search: the
@@ -151,7 +168,7 @@ __global__ void __launch_bounds__(64) k_adjoint_tanh(Prob<float> P, Prm<float> p
         for (int s = 0; s < 4; ++s) pa += QN[i][k][s] * xc[k][s];
       lamr[i] = col_sum4(pa) + qr[i];
     }
-    adj_out<NT>(lamr, xr, qr, vr, cr, 0.f, 0.f, ws, b, N, n, N, g, c, acc);
+    adj_out<NT>(lamr, xr, qr, vr, cr, 0.f, 0.f, R.dv, N, n, g, c, acc);
     rows_to_cols<NT>(lamr, lamc, g);
   }
 
@@ -200,13 +217,16 @@ __global__ void __launch_bounds__(64) k_adjoint_tanh(Prob<float> P, Prm<float> p
 #pragma unroll
       for (int s = 0; s < 4; ++s) zc[k][s] *= lamc[k][s];
     // h = D dx + E du: the lane's rows 4 g + s, summed over the row of lanes
+    // (also the equality rows' slope along the step, for the trials)
     f4 w;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       float hp = d.Et[s] * d.du;
 #pragma unroll
       for (int i = 0; i < NT; ++i) hp += d.Dt[i][s] * d.xr[i];
-      w[s] = rc * row_sum16(hp);
+      const float h = row_sum16(hp);
+      rstore(R.hq, c == 0 && 4 * g + s < p ? (t * 2 * p + p + 4 * g + s) * 4 : kOOB, h);
+      w[s] = rc * h;
     }
 #pragma unroll
     for (int i = 0; i < NT; ++i) {
@@ -225,24 +245,24 @@ __global__ void __launch_bounds__(64) k_adjoint_tanh(Prob<float> P, Prm<float> p
       for (int s = 0; s < 4; ++s) pa += d.Dt[i][s] * w[s];
       lamr[i] = col_sum4(pa) + d.qr[i];
     }
-    adj_out<NT>(lamr, d.xr, d.qr, d.vr, d.cr, d.du, d.rh, ws, b, t, n, N, g, c, acc);
+    adj_out<NT>(lamr, d.xr, d.qr, d.vr, d.cr, d.du, d.rh, R.dv, t, n, g, c, acc);
     rows_to_cols<NT>(lamr, lamc, g);
   };
 
   // the stages N - 1 .. 0, each one's data requested two stages ahead
   AdjT<NT> d0, d1, d2;
-  adj_load<NT>(d0, P, it, ws, b, N - 1, g, c);
-  adj_load<NT>(d1, P, it, ws, b, N > 1 ? N - 2 : 0, g, c);
+  adj_load<NT>(d0, R, n, m, p, N - 1, g, c);
+  adj_load<NT>(d1, R, n, m, p, N > 1 ? N - 2 : 0, g, c);
   for (int t = N - 1; t >= 0; t -= 3) {
-    adj_load<NT>(d2, P, it, ws, b, t > 1 ? t - 2 : 0, g, c);
+    adj_load<NT>(d2, R, n, m, p, t > 1 ? t - 2 : 0, g, c);
     __builtin_amdgcn_sched_barrier(0);
     step(d0, t);
     if (t == 0) break;
-    adj_load<NT>(d0, P, it, ws, b, t > 2 ? t - 3 : 0, g, c);
+    adj_load<NT>(d0, R, n, m, p, t > 2 ? t - 3 : 0, g, c);
     __builtin_amdgcn_sched_barrier(0);
     step(d1, t - 1);
     if (t == 1) break;
-    adj_load<NT>(d1, P, it, ws, b, t > 3 ? t - 4 : 0, g, c);
+    adj_load<NT>(d1, R, n, m, p, t > 3 ? t - 4 : 0, g, c);
     __builtin_amdgcn_sched_barrier(0);
     step(d2, t - 2);
   }

@@ -10,8 +10,8 @@
 // diagonal, the defect c_t = f(x_t, u_t) - x_{t+1}, the merit parts and the
 // box multipliers that are not positive replaced -- and the projection of u_t
 // into the box interior, from the products A x, W x, B u, Q (x - x_ref) and
-// R (u - u_ref) on the matrix cores; and sech^2(W x_t) (k_fwd_tanh's form of
-// the Jacobian).  The Jacobian
+// R (u - u_ref) on the matrix cores; sech^2(W x_t) (k_fwd_tanh's form of
+// the Jacobian) and the equality rows h_t at the iterate (the trials').  The Jacobian
 //   f_x = I + h (A + kappa diag(sech^2(W x)) W)
 // is written last (wfx != 0) by the whole wave, stage by stage: each lane keeps the
 // entries e = 256 k + 4 lane .. + 3 of I + h A and h kappa W (the same for
@@ -187,6 +187,9 @@ k_linearize_mma(Prob<float> P, Prm<float> prm, Iter<float> it, Ws<float> ws, int
     const rsrc_t re = view_rsrc(P.e, b, N, p);
     const rsrc_t rmu = make_rsrc(it.Yeq + (size_t)b * N * p, N * p);
     const int od = (int)(tc * P.D.st) + 4 * g, oe = (int)(tc * P.E.st) + 4 * g;
+    // h at the iterate, for the trials (one descriptor for the instance: a
+    // per-lane base would make the descriptor divergent)
+    const rsrc_t rhq = make_rsrc(ws.hq + (size_t)b * N * 2 * p, N * 2 * p);
     for (int r0 = 0; r0 < p; r0 += RQ) {
       f4 Dr[RQ][NT], Er[RQ];
       float ev[RQ], mv[RQ];
@@ -210,6 +213,7 @@ k_linearize_mma(Prob<float> P, Prm<float> prm, Iter<float> it, Ws<float> ws, int
         for (int v = 0; v < 4; ++v) h += Er[q][v] * U0[v];
         h = col_sum4(h) + ev[q];
         const bool in = r0 + q < p;
+        rstore(rhq, live && in && ((r0 + q) & 3) == g ? (tc * 2 * p + r0 + q) * 4 : kOOB, h);
         const float w = in ? mv[q] + prm.rho_c * h : 0.f;
         cost += in && g == 0 ? mv[q] * h + 0.5f * prm.rho_c * h * h : 0.f;
 #pragma unroll

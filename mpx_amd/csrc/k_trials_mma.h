@@ -18,9 +18,10 @@
 // the box logarithms are evaluated per trial.
 //
 // The kernel is bound by the latency of its loads (each one gathers 16
-// stages): WIDE (n, m multiples of 4, every row 16-byte aligned) moves a
+// stages): the block's four waves (64 stages) stage the instance's matrices
+// in LDS once, WIDE (n, m multiples of 4, every row 16-byte aligned) moves a
 // lane's four consecutive entries with one 128-bit load, and the equality rows
-// go four at a time, so that one wait covers the loads of four rows.
+// go several at a time, so that one wait covers the loads of several rows.
 #pragma once
 #include "k_lqr_wave.h"
 #include "k_trials.h"
@@ -60,40 +61,83 @@ __device__ __forceinline__ f4 vload4(rsrc_t r, int o, int e0, int lim) {
   }
 }
 
-// Trials j0 <= j < j1 at the stages 16 blockIdx.x + c < N of instance blockIdx.y.
-template <int NT, bool WIDE>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NT == 1 ? 3 : 2)))
+// The stage-invariant matrices of one instance in LDS for a block of waves:
+// rows of LD = 16 NT + 4 floats (LD / 4 odd: the rows 16 i + c of a 128-bit
+// A-operand read fall on distinct banks), 16 NT rows, 0 past the matrix.
+// (each array is allocated in whole passes of the block's 256 threads, so
+// that the staging stores need no guard)
+template <int NT>
+struct MmaLds {
+  static constexpr int LD = 16 * NT + 4, LDM = 20, ROWS = 16 * NT;
+  static constexpr int pad(int e) { return (e + 255) / 256 * 256; }
+  static constexpr int NN = pad(ROWS * LD), NM = pad(ROWS * LDM), MM = pad(16 * LDM);
+};
+
+// rows x cols row-major M into sm (ROWS x LD, allocated in whole 256-entry
+// passes), every entry of sm written; the block's 256 threads request all
+// their loads before the first store
+template <int ROWS, int LD>
+__device__ __forceinline__ void lds_stage(float* sm, const float* M, int rows, int cols, int tid) {
+  constexpr int PER = (ROWS * LD + 255) / 256;
+  const rsrc_t r = make_rsrc(M, rows * cols);
+  float v[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = tid + 256 * q, row = e / LD, col = e - row * LD;
+    v[q] = rload(r, row < rows && col < cols ? (row * cols + col) * 4 : kOOB);
+  }
+#pragma unroll
+  for (int q = 0; q < PER; ++q) sm[tid + 256 * q] = v[q];
+}
+
+// A-operand tile (i, k) of an LDS matrix: row 16 i + c, columns 16 k + 4 g .. + 3
+template <int LD>
+__device__ __forceinline__ f4 lds_aop(const float* sm, int i, int k, int g, int c) {
+  return *reinterpret_cast<const f4*>(sm + (16 * i + c) * LD + 16 * k + 4 * g);
+}
+
+// Trials j0 <= j < j1 at the stages 64 blockIdx.x + 16 w + c < N of instance
+// blockIdx.y (wave w of the block).  The block stages A, W, Q, B, R in LDS
+// once (MmaLds); each wave reads its A-operand tiles from there where a
+// product uses them.
+// HQ: the equality rows at the iterate and their slope along the step are
+// in hq already (k_linearize_mma, k_adjoint_tanh): the stage reads 2 p values
+// instead of D_t and E_t.
+template <int NT, bool WIDE, bool HQ>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 1 || (HQ && NT == 2) ? 3 : 2)))
 k_trials_mma(Prob<float> P, Prm<float> prm, Iter<float> it, Ws<float> ws, int j0, int j1) {
-  const int b = blockIdx.y, lane = threadIdx.x, g = lane >> 4, c = lane & 15;
+  using L = MmaLds<NT>;
+  __shared__ __align__(16) float smA[L::NN], smW[L::NN], smQ[L::NN], smB[L::NM], smR[L::MM];
+  const int b = blockIdx.y, tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, g = lane >> 4, c = lane & 15;
   if (skip_instance(ws, b)) return;
   const int n = P.n, m = P.m, p = P.p, N = P.N, N1 = N + 1;
-  // an earlier pass accepted: the same answer in every lane (one instance)
+  // an earlier pass accepted: the same answer in every wave (one instance)
   if (j0 > 0 && ls_first_accept(prm, ws, b, N1, j0) >= 0) return;
-  const int t = blockIdx.x * 16 + c;
+  lds_stage<L::ROWS, L::LD>(smA, P.A.at(b, 0), n, n, tid);
+  lds_stage<L::ROWS, L::LD>(smW, P.W.at(b, 0), n, n, tid);
+  lds_stage<L::ROWS, L::LD>(smQ, P.Q.at(b, 0), n, n, tid);
+  lds_stage<L::ROWS, L::LDM>(smB, P.Bm.at(b, 0), n, m, tid);
+  lds_stage<16, L::LDM>(smR, P.R.at(b, 0), m, m, tid);
+  __syncthreads();
+  const int t0 = blockIdx.x * 64 + wv * 16;
+  if (t0 >= N) return;   // no live stage (after the barrier)
+  const int t = t0 + c;
   const bool live = t < N;
   const int tc = live ? t : N - 1;   // a stage with data for the columns past N - 1 (nothing of theirs is kept)
   constexpr int BIG = 1 << 30;       // no guard: entries past the vector's end meet zero matrix columns
+  constexpr int RQ = NT == 3 ? 2 : 4;
+  const float hs = P.h;
 
   // The loads go in three groups, each issued before the arithmetic of the
   // one before it (pinned by scheduling barriers: the scheduler would sink
-  // each load to its use, one wait per tile): (1) the stage vectors and the
-  // data of the defect and of W x, (2) the data of the tracking and control
-  // costs, (3) the costate at t + 1 and the box; the equality rows follow in
-  // passes of RQ rows.
-  constexpr int RQ = NT == 3 ? 2 : 4;
-  const float hs = P.h;
-  int off[NT][4];
-  aop_offsets<NT>(off, n, g, c);
-
-  // (1) B-operand tiles of the stage vectors (entries past the vector's end
-  // read the next stage's or 0: they meet the zero columns of the matrix
-  // tiles); the A-operand tiles of A, B, W (rows 16 i + c, columns
-  // 16 k + 4 g + s; 0 past their ends); x_{t+1} and dx_{t+1}
+  // each load to its use): (1) the stage vectors (B-operand tiles; entries
+  // past the vector's end read the next stage's or 0: they meet the zero
+  // columns of the matrix tiles), x_{t+1} and dx_{t+1}
   const rsrc_t rX = make_rsrc(it.X + (size_t)b * N1 * n, N1 * n);
   const rsrc_t rdX = make_rsrc(ws.dx + (size_t)b * N1 * n, N1 * n);
   const rsrc_t rU = make_rsrc(it.U + (size_t)b * N * m, N * m);
   const rsrc_t rdU = make_rsrc(ws.du + (size_t)b * N * m, N * m);
-  f4 X0[NT], X1[NT], U0, U1;
+  f4 X0[NT], X1[NT], U0, U1, Xn[NT], dXn[NT];
 #pragma unroll
   for (int k = 0; k < NT; ++k) {
     X0[k] = vload4<WIDE>(rX, tc * n + 16 * k + 4 * g, 0, BIG);
@@ -101,19 +145,20 @@ k_trials_mma(Prob<float> P, Prm<float> prm, Iter<float> it, Ws<float> ws, int j0
   }
   U0 = vload4<WIDE>(rU, tc * m + 4 * g, 0, BIG);
   U1 = vload4<WIDE>(rdU, tc * m + 4 * g, 0, BIG);
-  f4 Mt[NT][NT], Wt[NT][NT], Bt[NT], Xn[NT], dXn[NT];
-  load_aop_raw<NT, NT, WIDE>(Mt, P.A.at(b, 0), n, n, off);
-  {
-    const rsrc_t rB = make_rsrc(P.Bm.at(b, 0), n * m);
-#pragma unroll
-    for (int i = 0; i < NT; ++i) Bt[i] = vload4<WIDE>(rB, (16 * i + c) * m + 4 * g, 4 * g, m);
-  }
 #pragma unroll
   for (int i = 0; i < NT; ++i) {
     Xn[i] = vload4<WIDE>(rX, (tc + 1) * n + 16 * i + 4 * g, 0, BIG);
     dXn[i] = vload4<WIDE>(rdX, (tc + 1) * n + 16 * i + 4 * g, 0, BIG);
   }
-  load_aop_raw<NT, NT, WIDE>(Wt, P.W.at(b, 0), n, n, off);
+  // (2) x_ref, u_ref
+  f4 xr[NT], ur;
+  {
+    const rsrc_t rxr = view_rsrc(P.xref, b, N1, n), rur = view_rsrc(P.uref, b, N, m);
+#pragma unroll
+    for (int k = 0; k < NT; ++k)
+      xr[k] = vload4<WIDE>(rxr, (int)(tc * P.xref.st) + 16 * k + 4 * g, 16 * k + 4 * g, n);
+    ur = vload4<WIDE>(rur, (int)(tc * P.uref.st) + 4 * g, 4 * g, m);
+  }
   __builtin_amdgcn_sched_barrier(0);
 
   // the defect c = x + h (A x + B u) - x_{t+1} at the iterate (0) and along the
@@ -123,16 +168,19 @@ k_trials_mma(Prob<float> P, Prm<float> prm, Iter<float> it, Ws<float> ws, int j0
   for (int i = 0; i < NT; ++i) {
     f4 a0 = zero4(), a1 = zero4();
 #pragma unroll
-    for (int k = 0; k < NT; ++k)
+    for (int k = 0; k < NT; ++k) {
+      const f4 Mt = lds_aop<L::LD>(smA, i, k, g, c);
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
-        a0 = mma(Mt[i][k][s], X0[k][s], a0);
-        a1 = mma(Mt[i][k][s], X1[k][s], a1);
+        a0 = mma(Mt[s], X0[k][s], a0);
+        a1 = mma(Mt[s], X1[k][s], a1);
       }
+    }
+    const f4 Bt = lds_aop<L::LDM>(smB, i, 0, g, c);
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      a0 = mma(Bt[i][s], U0[s], a0);
-      a1 = mma(Bt[i][s], U1[s], a1);
+      a0 = mma(Bt[s], U0[s], a0);
+      a1 = mma(Bt[s], U1[s], a1);
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -140,40 +188,9 @@ k_trials_mma(Prob<float> P, Prm<float> prm, Iter<float> it, Ws<float> ws, int j0
       C0[i][v] = ok ? X0[i][v] + hs * a0[v] - Xn[i][v] : 0.f;
       C1[i][v] = ok ? X1[i][v] + hs * a1[v] - dXn[i][v] : 0.f;
     }
+    __builtin_amdgcn_sched_barrier(0);   // one row tile's LDS reads at a time
   }
-
-  // (2) Q, R, x_ref, u_ref
-  load_aop_raw<NT, NT, WIDE>(Mt, P.Q.at(b, 0), n, n, off);
-  f4 xr[NT], ur, Rt;
-  {
-    const rsrc_t rxr = view_rsrc(P.xref, b, N1, n), rur = view_rsrc(P.uref, b, N, m);
-#pragma unroll
-    for (int k = 0; k < NT; ++k)
-      xr[k] = vload4<WIDE>(rxr, (int)(tc * P.xref.st) + 16 * k + 4 * g, 16 * k + 4 * g, n);
-    ur = vload4<WIDE>(rur, (int)(tc * P.uref.st) + 4 * g, 4 * g, m);
-    const rsrc_t rR = make_rsrc(P.R.at(b, 0), m * m);
-    Rt = vload4<WIDE>(rR, c * m + 4 * g, 4 * g, m);
-  }
-  __builtin_amdgcn_sched_barrier(0);
-
-  // W x: the argument of the model's tanh (W's rows past n are 0)
-  f4 w0[NT], w1[NT];
-#pragma unroll
-  for (int i = 0; i < NT; ++i) {
-    f4 a0 = zero4(), a1 = zero4();
-#pragma unroll
-    for (int k = 0; k < NT; ++k)
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        a0 = mma(Wt[i][k][s], X0[k][s], a0);
-        a1 = mma(Wt[i][k][s], X1[k][s], a1);
-      }
-    w0[i] = a0;
-    w1[i] = a1;
-  }
-
-  // (3) the multiplier V_{t+1} and its step (rows past n: their defect is 0);
-  // the box rows u = U + a dU within (lo, hi) (rows past m: a term of 0)
+  // (3) V_{t+1} and its step, the box
   f4 vn0[NT], vn1[NT], lo, hi;
   {
     const rsrc_t rV = make_rsrc(it.V + (size_t)b * N1 * n, N1 * n);
@@ -183,14 +200,31 @@ k_trials_mma(Prob<float> P, Prm<float> prm, Iter<float> it, Ws<float> ws, int j0
       vn0[i] = vload4<WIDE>(rV, (tc + 1) * n + 16 * i + 4 * g, 0, BIG);
       vn1[i] = vload4<WIDE>(rdV, (tc + 1) * n + 16 * i + 4 * g, 0, BIG);
     }
-  }
-  const bool box = P.box != 0;
-  if (box) {
+    const int mb = P.box ? m : 0;   // a problem without boxes: the loads go nowhere
     const rsrc_t rlo = view_rsrc(P.ulo, b, N, m), rhi = view_rsrc(P.uhi, b, N, m);
-    lo = vload4<WIDE>(rlo, (int)(tc * P.ulo.st) + 4 * g, 4 * g, m);
-    hi = vload4<WIDE>(rhi, (int)(tc * P.uhi.st) + 4 * g, 4 * g, m);
+    lo = vload4<WIDE>(rlo, (int)(tc * P.ulo.st) + 4 * g, 4 * g, mb);
+    hi = vload4<WIDE>(rhi, (int)(tc * P.uhi.st) + 4 * g, 4 * g, mb);
   }
   __builtin_amdgcn_sched_barrier(0);
+
+  // W x: the argument of the model's tanh (W's rows past n are 0)
+  f4 w0[NT], w1[NT];
+#pragma unroll
+  for (int i = 0; i < NT; ++i) {
+    f4 a0 = zero4(), a1 = zero4();
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+      const f4 Mt = lds_aop<L::LD>(smW, i, k, g, c);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        a0 = mma(Mt[s], X0[k][s], a0);
+        a1 = mma(Mt[s], X1[k][s], a1);
+      }
+    }
+    w0[i] = a0;
+    w1[i] = a1;
+    __builtin_amdgcn_sched_barrier(0);
+  }
 
   // tracking cost 1/2 (d0 + a x1)' Q (d0 + a x1) and control cost
   // 1/2 (e0 + a u1)' R (e0 + a u1), d0 = x - x_ref, e0 = u - u_ref: the lane's
@@ -208,19 +242,23 @@ k_trials_mma(Prob<float> P, Prm<float> prm, Iter<float> it, Ws<float> ws, int j0
     for (int i = 0; i < NT; ++i) {
       f4 q0 = zero4(), q1 = zero4();
 #pragma unroll
-      for (int k = 0; k < NT; ++k)
+      for (int k = 0; k < NT; ++k) {
+        const f4 Mt = lds_aop<L::LD>(smQ, i, k, g, c);
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-          q0 = mma(Mt[i][k][s], D0[k][s], q0);
-          q1 = mma(Mt[i][k][s], X1[k][s], q1);
+          q0 = mma(Mt[s], D0[k][s], q0);
+          q1 = mma(Mt[s], X1[k][s], q1);
         }
+      }
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         pq0 += D0[i][v] * q0[v];
         pq1 += D0[i][v] * q1[v] + X1[i][v] * q0[v];
         pq2 += X1[i][v] * q1[v];
       }
+      __builtin_amdgcn_sched_barrier(0);
     }
+    const f4 Rt = lds_aop<L::LDM>(smR, 0, 0, g, c);
     f4 r0 = zero4(), r1 = zero4();
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
@@ -240,7 +278,24 @@ k_trials_mma(Prob<float> P, Prm<float> prm, Iter<float> it, Ws<float> ws, int j0
   // of a column holds the stage's coefficients); RQ rows per pass, the rows
   // past p of the last pass read other rows' data (or 0) and are dropped
   float pe0 = 0.f, pe1 = 0.f, pe2 = 0.f;
-  if (p > 0) {
+  if (HQ && p > 0) {   // rows 4 g + s of the lane's stage, summed over the column
+    const rsrc_t rh = make_rsrc(ws.hq + (size_t)b * N * 2 * p, N * 2 * p);
+    const rsrc_t rmu = make_rsrc(it.Yeq + (size_t)b * N * p, N * p);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int r = 4 * g + s;
+      const bool in = r < p;
+      const float h0 = rload(rh, in ? (tc * 2 * p + r) * 4 : kOOB);
+      const float h1 = rload(rh, in ? (tc * 2 * p + p + r) * 4 : kOOB);
+      const float mu = rload(rmu, in ? (tc * p + r) * 4 : kOOB);
+      pe0 += mu * h0 + 0.5f * prm.rho_c * h0 * h0;
+      pe1 += mu * h1 + prm.rho_c * h0 * h1;
+      pe2 += 0.5f * prm.rho_c * h1 * h1;
+    }
+    pe0 = col_sum4(pe0);
+    pe1 = col_sum4(pe1);
+    pe2 = col_sum4(pe2);
+  } else if (p > 0) {
     const rsrc_t rD = view_rsrc(P.D, b, N, p * n), rE = view_rsrc(P.E, b, N, p * m);
     const rsrc_t re = view_rsrc(P.e, b, N, p);
     const rsrc_t rmu = make_rsrc(it.Yeq + (size_t)b * N * p, N * p);
@@ -292,16 +347,15 @@ k_trials_mma(Prob<float> P, Prm<float> prm, Iter<float> it, Ws<float> ws, int j0
   const float c1 = 0.5f * pq1 + (g == 0 ? pe1 : 0.f);
   const float c2 = 0.5f * pq2 + (g == 0 ? pe2 : 0.f);
 
+  const bool box = P.box != 0;
   f4 bu0, bu1, blo, bhi;
-  if (box) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const bool ok = 4 * g + v < m;
-      bu0[v] = ok ? U0[v] : 0.f;
-      bu1[v] = ok ? U1[v] : 0.f;
-      blo[v] = ok ? lo[v] : -1.f;
-      bhi[v] = ok ? hi[v] : 1.f;
-    }
+  for (int v = 0; v < 4; ++v) {
+    const bool ok = 4 * g + v < m;
+    bu0[v] = ok ? U0[v] : 0.f;
+    bu1[v] = ok ? U1[v] : 0.f;
+    blo[v] = ok ? lo[v] : -1.f;
+    bhi[v] = ok ? hi[v] : 1.f;
   }
 
   const float hk = P.h * P.kappa;

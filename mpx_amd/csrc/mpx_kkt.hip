@@ -643,6 +643,7 @@ struct TrialShape {
   bool sh, v4;
   int mma;   // > 0: k_trials_mma<mma> for the stages < N (its state tiles)
   bool mma_wide;   // the problem's rows of k_trials_mma 16-byte aligned (n, m multiples of 4)
+  bool hq;         // the equality rows and their slopes precomputed (the tanh model on the wave path)
   int waves;
   size_t lds;
   dim3 grid;
@@ -660,6 +661,7 @@ TrialShape trial_shape(const mpxkkt_problem* pr, int flags) {
   ts.waves = ts.sh ? 8 : kTrialWaves;
   ts.mma = ts.sh ? mma_tiles<T>(pr, flags) : 0;
   ts.mma_wide = ts.mma && mma_rows16<T>(pr);
+  ts.hq = false;
   ts.lds = (ts.sh ? shared : (size_t)kTrialWaves * trial_wave_elems(pr->n, pr->m)) * sizeof(T);
   ts.grid = dim3((N1 + ts.waves - 1) / ts.waves, pr->batch);
   return ts;
@@ -673,20 +675,26 @@ mpxkkt_status launch_trials_k(hipStream_t st, const TrialShape& ts, const Prob<T
     if (ts.mma) {
       // stages 0 .. N - 1 on the matrix cores, the terminal stage by one wave of k_trials
       const int N = P.N;
-      const dim3 grid((N + 15) / 16, P.B);
+      const dim3 grid((N + 63) / 64, P.B);
       // 128-bit loads when the iterate and the step are 16-byte aligned too
       const bool wide = ts.mma_wide && al16(I.X) && al16(I.U) && al16(I.V) && al16(W.dx) && al16(W.du) &&
                         al16(W.dv);
-      auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(64), 0, st, P, R, I, W, j0, j1); };
-      if (wide) {
-        if (ts.mma == 1) go(k_trials_mma<1, true>);
-        else if (ts.mma == 2) go(k_trials_mma<2, true>);
-        else go(k_trials_mma<3, true>);
-      } else {
-        if (ts.mma == 1) go(k_trials_mma<1, false>);
-        else if (ts.mma == 2) go(k_trials_mma<2, false>);
-        else go(k_trials_mma<3, false>);
-      }
+      auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, P, R, I, W, j0, j1); };
+      auto forms = [&](auto HQ) {
+        constexpr bool hq = decltype(HQ)::value;
+        if (wide) {
+          if (ts.mma == 1) go(k_trials_mma<1, true, hq>);
+          else if (ts.mma == 2) go(k_trials_mma<2, true, hq>);
+          else go(k_trials_mma<3, true, hq>);
+        } else {
+          if (ts.mma == 1) go(k_trials_mma<1, false, hq>);
+          else if (ts.mma == 2) go(k_trials_mma<2, false, hq>);
+          else go(k_trials_mma<3, false, hq>);
+        }
+      };
+      // after k_linearize_mma and k_adjoint_tanh the equality rows are in hq
+      if (ts.hq) forms(std::true_type{});
+      else forms(std::false_type{});
       if ((s = launch_check("k_trials_mma")) != MPXKKT_OK) return s;
       const size_t lds = (size_t)trial_wave_elems(P.n, P.m) * sizeof(T);
       if ((s = prep_lds(k_trials<T, RPL, V4, false>, lds)) != MPXKKT_OK) return s;
@@ -752,7 +760,8 @@ mpxkkt_status run_linesearch(hipStream_t st, const mpxkkt_problem* pr, const Pro
     const size_t setup_lds = (size_t)6 * 256 * sizeof(T);
     hipLaunchKernelGGL(k_ls_setup<T>, dim3(B), dim3(256), setup_lds, st, P, R, I, W);
     if ((s = launch_check("k_ls_setup")) != MPXKKT_OK) return s;
-    const TrialShape ts = trial_shape<T>(pr, R.flags);
+    TrialShape ts = trial_shape<T>(pr, R.flags);
+    ts.hq = ts.mma && fx_in_lqr<T>(pr, R);
     if ((s = run_trials<T>(st, pr, ts, P, R, I, W, 0, R.group)) != MPXKKT_OK) return s;
     if (R.group < R.max_trials && (s = run_trials<T>(st, pr, ts, P, R, I, W, R.group, R.max_trials)) != MPXKKT_OK)
       return s;
