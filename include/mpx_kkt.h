@@ -129,10 +129,12 @@ typedef struct {
                              instance) instead of the stage-parallel one;
                              bit 3: linearisation parallel over stages (lower
                              latency for small batches; n, m, n_eq <= 64);
-                             bit 4: line-search trials with one row per lane
-                             instead of the matrix-core form (FP32, tanh model
-                             with stage-invariant A, W, B, Q, R, n <= 48,
-                             m <= 16, n_eq <= 16) */
+                             bit 4: the row-per-lane linearisation and
+                             line-search trials and the S-based forward pass
+                             and costates instead of the matrix-core and
+                             adjoint forms (FP32, tanh model with
+                             stage-invariant A, W, B, Q, R, n <= 48, m <= 16,
+                             n_eq <= 16) */
   double hess_floor;      /* pivot floor of the m x m Cholesky factorisations
                              (H_t = R + B'S B, ...): pivot j is raised to
                              hess_floor * |H_jj| when it falls below; keeps
