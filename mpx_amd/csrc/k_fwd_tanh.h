@@ -3,8 +3,8 @@
 //   du_t = -(K_t dx_t + k_t),
 //   dx_{t+1} = A_t dx_t + B du_t + c_{t+1},   A_t = I + h (A + kappa diag(sd_t) W)
 // with sd_t = sech^2(W x_t) from k_linearize_mma and A, W, B stage-invariant.
-// A_t is never read: A_t dx = (I + h A) dx + sd_t (h kappa W dx), with I + h A,
-// h kappa W and h B held in registers for the whole chain, so a stage reads
+// A_t is never read: A_t dx = (I + h A) dx + sd_t (h kappa W dx), with I + h A
+// and h B in registers and h kappa W in LDS for the whole chain, so a stage reads
 // only K_t, k_t, c_{t+1} and sd_t (about 1.7 KiB at n = 36, m = 12 instead of
 // 7 KiB with A_t).  Lane (g, c) holds rows 16 i + c, columns 16 k + 4 g + s of
 // the matrices (A-operand tiles) and dx's entries 16 k + 4 g + s; a product is
@@ -50,8 +50,12 @@ __device__ __forceinline__ float lane_read(float x, int src) {
   return __int_as_float(__builtin_amdgcn_ds_bpermute(src * 4, __float_as_int(x)));
 }
 
+// (h kappa W stays in LDS, each lane reading back only its own entries, so
+// that the kernel fits 128 VGPRs: four waves per SIMD, the whole batch of
+// 4096 instances resident at once on 1024 SIMDs)
 template <int NT, bool WIDE, int R>
-__global__ void __launch_bounds__(64) k_fwd_tanh(Prob<float> P, Ws<float> ws) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WIDE ? 4 : 3))) k_fwd_tanh(Prob<float> P, Ws<float> ws) {
+  __shared__ f4 wl[NT * NT * 64];
   const int b = blockIdx.x;
   if (skip_instance(ws, b)) return;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
@@ -86,13 +90,15 @@ __global__ void __launch_bounds__(64) k_fwd_tanh(Prob<float> P, Ws<float> ws) {
 #pragma unroll
   for (int i = 0; i < NT; ++i)
 #pragma unroll
-    for (int k = 0; k < NT; ++k)
+    for (int k = 0; k < NT; ++k) {
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         const int row = 16 * i + c, col = 16 * k + 4 * g + s;
         Ah[i][k][s] = (row == col && row < n ? 1.f : 0.f) + hs * Ah[i][k][s];
         Wh[i][k][s] = hk * Wh[i][k][s];
       }
+      wl[(i * NT + k) * 64 + lane] = Wh[i][k];
+    }
 #pragma unroll
   for (int i = 0; i < NT; ++i)
 #pragma unroll
@@ -122,12 +128,14 @@ __global__ void __launch_bounds__(64) k_fwd_tanh(Prob<float> P, Ws<float> ws) {
     for (int i = 0; i < NT; ++i) {
       float pa = 0.f, pw = 0.f;
 #pragma unroll
-      for (int k = 0; k < NT; ++k)
+      for (int k = 0; k < NT; ++k) {
+        const f4 w = wl[(i * NT + k) * 64 + lane];
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
           pa += Ah[i][k][s] * xs[k][s];
-          pw += Wh[i][k][s] * xs[k][s];
+          pw += w[s] * xs[k][s];
         }
+      }
 #pragma unroll
       for (int s = 0; s < 4; ++s) pa += Bh[i][s] * dug[s];
       y[i] = col_sum4(pa) + d.sd[i] * col_sum4(pw) + d.cn[i];

@@ -441,11 +441,11 @@ mpxkkt_status run_fwd_stage(hipStream_t st, const mpxkkt_problem* pr, const Prob
         if (wide) {
           if (nf == 1) go(k_fwd_tanh<1, true, 4>);
           else if (nf == 2) go(k_fwd_tanh<2, true, 4>);
-          else go(k_fwd_tanh<3, true, 4>);
+          else go(k_fwd_tanh<3, true, 2>);
         } else {
           if (nf == 1) go(k_fwd_tanh<1, false, 4>);
           else if (nf == 2) go(k_fwd_tanh<2, false, 4>);
-          else go(k_fwd_tanh<3, false, 4>);
+          else go(k_fwd_tanh<3, false, 2>);
         }
         mpxkkt_status s = launch_check("k_fwd_tanh");
         if (s != MPXKKT_OK) return s;
