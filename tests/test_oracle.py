@@ -45,6 +45,43 @@ def test_riccati_matches_dense_kkt(N, n, m):
         assert _rel(r[k], d[k]) < 1e-10, k
 
 
+@pytest.mark.parametrize("wl,shape", [
+    (3, dict(n=36, m=12, N=50, p=8)),   # the benchmark shape: equality rows
+    (2, dict(n=8, m=4, N=12)),          # boxes (R with the barrier diagonal)
+    (4, dict(n=12, m=5, N=9, p=4)),     # boxes and equality rows
+])
+def test_adjoint_costates_equal_riccati(wl, shape):
+    """The costates of the tanh model's FP32 wave path come from the adjoint
+    recursion lam_N = Q_N dx_N + q_N, lam_t = Q_t dx_t + M_t du_t + q_t +
+    A_t' lam_{t+1} (stationarity in dx_t of the LQR subproblem; with Q_t =
+    Q + rho_c D'D, M_t = rho_c D'E as the kernel forms them) instead of
+    lam = S dx + s: both must be the dense KKT system's multipliers."""
+    d = syn.make_instance(wl, 7, **shape)
+    X, U, V, Yeq, Ybox = syn.initial_iterate(d, "rollout")
+    U = orc.box_interior(d, U)
+    if orc.has_box(d):
+        _, Ybox = orc.box_duals(d, U, Ybox, 1e-2)
+    rho_c = 10.0
+    L = orc.linearize(d, X, U, Yeq, Ybox, rho_c)
+    sol = orc.lqr_riccati(L)
+    dx, du, lam = sol["dx"], sol["du"], sol["lam"]
+    N = d["N"]
+    la = np.zeros_like(lam)
+    la[N] = L["QN"] @ dx[N] + L["qN"]
+    for t in range(N - 1, -1, -1):
+        Qt, Mt = orc.Q_at(d, t), np.zeros((d["n"], d["m"]))
+        if d["p"] > 0:   # the kernel's split: problem Q plus the equality rows' terms
+            D, E = d["D"][t], d["E"][t]
+            Qt = Qt + rho_c * D.T @ D
+            Mt = rho_c * D.T @ E
+        assert np.allclose(Qt, L["Q"][t]) and np.allclose(Mt, L["M"][t])
+        la[t] = Qt @ dx[t] + Mt @ du[t] + L["q"][t] + L["A"][t].T @ la[t + 1]
+    assert np.max(np.abs(la - lam)) <= 1e-10 * max(1.0, np.max(np.abs(lam)))
+    if N * (d["n"] + d["m"]) <= 400:   # and the dense solve's multipliers
+        dn = orc.lqr_dense(L)
+        assert np.max(np.abs(la - dn["lam"])) <= 1e-8 * max(1.0, np.max(np.abs(dn["lam"])))
+
+
 @pytest.mark.parametrize("P", [1, 2, 3, 4, 8, 64])
 def test_chunked_scan_matches_riccati(P):
     rng = np.random.default_rng(7)
