@@ -317,24 +317,67 @@ k_linearize_mma(Prob<float> P, Prm<float> prm, Iter<float> it, Ws<float> ws, int
 
 // The terminal stage (the cost of x_N, q_N), the initial-state row
 // c_init = x0 - X_0 and the merit parts of both, and f_u = h B of the tanh
-// model: one wave per instance, the arithmetic of k_linearize_inst.
-template <typename T>
-__global__ void __launch_bounds__(64) k_linearize_term(Prob<T> P, Iter<T> it, Ws<T> ws) {
-  __shared__ T scr[64];
-  const int b = blockIdx.x, lane = threadIdx.x;
+// model: one wave per instance, the arithmetic of terminal_terms (k_inst.h)
+// with Q_N (x_N - x_ref,N) as A-operand tiles of Q_N (rows 16 i + c, columns
+// 16 k + 4 g + s) times the entries 16 k + 4 g + s of the difference: 4 NT^2
+// products per lane and a sum over the four lanes of a column.
+template <int NT, bool WIDE>
+__global__ void __launch_bounds__(64) k_linearize_term(Prob<float> P, Iter<float> it, Ws<float> ws) {
+  const int b = blockIdx.x, lane = threadIdx.x, g = lane >> 4, c = lane & 15;
   if (skip_instance(ws, b)) return;
   const int n = P.n, m = P.m, N = P.N, N1 = N + 1;
-  const T* Xb = it.X + (size_t)b * N1 * n;
-  T o[3];
-  terminal_terms<T, true>(P, ws, b, Xb + (size_t)N * n, Xb, it.V + (size_t)b * N1 * n, scr, o);
-  if (lane == 0) {
-    T* mp = ws.mpart + ((size_t)b * N1 + N) * NMP;
-    mp[MP_COST] = o[0];
-    mp[MP_VC] = o[1];
-    mp[MP_CC] = o[2];
+  int off[NT][4];
+  aop_offsets<NT>(off, n, g, c);
+  f4 QN[NT][NT], xc[NT], rc[NT];
+  load_aop_raw<NT, NT, WIDE>(QN, P.QN.at(b, 0), n, n, off);
+  const rsrc_t rX = make_rsrc(it.X + (size_t)b * N1 * n, N1 * n);
+  const rsrc_t rxr = make_rsrc(P.xref.at(b, N), n);
+  const rsrc_t rx0 = make_rsrc(P.x0.at(b, 0), n), rV = make_rsrc(it.V + (size_t)b * N1 * n, n);
+#pragma unroll
+  for (int k = 0; k < NT; ++k) {
+    xc[k] = vload4<WIDE>(rX, N * n + 16 * k + 4 * g, 16 * k + 4 * g, n);
+    rc[k] = vload4<WIDE>(rxr, 16 * k + 4 * g, 16 * k + 4 * g, n);
   }
-  if (P.model == 1)
-    for (int e = lane; e < n * m; e += 64) ws.Blin[(size_t)b * n * m + e] = P.h * P.Bm.at(b, 0)[e];
+  float dr[NT], x0r[NT], X0r[NT], V0r[NT];
+#pragma unroll
+  for (int i = 0; i < NT; ++i) {
+    const int row = 16 * i + c, o = row < n ? row * 4 : kOOB;
+    dr[i] = rload(rX, row < n ? (N * n + row) * 4 : kOOB) - rload(rxr, o);
+    x0r[i] = rload(rx0, o);
+    X0r[i] = rload(rX, o);
+    V0r[i] = rload(rV, o);
+  }
+  const rsrc_t rqh = make_rsrc(ws.qh + (size_t)b * N1 * n, N1 * n);
+  const rsrc_t rcd = make_rsrc(ws.cdef + (size_t)b * N1 * n, n);
+  float cost = 0.f, vc = 0.f, cc = 0.f;
+#pragma unroll
+  for (int i = 0; i < NT; ++i) {
+    float pa = 0.f;
+#pragma unroll
+    for (int k = 0; k < NT; ++k)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) pa += QN[i][k][s] * (xc[k][s] - rc[k][s]);
+    const float q = col_sum4(pa);
+    const int row = 16 * i + c;
+    const bool ok = g == 0 && row < n;
+    const float c0 = x0r[i] - X0r[i];
+    cost += ok ? 0.5f * dr[i] * q : 0.f;
+    vc += ok ? V0r[i] * c0 : 0.f;
+    cc += ok ? c0 * c0 : 0.f;
+    rstore(rqh, ok ? (N * n + row) * 4 : kOOB, q);
+    rstore(rcd, ok ? row * 4 : kOOB, c0);
+  }
+  cost = wave_sum(cost);
+  vc = wave_sum(vc);
+  cc = wave_sum(cc);
+  if (lane == 0) {
+    float* mp = ws.mpart + ((size_t)b * N1 + N) * NMP;
+    mp[MP_COST] = cost;
+    mp[MP_VC] = vc;
+    mp[MP_CC] = cc;
+  }
+  const rsrc_t rB = make_rsrc(P.Bm.at(b, 0), n * m), rBl = make_rsrc(ws.Blin + (size_t)b * n * m, n * m);
+  for (int e = lane; e < n * m; e += 64) rstore(rBl, e * 4, P.h * rload(rB, e * 4));
 }
 
 }  // namespace mpxk

@@ -540,7 +540,16 @@ mpxkkt_status run_linearize(hipStream_t st, const mpxkkt_problem* pr, const Prob
       }
       mpxkkt_status s = launch_check("k_linearize_mma");
       if (s != MPXKKT_OK) return s;
-      hipLaunchKernelGGL(k_linearize_term<T>, dim3(pr->batch), dim3(64), 0, st, P, I, W);
+      auto got = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(pr->batch), dim3(64), 0, st, P, I, W); };
+      if (wide) {
+        if (nt == 1) got(k_linearize_term<1, true>);
+        else if (nt == 2) got(k_linearize_term<2, true>);
+        else got(k_linearize_term<3, true>);
+      } else {
+        if (nt == 1) got(k_linearize_term<1, false>);
+        else if (nt == 2) got(k_linearize_term<2, false>);
+        else got(k_linearize_term<3, false>);
+      }
       return launch_check("k_linearize_term");
     }
   }
