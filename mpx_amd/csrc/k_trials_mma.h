@@ -104,7 +104,7 @@ __device__ __forceinline__ f4 lds_aop(const float* sm, int i, int k, int g, int 
 // in hq already (k_linearize_mma, k_adjoint_tanh): the stage reads 2 p values
 // instead of D_t and E_t.
 template <int NT, bool WIDE, bool HQ>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 1 || (HQ && NT == 2) ? 3 : 2)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 1 || (HQ && !(WIDE && NT == 3)) ? 3 : 2)))
 k_trials_mma(Prob<float> P, Prm<float> prm, Iter<float> it, Ws<float> ws, int j0, int j1) {
   using L = MmaLds<NT>;
   __shared__ __align__(16) float smA[L::NN], smW[L::NN], smQ[L::NN], smB[L::NM], smR[L::MM];
@@ -128,9 +128,9 @@ k_trials_mma(Prob<float> P, Prm<float> prm, Iter<float> it, Ws<float> ws, int j0
   constexpr int RQ = NT == 3 ? 2 : 4;
   const float hs = P.h;
 
-  // The loads go in three groups, each issued before the arithmetic of the
-  // one before it (pinned by scheduling barriers: the scheduler would sink
-  // each load to its use): (1) the stage vectors (B-operand tiles; entries
+  // The loads go in three groups, each issued ahead of the arithmetic that
+  // uses it (pinned by scheduling barriers: the scheduler would sink each
+  // load to its use): (1) the stage vectors (B-operand tiles; entries
   // past the vector's end read the next stage's or 0: they meet the zero
   // columns of the matrix tiles), x_{t+1} and dx_{t+1}
   const rsrc_t rX = make_rsrc(it.X + (size_t)b * N1 * n, N1 * n);
@@ -190,23 +190,6 @@ k_trials_mma(Prob<float> P, Prm<float> prm, Iter<float> it, Ws<float> ws, int j0
     }
     __builtin_amdgcn_sched_barrier(0);   // one row tile's LDS reads at a time
   }
-  // (3) V_{t+1} and its step, the box
-  f4 vn0[NT], vn1[NT], lo, hi;
-  {
-    const rsrc_t rV = make_rsrc(it.V + (size_t)b * N1 * n, N1 * n);
-    const rsrc_t rdV = make_rsrc(ws.dv + (size_t)b * N1 * n, N1 * n);
-#pragma unroll
-    for (int i = 0; i < NT; ++i) {
-      vn0[i] = vload4<WIDE>(rV, (tc + 1) * n + 16 * i + 4 * g, 0, BIG);
-      vn1[i] = vload4<WIDE>(rdV, (tc + 1) * n + 16 * i + 4 * g, 0, BIG);
-    }
-    const int mb = P.box ? m : 0;   // a problem without boxes: the loads go nowhere
-    const rsrc_t rlo = view_rsrc(P.ulo, b, N, m), rhi = view_rsrc(P.uhi, b, N, m);
-    lo = vload4<WIDE>(rlo, (int)(tc * P.ulo.st) + 4 * g, 4 * g, mb);
-    hi = vload4<WIDE>(rhi, (int)(tc * P.uhi.st) + 4 * g, 4 * g, mb);
-  }
-  __builtin_amdgcn_sched_barrier(0);
-
   // W x: the argument of the model's tanh (W's rows past n are 0)
   f4 w0[NT], w1[NT];
 #pragma unroll
@@ -272,6 +255,23 @@ k_trials_mma(Prob<float> P, Prm<float> prm, Iter<float> it, Ws<float> ws, int j0
       pq2 += U1[v] * r1[v];
     }
   }
+
+  // (3) V_{t+1} and its step, the box
+  f4 vn0[NT], vn1[NT], lo, hi;
+  {
+    const rsrc_t rV = make_rsrc(it.V + (size_t)b * N1 * n, N1 * n);
+    const rsrc_t rdV = make_rsrc(ws.dv + (size_t)b * N1 * n, N1 * n);
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+      vn0[i] = vload4<WIDE>(rV, (tc + 1) * n + 16 * i + 4 * g, 0, BIG);
+      vn1[i] = vload4<WIDE>(rdV, (tc + 1) * n + 16 * i + 4 * g, 0, BIG);
+    }
+    const int mb = P.box ? m : 0;   // a problem without boxes: the loads go nowhere
+    const rsrc_t rlo = view_rsrc(P.ulo, b, N, m), rhi = view_rsrc(P.uhi, b, N, m);
+    lo = vload4<WIDE>(rlo, (int)(tc * P.ulo.st) + 4 * g, 4 * g, mb);
+    hi = vload4<WIDE>(rhi, (int)(tc * P.uhi.st) + 4 * g, 4 * g, mb);
+  }
+  __builtin_amdgcn_sched_barrier(0);
 
   // equality rows h = e + D x + E u at the iterate and along the step, and
   // their part of phi, mu h + rho_c / 2 h^2, as a polynomial in a (every lane

@@ -232,6 +232,7 @@ def test_stage_parallel_line_search(wl, shape, iters, start, group, dtype):
     (2, dict(n=20, m=13, N=37), 3, "zero"),            # box, two tiles, 3 waves of 16 stages (the last partial)
     (3, dict(n=36, m=12, N=50, p=8), 2, None),         # the benchmark shape, equality rows
     (3, dict(n=47, m=16, N=17, p=16), 2, "zero"),      # three full tiles, 16 rows, N = 16 + 1
+    (4, dict(n=12, m=8, N=21, p=4), 3, None),          # boxes and equality rows with 128-bit loads
 ])
 def test_trials_matrix_core_form(wl, shape, iters, start, group):
     """FP32 linearisation and trials on the matrix cores (k_linearize_mma and
