@@ -134,7 +134,9 @@ typedef struct {
                              and costates instead of the matrix-core and
                              adjoint forms (FP32, tanh model with
                              stage-invariant A, W, B, Q, R, n <= 48, m <= 16,
-                             n_eq <= 16) */
+                             n_eq <= 16), and in the wave LQR kernel the
+                             homogeneous offset n on the matrix cores (when
+                             n % 4 == 0 it is otherwise added on the VALU) */
   double hess_floor;      /* pivot floor of the m x m Cholesky factorisations
                              (H_t = R + B'S B, ...): pivot j is raised to
                              hess_floor * |H_jj| when it falls below; keeps

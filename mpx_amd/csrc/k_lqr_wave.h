@@ -471,10 +471,19 @@ __device__ __forceinline__ void il_store_value(const f4 (&S)[NT][NT], const Ws<f
 // for the whole recursion instead of being re-read every stage.
 // KSN: the k steps of the last state tile, (n + 1 - 16 (NT - 1) + 3) / 4 or
 // more (steps past n + 1 add zeros).
-template <int NT, bool CQ, int KSN, bool FA>
+// AS (n % 4 == 0, KSN exact): the offset n, the homogeneous index of A~, S~ and B~, sits
+// alone in step KSN - 1 of the last state tile (lanes g == 0, the rest of the
+// step is padding).  On that step B~ has its zero row n and A~ its row
+// n = e_n', so the contractions over the state stop a step short (KSS steps)
+// and row n's part is added on the vector ALU: column n of S~ A~ gains column
+// n of S~, and row n of A~' (S~ A~) gains row n of S~ A~ (254 instead of 276
+// matrix instructions per stage at n = 36, m = 12, p = 8).
+template <int NT, bool CQ, int KSN, bool FA, bool AS>
 __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> prm, Iter<float> it_unused,
                                                     Ws<float> ws, LqrIn<float> in) {
   static_assert(!FA || CQ, "A~ is formed from stage-invariant A, W only with stage-invariant Q, B, R");
+  static_assert(!AS || CQ, "the homogeneous step is split off only with the step count of n");
+  constexpr int KSS = AS ? KSN - 1 : KSN;   // steps of the last state tile in a contraction over the state
   (void)it_unused;
   const int b = blockIdx.x;
   if (skip_instance(ws, b)) return;
@@ -559,7 +568,7 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
       f4 acc = zero4();
 #pragma unroll
       for (int k = 0; k < NT - 1; ++k) acc = mma_k<4>(S[k][i], Bt[k], acc);
-      SB[i] = mma_k<KSN>(S[NT - 1][i], Bt[NT - 1], acc);
+      SB[i] = mma_k<KSS>(S[NT - 1][i], Bt[NT - 1], acc);
     }
     f4 H;
 #pragma unroll
@@ -567,7 +576,7 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
       H[v] = fmaf(dmask[v], cur.rdv[v], CQ ? Rc[v] : cur.R[v] + (dmask[v] != 0.f && 4 * v + gf >= m ? 1.f : 0.f));
 #pragma unroll
     for (int k = 0; k < NT - 1; ++k) H = mma_k<4>(Bt[k], SB[k], H);
-    H = mma_k<KSN>(Bt[NT - 1], SB[NT - 1], H);
+    H = mma_k<KSS>(Bt[NT - 1], SB[NT - 1], H);
     // G~ = [0 | r] + rho_c E'[D | 0] (+ B~' S~ A~ below)
     f4 G[NT];
 #pragma unroll
@@ -612,8 +621,10 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
           }
         }
     }
+    if constexpr (!AS) {   // the corner 1 (with AS no matrix instruction reads row n of A~)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) cur.A[NT - 1][NT - 1][v] += homlane ? rown[v] : 0.f;
+      for (int v = 0; v < 4; ++v) cur.A[NT - 1][NT - 1][v] += homlane ? rown[v] : 0.f;
+    }
     f4 Pm[NT][NT];
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
@@ -623,11 +634,18 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
         f4 acc = zero4();
 #pragma unroll
         for (int k = 0; k < NT - 1; ++k) acc = mma_k<4>(S[k][i], cur.A[k][j], acc);
-        SAc[i] = mma_k<KSN>(S[NT - 1][i], cur.A[NT - 1][j], acc);
+        SAc[i] = mma_k<KSS>(S[NT - 1][i], cur.A[NT - 1][j], acc);
+        if constexpr (AS) {
+          if (j == NT - 1) {   // + S~[:, n] A~[n][n], on the lanes of column n
+            const float hm = homlane ? 1.f : 0.f;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) SAc[i][v] = fmaf(hm, S[i][NT - 1][v], SAc[i][v]);
+          }
+        }
       }
 #pragma unroll
       for (int k = 0; k < NT - 1; ++k) G[j] = mma_k<4>(Bt[k], SAc[k], G[j]);
-      G[j] = mma_k<KSN>(Bt[NT - 1], SAc[NT - 1], G[j]);
+      G[j] = mma_k<KSS>(Bt[NT - 1], SAc[NT - 1], G[j]);
 #pragma unroll
       for (int i = 0; i <= j; ++i) {
         f4 acc;
@@ -647,7 +665,14 @@ __global__ void __launch_bounds__(64, 2) k_lqr_wave(Prob<float> P, Prm<float> pr
         if (p > 0) acc = mma_ks(cur.D[i], cur.D[j], acc, ksp);
 #pragma unroll
         for (int k = 0; k < NT - 1; ++k) acc = mma_k<4>(cur.A[k][i], SAc[k], acc);
-        Pm[i][j] = mma_k<KSN>(cur.A[NT - 1][i], SAc[NT - 1], acc);
+        acc = mma_k<KSS>(cur.A[NT - 1][i], SAc[NT - 1], acc);
+        if constexpr (AS) {
+          if (i == NT - 1) {   // + A~[n][n] (S~ A~)[n, :]: row n = 4 (KSN - 1) + g, g = 0
+            constexpr int v = KSN - 1;
+            acc[v] = fmaf(rown[v], SAc[NT - 1][v], acc[v]);
+          }
+        }
+        Pm[i][j] = acc;
       }
     }
     // A~, D and q (or Q~ and B~) of this stage are dead

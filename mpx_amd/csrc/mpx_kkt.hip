@@ -588,30 +588,36 @@ mpxkkt_status run_lqr_wave(hipStream_t st, const mpxkkt_problem* pr, const Prob<
     auto go = [&](auto kern) { hipLaunchKernelGGL(kern, gb, dim3(64), 0, st, P, R, I, W, in); };
     if (cq) {
       // one instantiation per step count: the steps are unrolled matrix instructions
-      auto cases = [&](auto FA) {
-        constexpr bool fa = decltype(FA)::value;
+      auto cases = [&](auto FA, auto AS) {
+        constexpr bool fa = decltype(FA)::value, as = decltype(AS)::value;
         switch (NT * 4 + ksn) {
-          case 5: go(k_lqr_wave<1, true, 1, fa>); break;
-          case 6: go(k_lqr_wave<1, true, 2, fa>); break;
-          case 7: go(k_lqr_wave<1, true, 3, fa>); break;
-          case 8: go(k_lqr_wave<1, true, 4, fa>); break;
-          case 9: go(k_lqr_wave<2, true, 1, fa>); break;
-          case 10: go(k_lqr_wave<2, true, 2, fa>); break;
-          case 11: go(k_lqr_wave<2, true, 3, fa>); break;
-          case 12: go(k_lqr_wave<2, true, 4, fa>); break;
-          case 13: go(k_lqr_wave<3, true, 1, fa>); break;
-          case 14: go(k_lqr_wave<3, true, 2, fa>); break;
-          case 15: go(k_lqr_wave<3, true, 3, fa>); break;
-          default: go(k_lqr_wave<3, true, 4, fa>); break;
+          case 5: go(k_lqr_wave<1, true, 1, fa, as>); break;
+          case 6: go(k_lqr_wave<1, true, 2, fa, as>); break;
+          case 7: go(k_lqr_wave<1, true, 3, fa, as>); break;
+          case 8: go(k_lqr_wave<1, true, 4, fa, as>); break;
+          case 9: go(k_lqr_wave<2, true, 1, fa, as>); break;
+          case 10: go(k_lqr_wave<2, true, 2, fa, as>); break;
+          case 11: go(k_lqr_wave<2, true, 3, fa, as>); break;
+          case 12: go(k_lqr_wave<2, true, 4, fa, as>); break;
+          case 13: go(k_lqr_wave<3, true, 1, fa, as>); break;
+          case 14: go(k_lqr_wave<3, true, 2, fa, as>); break;
+          case 15: go(k_lqr_wave<3, true, 3, fa, as>); break;
+          default: go(k_lqr_wave<3, true, 4, fa, as>); break;
         }
       };
-      // the tanh model after k_linearize_mma: A~ formed from A, W and sd_t
-      if (fx_in_lqr<T>(pr, R)) cases(std::true_type{});
-      else cases(std::false_type{});
+      // n % 4 == 0: the homogeneous offset n alone in the last k step of the
+      // state (split off the contractions); the tanh model after
+      // k_linearize_mma: A~ formed from A, W and sd_t
+      const bool as = pr->n % 4 == 0 && !(R.flags & 16);
+      const bool fa = fx_in_lqr<T>(pr, R);
+      if (fa && as) cases(std::true_type{}, std::true_type{});
+      else if (fa) cases(std::true_type{}, std::false_type{});
+      else if (as) cases(std::false_type{}, std::true_type{});
+      else cases(std::false_type{}, std::false_type{});
     } else {   // stage-varying data: the generic step count (the steps past n + 1 add zeros)
-      if (NT == 1) go(k_lqr_wave<1, false, 4, false>);
-      else if (NT == 2) go(k_lqr_wave<2, false, 4, false>);
-      else go(k_lqr_wave<3, false, 4, false>);
+      if (NT == 1) go(k_lqr_wave<1, false, 4, false, false>);
+      else if (NT == 2) go(k_lqr_wave<2, false, 4, false, false>);
+      else go(k_lqr_wave<3, false, 4, false, false>);
     }
     return launch_check("k_lqr_wave");
   } else {

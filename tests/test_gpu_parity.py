@@ -187,6 +187,33 @@ def test_fp32_wave_kernel_step_counts(shape):
                 assert _rel(g[key][i], gb[key][i]) < tol, (k, i, key, _rel(g[key][i], gb[key][i]))
 
 
+@pytest.mark.parametrize("shape", [
+    dict(n=12, m=5, N=9, p=3),     # one tile: offset 12 alone in step 4 of 4
+    dict(n=20, m=13, N=8, p=11),   # two tiles: step 2 of 2
+    dict(n=24, m=7, N=8, p=6),     # two tiles: step 3 of 3
+    dict(n=32, m=9, N=7, p=4),     # three tiles: the last state tile is offset 32 alone (no step left)
+    dict(n=36, m=12, N=10, p=8),   # the benchmark's tiles: step 2 of 2
+])
+def test_fp32_wave_kernel_homogeneous_step(shape):
+    """n % 4 == 0: the wave LQR kernel leaves the homogeneous offset n out of
+    its contractions over the state and adds its part on the VALU, with A~
+    formed in the kernel (flags 0) and read from f_x (flags bit 3); flags bit 4
+    keeps the offset on the matrix cores.  All three against the FP64 oracle
+    and against each other."""
+    runs = {f: _run(3, 3, 2, torch.float32, shape=shape, flags=f) for f in (0, 8, 16)}
+    refs = runs[0][2]
+    for k in range(2):
+        for i in range(3):
+            r = refs[i][k]
+            for f, res in runs.items():
+                g, st = res[3][k]
+                assert st["status"][i] == 0, (f, k, i)
+                for key, tol in (("X", TOL32_X), ("U", 10 * TOL32_X), ("V", 10 * TOL32_X)):
+                    assert _rel(g[key][i], r[key]) < tol, (f, k, i, key, _rel(g[key][i], r[key]))
+                    g16 = runs[16][3][k][0]
+                    assert _rel(g[key][i], g16[key][i]) < tol, (f, k, i, key, _rel(g[key][i], g16[key][i]))
+
+
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 @pytest.mark.parametrize("group", [1, 3, 0])
 @pytest.mark.parametrize("wl,shape,iters,start", [
