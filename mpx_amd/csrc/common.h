@@ -355,26 +355,42 @@ struct MSeg {
 };
 
 // acc += sign * op(A)[i, :] op(B)[:, j] over one K-segment.  Operands of the
-// next four k-steps are loaded while the current four MFMAs issue.
+// next four k-steps are loaded while the current four MFMAs issue.  No operand
+// load runs under an execution mask: a row i >= M (column j >= N) reads row
+// M - 1 (column N - 1) instead, which only feeds output entries that are never
+// stored, and k is clamped to K - 1 with the entries past K replaced by 0
+// after the load (kmask).
 #define MPXK_MF_LOAD(K0, A0, A1, A2, A3, B0, B1, B2, B3)                       \
   do {                                                                         \
     const int k_ = (K0) + q;                                                   \
-    A0 = (iok && k_ < g.K) ? pa[k_ * sak] * g.sign : 0.f;                      \
-    B0 = (jok && k_ < g.K) ? pb[k_ * sbk] : 0.f;                               \
-    A1 = (iok && k_ + 4 < g.K) ? pa[(k_ + 4) * sak] * g.sign : 0.f;            \
-    B1 = (jok && k_ + 4 < g.K) ? pb[(k_ + 4) * sbk] : 0.f;                     \
-    A2 = (iok && k_ + 8 < g.K) ? pa[(k_ + 8) * sak] * g.sign : 0.f;            \
-    B2 = (jok && k_ + 8 < g.K) ? pb[(k_ + 8) * sbk] : 0.f;                     \
-    A3 = (iok && k_ + 12 < g.K) ? pa[(k_ + 12) * sak] * g.sign : 0.f;          \
-    B3 = (jok && k_ + 12 < g.K) ? pb[(k_ + 12) * sbk] : 0.f;                   \
+    const int k0c_ = k_ < g.K ? k_ : g.K - 1;                                  \
+    const int k1c_ = k_ + 4 < g.K ? k_ + 4 : g.K - 1;                          \
+    const int k2c_ = k_ + 8 < g.K ? k_ + 8 : g.K - 1;                          \
+    const int k3c_ = k_ + 12 < g.K ? k_ + 12 : g.K - 1;                        \
+    const float a0_ = pa[k0c_ * sak], b0_ = pb[k0c_ * sbk];                    \
+    const float a1_ = pa[k1c_ * sak], b1_ = pb[k1c_ * sbk];                    \
+    const float a2_ = pa[k2c_ * sak], b2_ = pb[k2c_ * sbk];                    \
+    const float a3_ = pa[k3c_ * sak], b3_ = pb[k3c_ * sbk];                    \
+    A0 = kmask(a0_ * g.sign, k_ - g.K);                                        \
+    B0 = kmask(b0_, k_ - g.K);                                                 \
+    A1 = kmask(a1_ * g.sign, k_ + 4 - g.K);                                    \
+    B1 = kmask(b1_, k_ + 4 - g.K);                                             \
+    A2 = kmask(a2_ * g.sign, k_ + 8 - g.K);                                    \
+    B2 = kmask(b2_, k_ + 8 - g.K);                                             \
+    A3 = kmask(a3_ * g.sign, k_ + 12 - g.K);                                   \
+    B3 = kmask(b3_, k_ + 12 - g.K);                                            \
   } while (0)
 
+// x where d < 0, else +0 (whatever x holds): the sign of d spread over a word
+// and ANDed into x's bits, on the vector ALU (no lane-mask registers)
+__device__ __forceinline__ float kmask(float x, int d) { return __int_as_float(__float_as_int(x) & (d >> 31)); }
+
 __device__ __forceinline__ void mfma_seg(mpxk_f32x4& acc, const MSeg& g, int i, int j, int M, int N, int q) {
-  const bool iok = i < M, jok = j < N;
+  if (g.K <= 0) return;
   const int sak = g.ta ? g.lda : 1, sai = g.ta ? 1 : g.lda;
   const int sbk = g.tb ? 1 : g.ldb, sbj = g.tb ? g.ldb : 1;
-  const float* pa = g.A + (iok ? i : 0) * sai;
-  const float* pb = g.B + (jok ? j : 0) * sbj;
+  const float* pa = g.A + (i < M ? i : M - 1) * sai;
+  const float* pb = g.B + (j < N ? j : N - 1) * sbj;
   float a0, a1, a2, a3, b0, b1, b2, b3;
   MPXK_MF_LOAD(0, a0, a1, a2, a3, b0, b1, b2, b3);
   for (int k0 = 0; k0 < g.K; k0 += 16) {
