@@ -429,41 +429,61 @@ __device__ __forceinline__ void mfma_store(const mpxk_f32x4& acc, const float* a
   }
 }
 
+// The addend of the lane's entries; an entry outside M x N reads the clamped
+// one instead (no execution mask) and is never stored.
 template <bool SYM>
 __device__ __forceinline__ void mfma_addend(float* add, int ti, int tj, int M, int N, int q, int r,
                                             const float* addend, int ld_add, const float* diag) {
-  const int cj = tj * 16 + r;
+  const int cj = tj * 16 + r, cjc = cj < N ? cj : N - 1;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    const int ci = ti * 16 + 4 * q + v;
-    add[v] = 0.f;
-    if (ci < M && cj < N) {
-      if (addend) add[v] = addend[ci * ld_add + cj];
-      if (diag && ci == cj) add[v] += diag[ci];
+    const int ci = ti * 16 + 4 * q + v, cic = ci < M ? ci : M - 1;
+    add[v] = addend ? addend[cic * ld_add + cjc] : 0.f;
+    if (diag) {
+      const float dv = diag[cic];
+      add[v] += ci == cj ? dv : 0.f;
     }
   }
 }
 
 // C = sum_s sign_s op(A_s) op(B_s) + addend + diag, 16x16 tiles taken by the
-// waves of the block round-robin.
+// waves w0.. of the block round-robin (C may be the addend itself: each entry
+// is read before its tile is stored, and a mirrored store lands in a tile
+// below the diagonal, which no tile reads).
 template <bool SYM>
 __device__ void mfma_gemm(int M, int N, int nseg, MSeg s0, MSeg s1, MSeg s2, float* C, int ldc,
-                          const float* addend, int ld_add, const float* diag) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+                          const float* addend, int ld_add, const float* diag, int w0 = 0) {
+  // the waves w0.. of the block take part
+  const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) - w0, nw = (blockDim.x >> 6) - w0;
+  if (wave < 0) return;
   const int tm = (M + 15) >> 4, tn = (N + 15) >> 4;
   const int ntile = SYM ? tn * (tn + 1) / 2 : tm * tn;
   const int r = lane & 15, q = lane >> 4;
-  for (int t0 = wave; t0 < ntile; t0 += nw) {
-    int ti, tj;
-    tile_of<SYM>(t0, tn, &ti, &tj);
-    float add[4];
+  // the next tile's addend is requested before this tile's products (it may
+  // sit in global memory)
+  float add[4] = {0.f, 0.f, 0.f, 0.f};
+  int ti = 0, tj = 0;
+  if (wave < ntile) {
+    tile_of<SYM>(wave, tn, &ti, &tj);
     mfma_addend<SYM>(add, ti, tj, M, N, q, r, addend, ld_add, diag);
+  }
+  for (int t0 = wave; t0 < ntile; t0 += nw) {
+    float addn[4] = {0.f, 0.f, 0.f, 0.f};
+    int tin = 0, tjn = 0;
+    if (t0 + nw < ntile) {
+      tile_of<SYM>(t0 + nw, tn, &tin, &tjn);
+      mfma_addend<SYM>(addn, tin, tjn, M, N, q, r, addend, ld_add, diag);
+    }
     mpxk_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     const int i = ti * 16 + r, j = tj * 16 + r;
     mfma_seg(acc, s0, i, j, M, N, q);
     if (nseg > 1) mfma_seg(acc, s1, i, j, M, N, q);
     if (nseg > 2) mfma_seg(acc, s2, i, j, M, N, q);
     mfma_store<SYM>(acc, add, ti, tj, M, N, q, r, C, ldc);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) add[v] = addn[v];
+    ti = tin;
+    tj = tjn;
   }
 }
 

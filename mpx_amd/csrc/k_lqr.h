@@ -137,10 +137,14 @@ struct IsF32<float> { static constexpr bool value = true; };
 // (chol_upper_inv_wave), then two products; other m: Cholesky and the two
 // triangular solves block-wide in LDS (the register version spills at those
 // template sizes inside the Riccati kernels).
-template <typename T, int MMAX>
-__device__ void spd_apply_inverse(T* H, T* G, T* Kb, int m, int ncol, int ld, T floor, bool mfma) {
+// (during_chol: run by the waves other than wave 0 while wave 0 factors H,
+// m <= 32; returns whether it did its work, else it is left to the caller)
+template <typename T, int MMAX, typename F>
+__device__ bool spd_apply_inverse(T* H, T* G, T* Kb, int m, int ncol, int ld, T floor, bool mfma, F&& during_chol) {
+  bool done = false;
   if constexpr (MMAX == 32) {
     chol_upper_inv_wave<T, MMAX>(H, H, m, floor);   // Y = U^{-1} (upper) in H
+    done = during_chol();
     lds_sync();
     if constexpr (IsF32<T>::value) {
       if (mfma) {
@@ -178,6 +182,7 @@ __device__ void spd_apply_inverse(T* H, T* G, T* Kb, int m, int ncol, int ld, T 
     lds_sync();
     trsm_upper_t(H, m, Kb, ncol, ld);                // K = L^{-T} W
   }
+  return done;
 }
 
 // One backward Riccati step in LDS: (S, s) = V_{t+1} -> V_t; [K | k] in Kb;
@@ -246,8 +251,21 @@ __device__ void riccati_step(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& w
     L.G[i * ld + n] = v;
   }
   lds_sync();
-  // C, D: [W | w] = Z [G | g] in G's place, [K | k] = Z' [W | w] in Kb
-  spd_apply_inverse<T, MMAX>(L.H, L.G, L.Kb, m, n + 1, ld, prm.hfloor, mfma);
+  // C, D: [W | w] = Z [G | g] in G's place, [K | k] = Z' [W | w] in Kb.
+  // While wave 0 factors H (m <= 32, FP32 products on the matrix cores), the
+  // other waves form P = Q + A'T1 + rho_c D'D in S's place (S itself is no
+  // longer read: T1 and Sc hold its products); S_t = P - W'W follows.
+  const bool pre = spd_apply_inverse<T, MMAX>(L.H, L.G, L.Kb, m, n + 1, ld, prm.hfloor, mfma, [&]() -> bool {
+    if constexpr (IsF32<T>::value) {
+      if (mfma && nt > 64) {
+        const MSeg s0{L.A, L.T1, ldab, ldab, n, true, false, 1.f};
+        const MSeg s2{L.D, L.D, n, n, p, true, false, 1.f};
+        mfma_gemm<true>(n, n, p > 0 ? 2 : 1, s0, s2, s2, L.S, n, Q, n, nullptr, 1);
+        return true;
+      }
+    }
+    return false;
+  });
   lds_sync();
   // E: S_t, s_t ; F_t = A - B K and f_t = c - B k (to the workspace)
   if constexpr (IsF32<T>::value) {
@@ -255,7 +273,8 @@ __device__ void riccati_step(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& w
       const MSeg s0{L.A, L.T1, ldab, ldab, n, true, false, 1.f};
       const MSeg s1{L.G, L.G, ld, ld, m, true, false, -1.f};
       const MSeg s2{L.D, L.D, n, n, p, true, false, 1.f};
-      mfma_gemm<true>(n, n, p > 0 ? 3 : 2, s0, s1, s2, L.S, n, Q, n, nullptr);
+      if (pre) mfma_gemm<true>(n, n, 1, s1, s1, s1, L.S, n, L.S, n, nullptr);
+      else mfma_gemm<true>(n, n, p > 0 ? 3 : 2, s0, s1, s2, L.S, n, Q, n, nullptr);
       const MSeg f0{L.B, L.Kb, ldab, ld, m, false, false, -1.f};
       mfma_gemm<false>(n, n, 1, f0, f0, f0, Fo, n, L.A, ldab, nullptr);
     }
