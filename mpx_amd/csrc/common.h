@@ -504,8 +504,10 @@ __device__ inline void wave_fence() {
 // readlane and updates the trailing columns in place (lane k's h[j] becomes
 // U[j][k]), and the back substitution solves U y = e_k reading U[i][q] from
 // lane q the same way; LDS is touched only to load H and store Y.
+// (not inlined: it gets registers of its own; inlined into the block
+// kernels, its readlane results spill with their scalar registers)
 template <typename T, int MM>
-__device__ void chol_upper_inv_wave(const T* H, T* Y, int m, T floor) {
+__device__ __attribute__((noinline)) void chol_upper_inv_wave(const T* H, T* Y, int m, T floor) {
   if (threadIdx.x >= 64) return;
   const int k = threadIdx.x;
   const bool own = k < m;
@@ -526,13 +528,20 @@ __device__ void chol_upper_inv_wave(const T* H, T* Y, int m, T floor) {
 #pragma unroll
     for (int i = j + 1; i < MM; ++i) h[i] -= rlane(lk, i) * lk;
   }
+  // row i of U reaches the lanes only once y[i + 1] is known (an opaque copy
+  // that takes y[i + 1] in): otherwise all rows' readlanes are hoisted ahead
+  // of the substitution and their scalar results spill
   T y[MM];
+  T ylast = T(0);
 #pragma unroll
   for (int i = MM - 1; i >= 0; --i) {
+    T hi = h[i];
+    asm volatile("" : "+v"(hi) : "v"(ylast));
     T v = k == i ? T(1) : T(0);
 #pragma unroll
-    for (int q = i + 1; q < MM; ++q) v -= rlane(h[i], q) * y[q];
-    y[i] = v / rlane(h[i], i);
+    for (int q = i + 1; q < MM; ++q) v -= rlane(hi, q) * y[q];
+    y[i] = v / rlane(hi, i);
+    ylast = y[i];
   }
   if (own) {
     // rows past m land on row m - 1 first and are overwritten by it
