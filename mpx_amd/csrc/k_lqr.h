@@ -66,9 +66,41 @@ __device__ RicSmem<T> ric_carve(T* base, int n, int m, int p, int nt, bool hasM 
   return L;
 }
 
+// e / ld for 0 <= e < 2^24: the float estimate is off by at most one
+__device__ __forceinline__ int div_small(int e, int ld, float inv) {
+  int i = (int)((float)e * inv);
+  if (i * ld > e) --i;
+  else if ((i + 1) * ld <= e) ++i;
+  return i;
+}
+
 // Load the stage's QP data into LDS: [A | B], sqrt(rho_c) D, sqrt(rho_c) E, c, q, r, rd.
-// Each thread issues its loads eight at a time before storing them, so a stage
-// costs a couple of memory latencies rather than one per element.
+// The small parts (D .. rd, four entries per thread) are requested first and
+// [A | B] twenty entries per thread at a time, every request of a round
+// issued before its stores: at n = 76, m = 32 with 512 threads a stage costs
+// one memory latency.
+template <typename T>
+__device__ __forceinline__ T stage_small(const T* D, const T* E, const T* c, const T* q, const T* r, const T* rd,
+                                         T sr, int e, int o2, int o3, int o4, int o5, int o6, int tot) {
+  if (e < o2) return sr * D[e];
+  if (e < o3) return sr * E[e - o2];
+  if (e < o4) return c[e - o3];
+  if (e < o5) return q[e - o4];
+  if (e < o6) return r[e - o5];
+  if (e < tot) return rd[e - o6];
+  return T(0);
+}
+template <typename T>
+__device__ __forceinline__ void stage_small_store(RicSmem<T>& L, int e, T v, int o2, int o3, int o4, int o5, int o6,
+                                                  int tot) {
+  if (e < o2) L.D[e] = v;
+  else if (e < o3) L.E[e - o2] = v;
+  else if (e < o4) L.c[e - o3] = v;
+  else if (e < o5) L.q[e - o4] = v;
+  else if (e < o6) L.r[e - o5] = v;
+  else if (e < tot) L.rd[e - o6] = v;
+}
+
 template <typename T>
 __device__ void load_stage(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& ws, const LqrIn<T>& in, int b,
                            int t, RicSmem<T>& L) {
@@ -83,37 +115,34 @@ __device__ void load_stage(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& ws,
   const T* q = ws.qh + ((size_t)b * N1 + t) * n;
   const T* r = ws.rh + ((size_t)b * P.N + t) * m;
   const T* rd = ws.rd + ((size_t)b * P.N + t) * m;
-  const int o1 = n * ldab, o2 = o1 + p * n, o3 = o2 + p * m, o4 = o3 + n, o5 = o4 + n, o6 = o5 + m;
-  const int tot = o6 + m;
-  for (int base = tid; base < tot; base += 8 * nt) {
-    T v[8];
+  const int o2 = p * n, o3 = o2 + p * m, o4 = o3 + n, o5 = o4 + n, o6 = o5 + m, tot = o6 + m;
+  T w[4];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+  for (int u = 0; u < 4; ++u) w[u] = stage_small(D, E, c, q, r, rd, sr, tid + u * nt, o2, o3, o4, o5, o6, tot);
+  const int o1 = n * ldab;
+  const float inv = 1.f / (float)ldab;
+  for (int base = tid; base < o1; base += 20 * nt) {
+    T v[20];
+#pragma unroll
+    for (int u = 0; u < 20; ++u) {
       const int e = base + u * nt;
       T x = 0;
       if (e < o1) {
-        const int i = e / ldab, j = e - i * ldab;
+        const int i = div_small(e, ldab, inv), j = e - i * ldab;
         x = j < n ? A[i * n + j] : B[i * m + (j - n)];
-      } else if (e < o2) x = sr * D[e - o1];
-      else if (e < o3) x = sr * E[e - o2];
-      else if (e < o4) x = c[e - o3];
-      else if (e < o5) x = q[e - o4];
-      else if (e < o6) x = r[e - o5];
-      else if (e < tot) x = rd[e - o6];
+      }
       v[u] = x;
     }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < 20; ++u) {
       const int e = base + u * nt;
       if (e < o1) L.A[e] = v[u];
-      else if (e < o2) L.D[e - o1] = v[u];
-      else if (e < o3) L.E[e - o2] = v[u];
-      else if (e < o4) L.c[e - o3] = v[u];
-      else if (e < o5) L.q[e - o4] = v[u];
-      else if (e < o6) L.r[e - o5] = v[u];
-      else if (e < tot) L.rd[e - o6] = v[u];
     }
   }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) stage_small_store(L, tid + u * nt, w[u], o2, o3, o4, o5, o6, tot);
+  for (int e = tid + 4 * nt; e < tot; e += nt)
+    stage_small_store(L, e, stage_small(D, E, c, q, r, rd, sr, e, o2, o3, o4, o5, o6, tot), o2, o3, o4, o5, o6, tot);
   if (L.Mt) {
     // M_t (n x m, row-major) lands transposed: Mt[r][i] = M[i][r]
     const T* M = P.M.at(b, t);
