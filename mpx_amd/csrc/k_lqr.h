@@ -214,6 +214,22 @@ __device__ bool spd_apply_inverse(T* H, T* G, T* Kb, int m, int ncol, int ld, T 
   return done;
 }
 
+// Outputs i < cnt of a vector product, each summed over k by a group of
+// four consecutive threads (k = s, s + 4, ... on lane s of the group) and
+// reduced by two lane exchanges; fin(i, sum) on the group's first lane.
+// Groups are aligned to four lanes, so a group is active or idle as a whole.
+template <typename T, typename Part, typename Fin>
+__device__ __forceinline__ void dot4(int cnt, Part&& part, Fin&& fin) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int base = tid; base < 4 * cnt; base += nt) {
+    const int i = base >> 2, sl = base & 3;
+    T v = part(i, sl);
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    if (sl == 0) fin(i, v);
+  }
+}
+
 // One backward Riccati step in LDS: (S, s) = V_{t+1} -> V_t; [K | k] in Kb;
 // F_t = A - B K and f_t = c - B k go straight to the workspace.
 //   [T1 | SB] = S [A | B],  Sc = S c + s
@@ -234,12 +250,11 @@ __device__ void riccati_step(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& w
   bool mfma = false;
   if constexpr (IsF32<T>::value) mfma = (prm.flags & 1) == 0;
   // A: Sc = S c + s ; [T1 | SB] = S [A | B]
-  for (int i = tid; i < n; i += nt) {
-    T v = L.s[i];
-#pragma unroll 8
-    for (int j = 0; j < n; ++j) v += L.S[i * n + j] * L.c[j];
-    L.Sc[i] = v;
-  }
+  dot4<T>(n, [&](int i, int sl) {
+    T v = 0;
+    for (int j = sl; j < n; j += 4) v += L.S[i * n + j] * L.c[j];
+    return v;
+  }, [&](int i, T v) { L.Sc[i] = L.s[i] + v; });
   if constexpr (IsF32<T>::value) {
     if (mfma) {
       const MSeg sg{L.S, L.A, n, ldab, n, false, false, 1.f};
@@ -273,12 +288,11 @@ __device__ void riccati_step(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& w
       L.G[i * ld + j] = v;
     }
   }
-  for (int i = tid; i < m; i += nt) {
-    T v = L.r[i];
-#pragma unroll 8
-    for (int k = 0; k < n; ++k) v += L.B[k * ldab + i] * L.Sc[k];
-    L.G[i * ld + n] = v;
-  }
+  dot4<T>(m, [&](int i, int sl) {
+    T v = 0;
+    for (int k = sl; k < n; k += 4) v += L.B[k * ldab + i] * L.Sc[k];
+    return v;
+  }, [&](int i, T v) { L.G[i * ld + n] = L.r[i] + v; });
   lds_sync();
   // C, D: [W | w] = Z [G | g] in G's place, [K | k] = Z' [W | w] in Kb.
   // While wave 0 factors H (m <= 32, FP32 products on the matrix cores), the
@@ -323,19 +337,17 @@ __device__ void riccati_step(const Prob<T>& P, const Prm<T>& prm, const Ws<T>& w
       L.S[j * n + i] = v;
     }
   }
-  for (int i = tid; i < n; i += nt) {
-    T v = L.q[i];
-#pragma unroll 8
-    for (int k = 0; k < n; ++k) v += L.A[k * ldab + i] * L.Sc[k];
-    T w = L.c[i];
-#pragma unroll 4
-    for (int r = 0; r < m; ++r) {
-      v -= L.G[r * ld + i] * L.G[r * ld + n];
-      w -= L.B[i * ldab + r] * L.Kb[r * ld + n];
-    }
-    L.s[i] = v;
-    fo[i] = w;
-  }
+  dot4<T>(n, [&](int i, int sl) {
+    T v = 0;
+    for (int k = sl; k < n; k += 4) v += L.A[k * ldab + i] * L.Sc[k];
+    for (int r = sl; r < m; r += 4) v -= L.G[r * ld + i] * L.G[r * ld + n];
+    return v;
+  }, [&](int i, T v) { L.s[i] = L.q[i] + v; });
+  dot4<T>(n, [&](int i, int sl) {
+    T w = 0;
+    for (int r = sl; r < m; r += 4) w -= L.B[i * ldab + r] * L.Kb[r * ld + n];
+    return w;
+  }, [&](int i, T w) { fo[i] = L.c[i] + w; });
   lds_sync();
 }
 
