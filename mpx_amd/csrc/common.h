@@ -499,11 +499,14 @@ __device__ inline void wave_fence() {
 // symmetric m x m matrix H (LDS, row-major, m <= MM <= 64) the upper Cholesky
 // factor U (H = U'U, pivots floored as in chol_lds) and Y = U^{-1} (upper
 // triangular, zero below), written to Y.
-// Lane k keeps column k in registers, padded to MM with the identity so that
-// no step needs a bound: elimination step j takes row j of U from the lanes by
-// readlane and updates the trailing columns in place (lane k's h[j] becomes
-// U[j][k]), and the back substitution solves U y = e_k reading U[i][q] from
-// lane q the same way; LDS is touched only to load H and store Y.
+// Lane k keeps column k of H in registers, padded to MM with the identity so
+// that no step needs a bound, and column k of Z = L^{-1} (L = U'), starting
+// as e_k.  Elimination step j forms row j of U (lane k's h[j] becomes
+// U[j][k]), takes its entries L[i][j] = U[j][i] from the lanes by readlane and
+// with each one updates the trailing column (H[i][k] -= L[i][j] U[j][k]) and
+// the forward substitution of Z (Z[i][k] -= L[i][j] Z[j][k], after Z[j][k] /=
+// L[j][j]); Y = Z' is written row k from lane k.  LDS is touched only to load
+// H and store Y.
 // (not inlined: it gets registers of its own; inlined into the block
 // kernels, its readlane results spill with their scalar registers)
 template <typename T, int MM>
@@ -512,41 +515,38 @@ __device__ __attribute__((noinline)) void chol_upper_inv_wave(const T* H, T* Y, 
   const int k = threadIdx.x;
   const bool own = k < m;
   const int kc = own ? k : 0;
-  T h[MM];
+  T h[MM], z[MM];
   T hkk = 0;   // the lane's original diagonal entry (pivot floors)
 #pragma unroll
   for (int i = 0; i < MM; ++i) {
     const T v = H[(i < m ? i : 0) * m + kc];
     h[i] = (own && i < m) ? v : T(i == k ? 1 : 0);
+    z[i] = T(i == k ? 1 : 0);
     if (i == k) hkk = h[i];
   }
 #pragma unroll
   for (int j = 0; j < MM; ++j) {
     const T d = dev_sqrt(floored_pivot(rlane(h[j], j), rlane(hkk, j), floor));
-    const T lk = k > j ? h[j] / d : T(0);
+    T lk = k > j ? h[j] / d : T(0);
     h[j] = k == j ? d : lk;
+    const T zj = z[j] / d;
+    z[j] = zj;
+    // the readlanes of lk start once zj is known (an opaque copy that takes
+    // zj in), so that each L[i][j] serves both updates at once instead of
+    // waiting in a scalar register, which spills
+    asm volatile("" : "+v"(lk) : "v"(zj));
 #pragma unroll
-    for (int i = j + 1; i < MM; ++i) h[i] -= rlane(lk, i) * lk;
+    for (int i = j + 1; i < MM; ++i) {
+      const T lij = rlane(lk, i);
+      h[i] -= lij * lk;
+      z[i] -= lij * zj;
+    }
   }
-  // row i of U reaches the lanes only once y[i + 1] is known (an opaque copy
-  // that takes y[i + 1] in): otherwise all rows' readlanes are hoisted ahead
-  // of the substitution and their scalar results spill
-  T y[MM];
-  T ylast = T(0);
-#pragma unroll
-  for (int i = MM - 1; i >= 0; --i) {
-    T hi = h[i];
-    asm volatile("" : "+v"(hi) : "v"(ylast));
-    T v = k == i ? T(1) : T(0);
-#pragma unroll
-    for (int q = i + 1; q < MM; ++q) v -= rlane(hi, q) * y[q];
-    y[i] = v / rlane(hi, i);
-    ylast = y[i];
-  }
+  wave_lds_sync();   // every lane has read H before Y (H itself in the block kernels) is written
   if (own) {
-    // rows past m land on row m - 1 first and are overwritten by it
 #pragma unroll
-    for (int i = MM - 1; i >= 0; --i) Y[(i < m ? i : m - 1) * m + k] = y[i];
+    for (int i = 0; i < MM; ++i)
+      if (i < m) Y[k * m + i] = z[i];
   }
   wave_lds_sync();
 }

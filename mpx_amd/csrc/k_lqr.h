@@ -360,9 +360,14 @@ __device__ void store_value_and_gain(const Prob<T>& P, const Ws<T>& ws, int b, i
   if (with_gain) {
     T* K = ws.K + ((size_t)b * P.N + t) * m * n;
     T* k = ws.k + ((size_t)b * P.N + t) * m;
-    for (int i = 0; i < m; ++i)
-      for (int j = threadIdx.x; j < n; j += blockDim.x) K[i * n + j] = L.Kb[i * ld + j];
-    for (int i = threadIdx.x; i < m; i += blockDim.x) k[i] = L.Kb[i * ld + n];
+    // [K | k] as one flat pass over the m x (n + 1) entries of Kb
+    const float inv = 1.f / (float)ld;
+    for (int e = threadIdx.x; e < m * ld; e += blockDim.x) {
+      const int i = div_small(e, ld, inv), j = e - i * ld;
+      const T v = L.Kb[e];
+      if (j < n) K[i * n + j] = v;
+      else k[i] = v;
+    }
   }
 }
 
