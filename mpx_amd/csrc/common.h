@@ -526,10 +526,19 @@ __device__ __attribute__((noinline)) void chol_upper_inv_wave(const T* H, T* Y, 
   }
 #pragma unroll
   for (int j = 0; j < MM; ++j) {
-    const T d = dev_sqrt(floored_pivot(rlane(h[j], j), rlane(hkk, j), floor));
-    T lk = k > j ? h[j] / d : T(0);
+    const T piv = floored_pivot(rlane(h[j], j), rlane(hkk, j), floor);
+    T d, lk, zj;
+    if constexpr (sizeof(T) == 4) {   // FP32: one reciprocal square root, no divisions
+      const T id = __builtin_amdgcn_rsqf(piv);
+      d = piv * id;
+      lk = k > j ? h[j] * id : T(0);
+      zj = z[j] * id;
+    } else {
+      d = dev_sqrt(piv);
+      lk = k > j ? h[j] / d : T(0);
+      zj = z[j] / d;
+    }
     h[j] = k == j ? d : lk;
-    const T zj = z[j] / d;
     z[j] = zj;
     // the readlanes of lk start once zj is known (an opaque copy that takes
     // zj in), so that each L[i][j] serves both updates at once instead of
