@@ -187,6 +187,28 @@ def test_fp32_wave_kernel_step_counts(shape):
                 assert _rel(g[key][i], gb[key][i]) < tol, (k, i, key, _rel(g[key][i], gb[key][i]))
 
 
+@pytest.mark.parametrize("threads", [64, 256, 512])
+@pytest.mark.parametrize("wl,shape", [
+    (4, dict(n=40, m=20, N=6, p=6)),   # 16 < m <= 32, boxes and equality rows
+    (2, dict(n=24, m=32, N=5)),        # m = 32, boxes, no equality rows
+])
+def test_block_kernel_block_sizes(wl, shape, threads):
+    """The LDS block Riccati kernel (flags bit 1) for 16 < m <= 32, where one
+    wave factors H in registers: with one wave per block (it then forms S_t in
+    one pass) and with several (the others form Q + A'T1 + rho_c D'D while it
+    factors H, and S_t = that - W'W follows), in FP32 and FP64 against the
+    FP64 oracle."""
+    for dtype, tol in ((torch.float32, TOL32_X), (torch.float64, 1e-9)):
+        insts, starts, refs, out, _, _ = _run(wl, 3, 2, dtype, shape=shape, flags=2, block_threads=threads)
+        for k in range(2):
+            g, st = out[k]
+            for i in range(3):
+                r = refs[i][k]
+                assert st["status"][i] == 0, (dtype, k, i)
+                for key, t in (("X", tol), ("U", 10 * tol), ("V", 10 * tol)):
+                    assert _rel(g[key][i], r[key]) < t, (dtype, k, i, key, _rel(g[key][i], r[key]))
+
+
 @pytest.mark.parametrize("shape", [
     dict(n=12, m=5, N=9, p=3),     # one tile: offset 12 alone in step 4 of 4
     dict(n=20, m=13, N=8, p=11),   # two tiles: step 2 of 2
