@@ -303,11 +303,32 @@ __global__ void k_count_running(int B, Ws<T> ws) {
 // in-place row shift is race free) with the NaN fallback.
 template <typename T>
 __device__ void shift_rows(T* Z, int rows, int cols, int shift) {
-  for (int j = threadIdx.x; j < cols; j += blockDim.x) {
-    for (int r = 0; r < rows; ++r) {
-      const int src = r + shift < rows ? r + shift : rows - 1;
-      Z[(size_t)r * cols + j] = Z[(size_t)src * cols + j];
+  // in place, in chunks of 8 entries per thread taken in increasing order:
+  // every thread reads its chunk entries' sources (rows r + shift >= r, none
+  // written yet) before the barrier and stores them after it
+  const int tot = rows * cols, tid = threadIdx.x, nt = blockDim.x;
+  const float inv = 1.f / (float)cols;
+  for (int base = 0; base < tot; base += 8 * nt) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = base + u * nt + tid;
+      v[u] = T(0);
+      if (e < tot) {
+        int r = (int)((float)e * inv);   // e / cols (e < 2^24), corrected by one
+        if (r * cols > e) --r;
+        else if ((r + 1) * cols <= e) ++r;
+        const int src = r + shift < rows ? r + shift : rows - 1;
+        v[u] = Z[(size_t)src * cols + (e - r * cols)];
+      }
     }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = base + u * nt + tid;
+      if (e < tot) Z[e] = v[u];
+    }
+    __syncthreads();
   }
 }
 

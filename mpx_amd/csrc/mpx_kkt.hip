@@ -1257,10 +1257,10 @@ mpxkkt_status mpxkkt_shift_warmstart(void* stream, const mpxkkt_problem* prob, i
   if (shift < 0 || shift > prob->horizon) return fail(MPXKKT_ERR_ARG, "shift must be in [0, horizon]");
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (prob->dtype == MPXKKT_F64) {
-    hipLaunchKernelGGL(k_shift<double>, dim3(prob->batch), dim3(64), 0, st, make_prob<double>(prob), shift,
+    hipLaunchKernelGGL(k_shift<double>, dim3(prob->batch), dim3(256), 0, st, make_prob<double>(prob), shift,
                        make_iter<double>(it), (const double*)u_reset, reset);
   } else {
-    hipLaunchKernelGGL(k_shift<float>, dim3(prob->batch), dim3(64), 0, st, make_prob<float>(prob), shift,
+    hipLaunchKernelGGL(k_shift<float>, dim3(prob->batch), dim3(256), 0, st, make_prob<float>(prob), shift,
                        make_iter<float>(it), (const float*)u_reset, reset);
   }
   return launch_check("k_shift");
